@@ -1,0 +1,117 @@
+"""Buffer access helpers and polyhedron construction (reference: fresnel/util.py)."""
+
+import io
+import itertools
+
+import numpy
+
+try:
+    import PIL.Image as PIL_Image
+except ImportError:  # pragma: no cover - PIL is optional
+    PIL_Image = None
+
+
+class Array(object):
+    """numpy-like view of a backend-owned buffer.
+
+    Reads return copies; writes copy into the buffer and, for geometry
+    buffers, notify the geometry so the next render rebuilds its acceleration
+    structure (reference: fresnel/util.py:15-70).
+
+    Attributes:
+        shape (tuple): dimensions of the array.
+        dtype: numpy data type.
+    """
+
+    def __init__(self, buf, geom):
+        self.buf = buf
+        self.geom = geom
+        self.buf.map()
+        a = numpy.asarray(self.buf)
+        self.shape = a.shape
+        self.dtype = a.dtype
+        self.buf.unmap()
+
+    def __setitem__(self, slice, data):
+        """Assign ``data`` to ``self[slice]``."""
+        self.buf.map()
+        a = numpy.asarray(self.buf)
+        a[slice] = data
+        self.buf.unmap()
+        if self.geom is not None:
+            self.geom._geometry.update()
+
+    def __getitem__(self, slice):
+        """Return a copy of ``self[slice]``."""
+        self.buf.map()
+        a = numpy.asarray(self.buf)
+        data = numpy.array(a[slice], copy=True)
+        self.buf.unmap()
+        return data
+
+
+class ImageArray(Array):
+    """An (H, W, 4) uint8 RGBA image buffer that Jupyter can display."""
+
+    def _repr_png_(self):
+        if PIL_Image is None:
+            raise RuntimeError("No PIL.Image module to format png")
+        self.buf.map()
+        f = io.BytesIO()
+        a = numpy.asarray(self.buf)
+        PIL_Image.fromarray(numpy.ascontiguousarray(a), mode='RGBA').save(f, 'png')
+        self.buf.unmap()
+        return f.getvalue()
+
+
+def convex_polyhedron_from_vertices(vertices):
+    """Build the plane description of the convex hull of ``vertices``.
+
+    The reference delegates to qhull (fresnel/util.py:98-126,
+    common/ConvexPolyhedronBuilder.cc:25-83).  This version finds hull faces
+    directly: every vertex triple spans a candidate plane, a candidate is a
+    face when no vertex lies outside it, and coplanar candidates merge into one
+    face.  It is exact for the small vertex counts polyhedra use.
+
+    Returns:
+        dict with ``face_origin`` (F,3), ``face_normal`` (F,3) outward unit
+        normals, ``face_color`` (F,3), ``face_sides`` (F,) vertex counts per
+        face and ``radius`` (circumsphere radius about the origin).
+    """
+    v = numpy.asarray(vertices, dtype=numpy.float64)
+    if v.ndim != 2 or v.shape[1] != 3 or v.shape[0] < 4:
+        raise ValueError("vertices must be an (N>=4, 3) array")
+    scale = numpy.max(numpy.abs(v))
+    tol = 1e-9 * max(scale, 1.0)
+    faces = []  # (normal, offset, vertex index set)
+    for i, j, k in itertools.combinations(range(v.shape[0]), 3):
+        n = numpy.cross(v[j] - v[i], v[k] - v[i])
+        norm = numpy.linalg.norm(n)
+        if norm <= tol * tol:
+            continue
+        n = n / norm
+        off = numpy.dot(n, v[i])
+        side = v @ n - off
+        if numpy.all(side <= tol):
+            pass
+        elif numpy.all(side >= -tol):
+            n, off, side = -n, -off, -side
+        else:
+            continue
+        on = frozenset(numpy.nonzero(numpy.abs(side) <= tol)[0].tolist())
+        if not any(numpy.allclose(n, f[0], atol=1e-9) and abs(off - f[1]) <= tol for f in faces):
+            faces.append((n, off, on))
+    origins = []
+    normals = []
+    sides = []
+    for n, off, on in faces:
+        pts = v[sorted(on)]
+        origins.append(numpy.mean(pts, axis=0))
+        normals.append(n)
+        sides.append(len(on))
+    radius = float(numpy.max(numpy.linalg.norm(v, axis=1)))
+    return dict(face_origin=numpy.array(origins, dtype=numpy.float32),
+                face_normal=numpy.array(normals, dtype=numpy.float32),
+                face_color=numpy.full((len(faces), 3), 0.9, dtype=numpy.float32),
+                face_sides=numpy.array(sides, dtype=numpy.uint32),
+                radius=radius)

@@ -1,0 +1,233 @@
+/* fresnel_hip.h -- C ABI of the MI355X render backend (libfresnel_hip.so).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never
+ * throws, and returns FH_OK (0) or a negative status; fh_last_error() then
+ * holds a message for the calling thread.  Handles are opaque.
+ *
+ * The entry points mirror the class set of the reference's `_gpu` binding
+ * module (fresnel/gpu/module-gpu.cc:22-43), which a pybind11 module
+ * (fresnel_amd/csrc/bindings/gpu_module.cpp) binds one-to-one.  Each
+ * declaration cites the reference interface it replaces.
+ *
+ * All calls are synchronous for the caller: a render returns once the output
+ * is complete on the device, as the reference's render() does
+ * (fresnel/cpu/TracerPath.cc:49-54, fresnel/gpu/TracerPath.cc:101).
+ */
+#ifndef FRESNEL_HIP_H
+#define FRESNEL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status */
+#define FH_OK 0
+#define FH_ERR_INVALID (-1)   /* bad argument (maps to RuntimeError) */
+#define FH_ERR_HIP (-2)       /* HIP runtime error */
+#define FH_ERR_STATE (-3)     /* object in the wrong state (e.g. removed geometry) */
+#define FH_ERR_NOMEM (-4)     /* device or host allocation failed / over budget */
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* fh_last_error(void);
+
+/* --------------------------------------------------------- plain data types
+ * Field order and meaning follow the reference's POD types exported by
+ * `_common` (fresnel/common/module-common.cc:37-115). */
+typedef struct { float r, g, b; } fh_rgb;                 /* RGB<float>, ColorMath.h:37-63 */
+typedef struct { float x, y, z; } fh_vec3;                /* vec3<float>, VectorMath.h:330 */
+typedef struct {                                           /* Material, Material.h:47-56 */
+    float solid;
+    fh_rgb color;
+    float primitive_color_mix;
+    float roughness;
+    float specular;
+    float metal;
+    float spec_trans;
+} fh_material;
+typedef struct {                                           /* UserCamera, Camera.h:55-77 */
+    fh_vec3 position;
+    fh_vec3 look_at;
+    fh_vec3 up;
+    float h;
+    float f;
+    float f_stop;
+    float focus_distance;
+    int32_t model;                                         /* 0 orthographic, 1 perspective */
+} fh_user_camera;
+typedef struct {                                           /* Lights, Light.h:36-45 */
+    fh_vec3 direction[4];
+    fh_rgb color[4];
+    float theta[4];
+    uint32_t N;
+} fh_lights;
+
+typedef struct fh_device_s* fh_device;
+typedef struct fh_scene_s* fh_scene;
+typedef struct fh_geometry_s* fh_geometry;
+typedef struct fh_tracer_s* fh_tracer;
+
+/* ------------------------------------------------------------------ device */
+/* get_num_available_devices(), fresnel/gpu/module-gpu.cc:22-25 */
+int fh_get_num_available_devices(int* n);
+/* Device(ptx_root, n), fresnel/gpu/Device.cc:29-54.  limit = -1: all GPUs.
+ * Only the first GPU is used: a Device drives one GPU. */
+int fh_device_create(int limit, fh_device* out);
+int fh_device_destroy(fh_device dev);
+/* Device.describe(), fresnel/gpu/Device.cc:114-118 */
+int fh_device_describe(fh_device dev, char* buf, size_t cap);
+/* Device.getAllGPUs(), fresnel/gpu/Device.cc:120-128 (newline-terminated lines) */
+int fh_device_get_all_gpus(char* buf, size_t cap);
+
+/* ------------------------------------------------------------------- scene */
+/* Scene(device), fresnel/gpu/Scene.cc (cpu/Scene.cc:15-27 defaults) */
+int fh_scene_create(fh_device dev, fh_scene* out);
+int fh_scene_destroy(fh_scene scene);
+/* set/getCamera, set/getBackgroundColor, set/getBackgroundAlpha,
+ * set/getLights: fresnel/gpu/Scene.cc:40-52 */
+int fh_scene_set_camera(fh_scene scene, const fh_user_camera* cam);
+int fh_scene_get_camera(fh_scene scene, fh_user_camera* cam);
+int fh_scene_set_background(fh_scene scene, fh_rgb color, float alpha);
+int fh_scene_get_background(fh_scene scene, fh_rgb* color, float* alpha);
+int fh_scene_set_lights(fh_scene scene, const fh_lights* lights);
+int fh_scene_get_lights(fh_scene scene, fh_lights* lights);
+
+/* ---------------------------------------------------------------- geometry */
+#define FH_GEOM_SPHERE 0
+#define FH_GEOM_CYLINDER 1
+#define FH_GEOM_POLYGON 2
+#define FH_GEOM_CONVEX_POLYHEDRON 3
+#define FH_GEOM_MESH 4
+
+/* GeometrySphere(scene, N), fresnel/gpu/GeometrySphere.cc */
+int fh_geometry_sphere_create(fh_scene scene, uint32_t N, fh_geometry* out);
+/* GeometryCylinder(scene, N), fresnel/gpu/GeometryCylinder.cc */
+int fh_geometry_cylinder_create(fh_scene scene, uint32_t N, fh_geometry* out);
+/* GeometryPolygon(scene, vertices[V,2], rounding_radius, N),
+ * fresnel/cpu/GeometryPolygon.cc:20-79 */
+int fh_geometry_polygon_create(fh_scene scene, const float* vertices, uint32_t n_verts,
+                               float rounding_radius, uint32_t N, fh_geometry* out);
+/* GeometryConvexPolyhedron(scene, origins[P,3], normals[P,3], colors[P,3], N, r),
+ * fresnel/cpu/GeometryConvexPolyhedron.cc:19-109 (normals are normalized here) */
+int fh_geometry_convex_polyhedron_create(fh_scene scene, const float* origins,
+                                         const float* normals, const float* colors,
+                                         uint32_t n_planes, uint32_t N, float radius,
+                                         fh_geometry* out);
+/* GeometryMesh(scene, vertices[3F,3], N), fresnel/cpu/GeometryMesh.cc:18-73 */
+int fh_geometry_mesh_create(fh_scene scene, const float* vertices, uint32_t n_verts,
+                            uint32_t N, fh_geometry* out);
+/* Releases the handle; the geometry leaves the scene (Geometry::~Geometry,
+ * fresnel/cpu/Geometry.cc:19-22). */
+int fh_geometry_destroy(fh_geometry geom);
+
+/* Buffer ids for fh_geometry_buffer */
+#define FH_BUF_POSITION 0     /* sphere/polyhedron/mesh (N,3); polygon (N,2) */
+#define FH_BUF_RADIUS 1       /* sphere/cylinder (N) */
+#define FH_BUF_COLOR 2        /* sphere/polyhedron/polygon (N,3); cylinder (N,2,3); mesh (3F,3) */
+#define FH_BUF_POINTS 3       /* cylinder (N,2,3) */
+#define FH_BUF_ORIENTATION 4  /* polyhedron/mesh (N,4) quaternion (s,x,y,z) */
+#define FH_BUF_ANGLE 5        /* polygon (N) */
+/* Host view of a geometry attribute buffer (getPositionBuffer() etc.,
+ * fresnel/cpu/Array.h:98-229).  The host copy is authoritative; it stays valid
+ * for the life of the geometry.  shape[0..ndim) receives the numpy shape. */
+int fh_geometry_buffer(fh_geometry geom, int which, float** data, uint32_t* ndim,
+                       uint64_t shape[3]);
+/* Geometry.update(): buffers changed, rebuild on the next render
+ * (fresnel/cpu/Geometry.h:91-94, fresnel/gpu/Geometry.h:85-88). */
+int fh_geometry_update(fh_geometry geom);
+/* get/setMaterial, get/setOutlineMaterial, get/setOutlineWidth, enable,
+ * disable, remove: fresnel/gpu/Geometry.cc:125-139 */
+int fh_geometry_set_material(fh_geometry geom, const fh_material* m);
+int fh_geometry_get_material(fh_geometry geom, fh_material* m);
+int fh_geometry_set_outline_material(fh_geometry geom, const fh_material* m);
+int fh_geometry_get_outline_material(fh_geometry geom, fh_material* m);
+int fh_geometry_set_outline_width(fh_geometry geom, float w);
+int fh_geometry_get_outline_width(fh_geometry geom, float* w);
+int fh_geometry_enable(fh_geometry geom);
+int fh_geometry_disable(fh_geometry geom);
+int fh_geometry_remove(fh_geometry geom);
+/* ConvexPolyhedron set/getColorByFace, fresnel/cpu/GeometryConvexPolyhedron.h:63-72 */
+int fh_geometry_set_color_by_face(fh_geometry geom, float f);
+int fh_geometry_get_color_by_face(fh_geometry geom, float* f);
+/* Polygon getRadius, fresnel/cpu/GeometryPolygon.h:56-59 */
+int fh_geometry_get_radius(fh_geometry geom, float* r);
+
+/* ----------------------------------------------------------------- tracers */
+/* TracerPath(device, w, h, light_samples), fresnel/cpu/TracerPath.cc:20-27 */
+int fh_tracer_path_create(fh_device dev, uint32_t w, uint32_t h, uint32_t light_samples,
+                          fh_tracer* out);
+/* TracerDirect(device, w, h), fresnel/cpu/TracerDirect.cc */
+int fh_tracer_direct_create(fh_device dev, uint32_t w, uint32_t h, fh_tracer* out);
+int fh_tracer_destroy(fh_tracer tr);
+/* Tracer.resize(w, h), fresnel/cpu/Tracer.cc:23-30 and cpu/TracerPath.h:48-53 */
+int fh_tracer_resize(fh_tracer tr, uint32_t w, uint32_t h);
+int fh_tracer_get_size(fh_tracer tr, uint32_t* w, uint32_t* h);
+/* Tracer.render(scene): one camera sample per pixel for a Path tracer
+ * (cpu/TracerPath.cc:56-256), the full image for a Direct tracer
+ * (cpu/TracerDirect.cc:33-225). */
+int fh_tracer_render(fh_tracer tr, fh_scene scene);
+/* `samples` consecutive Path render() calls in one submission; the result is
+ * bit-identical to calling fh_tracer_render `samples` times (tracer.py:249-255). */
+int fh_tracer_render_samples(fh_tracer tr, fh_scene scene, uint32_t samples);
+/* getSRGBOutputBuffer / getLinearOutputBuffer + map(): copies the device
+ * output into the tracer's host staging buffer and returns it
+ * (fresnel/gpu/Array.h:203-215).  which: 0 = sRGB uint8 (h,w,4),
+ * 1 = linear float32 (h,w,4). */
+#define FH_OUT_SRGB 0
+#define FH_OUT_LINEAR 1
+int fh_tracer_map_output(fh_tracer tr, int which, void** data);
+/* enable/disableHighlightWarning, get/setSeed: cpu/Tracer.h:57-87 */
+int fh_tracer_enable_highlight_warning(fh_tracer tr, fh_rgb color);
+int fh_tracer_disable_highlight_warning(fh_tracer tr);
+int fh_tracer_set_seed(fh_tracer tr, uint32_t seed);
+int fh_tracer_get_seed(fh_tracer tr, uint32_t* seed);
+/* TracerPath reset / setLightSamples / getNumSamples, cpu/TracerPath.cc:31-47,
+ * cpu/TracerPath.h:55-65 */
+int fh_tracer_path_reset(fh_tracer tr);
+int fh_tracer_path_set_light_samples(fh_tracer tr, uint32_t light_samples);
+int fh_tracer_path_get_num_samples(fh_tracer tr, uint32_t* n);
+/* TracerDirect set/getAntialiasingN, cpu/TracerDirect.h:33-42 */
+int fh_tracer_direct_set_aa_n(fh_tracer tr, uint32_t n);
+int fh_tracer_direct_get_aa_n(fh_tracer tr, uint32_t* n);
+
+/* ------------------------------------------------------ batching (no reference
+ * counterpart: the reference renders one (scene, tracer) pair per call).
+ * Renders `samples` passes for every (scenes[k], tracers[k]) pair in one
+ * submission: all pairs' rays share scene-tagged queues in one set of
+ * launches.  Each pair's output is bit-identical to fh_tracer_render_samples
+ * on that pair alone. */
+int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t samples);
+
+/* ------------------------------------------------------ diagnostics (no
+ * reference counterpart; used by the parity tests and the benchmark) */
+/* Philox4x32-10 on the device: in[6k..6k+5] = ctr0..3, key0, key1 -> out[4k..] */
+int fh_debug_philox(fh_device dev, const uint32_t* in, uint32_t n, uint32_t* out);
+/* Primary rays and closest hits of pass `sample` for every pixel of a Path
+ * tracer's image: rays[6p] = org.xyz, dir.xyz; geom_id/prim = 0xffffffff on a
+ * miss; t = +inf on a miss.  Any pointer may be NULL. */
+int fh_debug_primary(fh_tracer tr, fh_scene scene, uint32_t sample, float* rays,
+                     uint32_t* geom_id, uint32_t* prim, float* t);
+/* Closest hits for arbitrary rays (6 floats each) with tnear. */
+int fh_debug_trace(fh_scene scene, const float* rays, uint32_t n, float tnear,
+                   uint32_t* geom_id, uint32_t* prim, float* t);
+/* Per-kernel device time of the most recent fh_tracer_render_samples /
+ * fh_batch_render call, from HIP events recorded on the render stream.
+ * names: '\n'-separated kernel names; ms[k]: summed time; launches[k]: count. */
+int fh_stats_last(fh_device dev, char* names, size_t names_cap, double* ms,
+                  uint32_t* launches, uint32_t cap, uint32_t* n_kernels);
+/* Enable per-kernel event timing (adds one event pair per launch). */
+int fh_stats_enable(fh_device dev, int enable);
+/* Deterministic work counters of the most recent render: [0] rays traced,
+ * [1] BVH nodes visited, [2] primitive tests, [3] paths finished, [4] paths
+ * stopped at the depth cap.  Counting is enabled with fh_counters_enable and
+ * uses an instrumented traversal kernel. */
+int fh_counters_enable(fh_device dev, int enable);
+int fh_counters_last(fh_device dev, uint64_t out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRESNEL_HIP_H */
