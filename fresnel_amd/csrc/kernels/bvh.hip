@@ -1,0 +1,410 @@
+// bvh.hip -- linear BVH construction and refit on gfx950.
+//
+//   prim_bounds  per-primitive boxes (the reference bounds callbacks)
+//   centroid bounds reduction
+//   morton       30-bit Morton code of each box centroid
+//   radix sort   stable LSD sort of (code, index), 4 passes of 8 bits; ranks
+//                inside a block come from wavefront ballots and LDS counters
+//   karras       hierarchy emission over the sorted codes with the position
+//                as tie break (Karras, HPG 2012): every key is distinct, so the
+//                tree depth is at most 64
+//   refit        bottom-up box union with one atomic arrival counter per node
+#include "rt/isect.h"
+
+namespace fh
+{
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+
+// ---------------------------------------------------------------- bounds
+__global__ void k_prim_bounds(const GeomDev* __restrict__ geoms, const uint32_t* __restrict__ prefix,
+                              const uint32_t* __restrict__ slots, uint32_t n_active, uint32_t N, float4* lo_out,
+                              float4* hi_out)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    uint32_t lo = 0, hi = n_active;
+    while (hi - lo > 1)
+        {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+        }
+    const uint32_t slot = slots[lo];
+    const uint32_t local = i - prefix[lo];
+    v3 bl, bh;
+    prim_bounds(geoms[slot], local, bl, bh);
+    lo_out[i] = make_float4(bl.x, bl.y, bl.z, __uint_as_float(slot));
+    hi_out[i] = make_float4(bh.x, bh.y, bh.z, __uint_as_float(local));
+    }
+
+// Centroid bounds: per-block partials, then one block reduces the partials.
+__device__ __forceinline__ void block_minmax(float v[6], float* sh)
+    {
+    // v[0..2] minimum, v[3..5] maximum; sh holds 6 * blockDim floats
+    for (int k = 0; k < 6; k++)
+        sh[k * blockDim.x + threadIdx.x] = v[k];
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1)
+        {
+        if (threadIdx.x < s)
+            for (int k = 0; k < 6; k++)
+                {
+                const float a = sh[k * blockDim.x + threadIdx.x], b = sh[k * blockDim.x + threadIdx.x + s];
+                sh[k * blockDim.x + threadIdx.x] = (k < 3) ? fminf(a, b) : fmaxf(a, b);
+                }
+        __syncthreads();
+        }
+    for (int k = 0; k < 6; k++)
+        v[k] = sh[k * blockDim.x];
+    }
+
+__global__ void k_centroid_partial(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t N,
+                                   float* partial)
+    {
+    __shared__ float sh[6 * 256];
+    float v[6] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf(),
+                  -__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        {
+        const float4 a = lo[i], b = hi[i];
+        const float c[3] = {(a.x + b.x) * 0.5f, (a.y + b.y) * 0.5f, (a.z + b.z) * 0.5f};
+        for (int k = 0; k < 3; k++)
+            {
+            v[k] = fminf(v[k], c[k]);
+            v[3 + k] = fmaxf(v[3 + k], c[k]);
+            }
+        }
+    block_minmax(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++)
+            partial[6 * blockIdx.x + k] = v[k];
+    }
+
+__global__ void k_centroid_final(const float* partial, uint32_t n_part, float* out)
+    {
+    __shared__ float sh[6 * 256];
+    float v[6] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf(),
+                  -__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+    for (uint32_t i = threadIdx.x; i < n_part; i += blockDim.x)
+        for (int k = 0; k < 6; k++)
+            v[k] = (k < 3) ? fminf(v[k], partial[6 * i + k]) : fmaxf(v[k], partial[6 * i + k]);
+    block_minmax(v, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++)
+            out[k] = v[k];
+    }
+
+// ---------------------------------------------------------------- morton
+__device__ __forceinline__ uint32_t expand10(uint32_t x)
+    {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+    }
+__device__ __forceinline__ uint32_t quant10(float c, float lo, float hi)
+    {
+    const float e = hi - lo;
+    const float f = (e > 0.0f) ? (c - lo) / e : 0.5f;
+    const float s = fminf(fmaxf(f * 1024.0f, 0.0f), 1023.0f);
+    return (uint32_t)s;
+    }
+__global__ void k_morton(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t N,
+                         const float* __restrict__ cb, uint32_t* keys, uint32_t* vals)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const float4 a = lo[i], b = hi[i];
+    const uint32_t x = quant10((a.x + b.x) * 0.5f, cb[0], cb[3]);
+    const uint32_t y = quant10((a.y + b.y) * 0.5f, cb[1], cb[4]);
+    const uint32_t z = quant10((a.z + b.z) * 0.5f, cb[2], cb[5]);
+    keys[i] = (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+    vals[i] = i;
+    }
+
+// ---------------------------------------------------------------- radix sort
+__global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t N,
+                                                           uint32_t shift, uint32_t n_blocks, uint32_t* hist)
+    {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ITEMS; r++)
+        {
+        const uint32_t i = base + r * SORT_BLOCK + threadIdx.x;
+        if (i < N)
+            atomicAdd(&cnt[(keys[i] >> shift) & 0xffu], 1u);
+        }
+    __syncthreads();
+    hist[threadIdx.x * n_blocks + blockIdx.x] = cnt[threadIdx.x];
+    }
+
+// Exclusive scan of n uint32 in place: blocks of 2048 (256 x 8), then the
+// block sums (at most 2048) in one block, then the add-back.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* sh, uint32_t& total)
+    {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1)
+        {
+        const uint32_t t = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+        }
+    total = sh[255];
+    const uint32_t incl = sh[threadIdx.x];
+    __syncthreads();
+    return incl - v;
+    }
+__global__ __launch_bounds__(256) void k_scan_blocks(uint32_t* data, uint32_t n, uint32_t* sums)
+    {
+    __shared__ uint32_t sh[256];
+    const uint32_t base = blockIdx.x * 2048 + threadIdx.x * 8;
+    uint32_t v[8], s = 0;
+    for (int k = 0; k < 8; k++)
+        {
+        v[k] = (base + k < n) ? data[base + k] : 0u;
+        s += v[k];
+        }
+    uint32_t total;
+    uint32_t run = block_excl_scan256(s, sh, total);
+    for (int k = 0; k < 8; k++)
+        {
+        if (base + k < n)
+            data[base + k] = run;
+        run += v[k];
+        }
+    if (threadIdx.x == 0)
+        sums[blockIdx.x] = total;
+    }
+__global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t n)
+    {
+    __shared__ uint32_t sh[256];
+    const uint32_t base = threadIdx.x * 8;
+    uint32_t v[8], s = 0;
+    for (int k = 0; k < 8; k++)
+        {
+        v[k] = (base + k < n) ? sums[base + k] : 0u;
+        s += v[k];
+        }
+    uint32_t total;
+    uint32_t run = block_excl_scan256(s, sh, total);
+    for (int k = 0; k < 8; k++)
+        {
+        if (base + k < n)
+            sums[base + k] = run;
+        run += v[k];
+        }
+    }
+__global__ void k_scan_add(uint32_t* data, uint32_t n, const uint32_t* sums)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        data[i] += sums[i / 2048];
+    }
+
+__global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                              const uint32_t* __restrict__ vin, uint32_t* kout,
+                                                              uint32_t* vout, uint32_t N, uint32_t shift,
+                                                              uint32_t n_blocks, const uint32_t* __restrict__ offs)
+    {
+    constexpr int WAVES = SORT_BLOCK / 64;
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wcnt[WAVES * 256];
+    running[threadIdx.x] = offs[threadIdx.x * n_blocks + blockIdx.x];
+    for (int w = 0; w < WAVES; w++)
+        wcnt[w * 256 + threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * SORT_TILE;
+    for (int r = 0; r < SORT_ITEMS; r++)
+        {
+        const uint32_t i = base + r * SORT_BLOCK + threadIdx.x;
+        const bool valid = i < N;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t val = valid ? vin[i] : 0u;
+        const uint32_t digit = (key >> shift) & 0xffu;
+        // lanes holding the same digit
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+            {
+            const bool bit = (digit >> b) & 1u;
+            const uint64_t m = __ballot(valid && bit);
+            peers &= bit ? m : ~m;
+            }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && rank == 0)
+            wcnt[wave * 256 + digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid)
+            {
+            uint32_t pos = running[digit] + rank;
+            for (uint32_t w = 0; w < wave; w++)
+                pos += wcnt[w * 256 + digit];
+            kout[pos] = key;
+            vout[pos] = val;
+            }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < WAVES; w++)
+            {
+            add += wcnt[w * 256 + threadIdx.x];
+            wcnt[w * 256 + threadIdx.x] = 0;
+            }
+        running[threadIdx.x] += add;
+        __syncthreads();
+        }
+    }
+
+// ---------------------------------------------------------------- karras
+__device__ __forceinline__ int kdelta(const uint32_t* keys, int N, int i, int j)
+    {
+    if (j < 0 || j >= N)
+        return -1;
+    const uint64_t a = ((uint64_t)keys[i] << 32) | (uint32_t)i;
+    const uint64_t b = ((uint64_t)keys[j] << 32) | (uint32_t)j;
+    return __clzll(a ^ b);
+    }
+
+__device__ __forceinline__ void leaf_link(const uint32_t* vals, const float4* lo, const float4* hi, int p, uint32_t& c,
+                                          uint32_t& g)
+    {
+    const uint32_t orig = vals[p];
+    g = __float_as_uint(lo[orig].w);
+    c = LEAF | __float_as_uint(hi[orig].w);
+    }
+
+// Internal node i of N-1; parent[] holds (parent << 1 | side) for inner nodes
+// [0, N-1) and leaves [N-1, 2N-1).
+__global__ void k_karras(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                         const float4* __restrict__ lo, const float4* __restrict__ hi, int N, Node* nodes,
+                         uint32_t* parent)
+    {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N - 1)
+        return;
+    const int d = (kdelta(keys, N, i, i + 1) - kdelta(keys, N, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(keys, N, i, i - d);
+    int lmax = 2;
+    while (kdelta(keys, N, i, i + lmax * d) > dmin)
+        lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(keys, N, i, i + (l + t) * d) > dmin)
+            l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta(keys, N, i, j);
+    int s = 0;
+    for (int div = 2;; div <<= 1)
+        {
+        const int t = (l + div - 1) / div;
+        if (kdelta(keys, N, i, i + (s + t) * d) > dnode)
+            s += t;
+        if (t <= 1)
+            break;
+        }
+    const int gamma = i + s * d + (d < 0 ? d : 0);
+    uint32_t c0, c1, g0 = 0, g1 = 0;
+    const int first = i < j ? i : j, last = i < j ? j : i;
+    if (first == gamma)
+        {
+        leaf_link(vals, lo, hi, gamma, c0, g0);
+        parent[(N - 1) + gamma] = (uint32_t)(i << 1);
+        }
+    else
+        {
+        c0 = (uint32_t)gamma;
+        parent[gamma] = (uint32_t)(i << 1);
+        }
+    if (last == gamma + 1)
+        {
+        leaf_link(vals, lo, hi, gamma + 1, c1, g1);
+        parent[(N - 1) + gamma + 1] = (uint32_t)(i << 1) | 1u;
+        }
+    else
+        {
+        c1 = (uint32_t)(gamma + 1);
+        parent[gamma + 1] = (uint32_t)(i << 1) | 1u;
+        }
+    nodes[i].link = make_uint4(c0, c1, g0, g1);
+    }
+
+// One-primitive scene: root with the leaf and an EMPTY child whose NaN box
+// fails every slab test.
+__global__ void k_single_leaf(const float4* __restrict__ lo, const float4* __restrict__ hi, Node* nodes)
+    {
+    const float4 a = lo[0], b = hi[0];
+    const float nan = __builtin_nanf("");
+    nodes[0].a_lo = make_float4(a.x, a.y, a.z, b.x);
+    nodes[0].a_hi = make_float4(b.y, b.z, nan, nan);
+    nodes[0].b_box = make_float4(nan, nan, nan, nan);
+    nodes[0].link = make_uint4(LEAF | __float_as_uint(b.w), EMPTY, __float_as_uint(a.w), 0u);
+    }
+
+// ---------------------------------------------------------------- refit
+__device__ __forceinline__ void store_box(Node* n, uint32_t side, const float b[6])
+    {
+    float* f = reinterpret_cast<float*>(n);
+    const int o = side ? 6 : 0;
+    for (int k = 0; k < 6; k++)
+        __hip_atomic_store(f + o + k, b[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+__device__ __forceinline__ void load_box(Node* n, uint32_t side, float b[6])
+    {
+    float* f = reinterpret_cast<float*>(n);
+    const int o = side ? 6 : 0;
+    for (int k = 0; k < 6; k++)
+        b[k] = __hip_atomic_load(f + o + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+__global__ void k_refit(const uint32_t* __restrict__ vals, const float4* __restrict__ lo,
+                        const float4* __restrict__ hi, int N, Node* nodes, const uint32_t* __restrict__ parent,
+                        uint32_t* flags, float* root_box)
+    {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N)
+        return;
+    const uint32_t orig = vals[p];
+    const float4 a = lo[orig], b = hi[orig];
+    float box[6] = {a.x, a.y, a.z, b.x, b.y, b.z};
+    uint32_t par = parent[(N - 1) + p];
+    for (;;)
+        {
+        const uint32_t node = par >> 1;
+        store_box(nodes + node, par & 1u, box);
+        __threadfence();
+        const uint32_t prev = atomicAdd(flags + node, 1u);
+        if (prev == 0)
+            return; // the sibling finishes this node
+        __threadfence();
+        float b0[6], b1[6];
+        load_box(nodes + node, 0, b0);
+        load_box(nodes + node, 1, b1);
+        for (int k = 0; k < 3; k++)
+            {
+            box[k] = fminf(b0[k], b1[k]);
+            box[3 + k] = fmaxf(b0[3 + k], b1[3 + k]);
+            }
+        if (node == 0)
+            {
+            for (int k = 0; k < 6; k++)
+                root_box[k] = box[k];
+            return;
+            }
+        par = parent[node];
+        }
+    }
+    } // namespace fh

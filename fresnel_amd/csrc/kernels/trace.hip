@@ -1,0 +1,700 @@
+// trace.hip -- wavefront path tracer kernels for gfx950.
+//
+// One camera-sample batch of a Path render runs as
+//   camgen -> traverse -> shade_primary -> { traverse -> shade }* -> finish -> accumulate
+// over ping-pong SoA ray queues (float4 origin|tnear, float4 dir|job,
+// float4 attenuation, uint4 job|pixel|sample|depth).  shade appends the rays
+// that continue with one atomic per wavefront (ballot + popcount) and writes a
+// termination record per light-sample path; accumulate replays the records of
+// each pixel in (sample, light sample) order, which is the order the
+// reference adds them (fresnel/cpu/TracerPath.cc:141-239), so the image does
+// not depend on how paths were scheduled.
+#include "rt/isect.h"
+#include "rt/shade.h"
+
+namespace fh
+{
+// relative slack of the cull by the current closest hit (shared with the CPU checker)
+constexpr float CULL_SLACK = 1.0000152587890625f; // 1 + 2^-16
+constexpr uint32_t STACK_MAX = 64;               // >= depth of a Karras tree over 64-bit keys
+
+struct HitT
+    {
+    float t;
+    uint32_t geom;
+    uint32_t prim;
+    };
+
+__device__ __forceinline__ void slab(v3 o, v3 inv, float lx, float ly, float lz, float hx, float hy, float hz,
+                                     float& tmin, float& tmax)
+    {
+    const float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+    const float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+    const float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+    tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    }
+
+__device__ __forceinline__ bool better(float t, uint32_t g, uint32_t p, const HitT& b)
+    {
+    return (t < b.t) || (t == b.t && (g < b.geom || (g == b.geom && p < b.prim)));
+    }
+
+// Stack with the top SS entries in LDS (interleaved by thread) and the rest
+// in a per-thread global region.
+template<int SS> struct LdsStack
+    {
+    uint2* lds;
+    uint2* glob;
+    uint32_t lstride, gstride;
+    uint32_t sp;
+    __device__ __forceinline__ void push(uint32_t n, float t, uint32_t& err)
+        {
+        const uint2 e = make_uint2(n, __float_as_uint(t));
+        if (sp < SS)
+            lds[sp * lstride] = e;
+        else if (sp < STACK_MAX)
+            glob[(sp - SS) * gstride] = e;
+        else
+            {
+            err = 1;
+            return;
+            }
+        sp++;
+        }
+    __device__ __forceinline__ void pop(uint32_t& n, float& t)
+        {
+        sp--;
+        const uint2 e = (sp < SS) ? lds[sp * lstride] : glob[(sp - SS) * gstride];
+        n = e.x;
+        t = __uint_as_float(e.y);
+        }
+    };
+
+// Stack in private memory (for the preview and diagnostic kernels).
+struct LocalStack
+    {
+    uint2 s[STACK_MAX];
+    uint32_t sp;
+    __device__ __forceinline__ void push(uint32_t n, float t, uint32_t& err)
+        {
+        if (sp < STACK_MAX)
+            s[sp++] = make_uint2(n, __float_as_uint(t));
+        else
+            err = 1;
+        }
+    __device__ __forceinline__ void pop(uint32_t& n, float& t)
+        {
+        sp--;
+        n = s[sp].x;
+        t = __uint_as_float(s[sp].y);
+        }
+    };
+
+// Closest hit in the order (t, geometry, primitive) over the primitives whose
+// bounds pass the slab test with tmax >= tnear; a subtree is culled when its
+// entry distance exceeds the closest hit by the relative slack.
+template<class Stack, bool COUNT>
+__device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear, HitT& best, Stack& st,
+                                     uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
+    {
+    best.t = __builtin_huge_valf();
+    best.geom = EMPTY;
+    best.prim = EMPTY;
+    if (sc.root == EMPTY)
+        return;
+    const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    st.sp = 0;
+    uint32_t node = sc.root;
+    for (;;)
+        {
+        const Node* np = sc.nodes + node;
+        const float4 A = np->a_lo;
+        const float4 B = np->a_hi;
+        const float4 C = np->b_box;
+        const uint4 lk = np->link;
+        if (COUNT)
+            n_nodes++;
+        float t0min, t0max, t1min, t1max;
+        slab(o, inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
+        slab(o, inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
+        bool h0 = (t0min <= t0max) && (t0max >= tnear) && !(t0min > best.t * CULL_SLACK);
+        bool h1 = (t1min <= t1max) && (t1max >= tnear) && !(t1min > best.t * CULL_SLACK);
+        if (h0 && (lk.x & LEAF))
+            {
+            const uint32_t prim = lk.x & ~LEAF;
+            float t;
+            if (COUNT)
+                n_prims++;
+            if (intersect_prim<false>(sc.geoms[lk.z], prim, o, d, tnear, t, nullptr) && better(t, lk.z, prim, best))
+                {
+                best.t = t;
+                best.geom = lk.z;
+                best.prim = prim;
+                }
+            h0 = false;
+            }
+        if (h1 && (lk.y & LEAF))
+            {
+            const uint32_t prim = lk.y & ~LEAF;
+            float t;
+            if (COUNT)
+                n_prims++;
+            if (intersect_prim<false>(sc.geoms[lk.w], prim, o, d, tnear, t, nullptr) && better(t, lk.w, prim, best))
+                {
+                best.t = t;
+                best.geom = lk.w;
+                best.prim = prim;
+                }
+            h1 = false;
+            }
+        h0 = h0 && !(t0min > best.t * CULL_SLACK);
+        h1 = h1 && !(t1min > best.t * CULL_SLACK);
+        if (h0 && h1)
+            {
+            const bool swap = t1min < t0min;
+            st.push(swap ? lk.x : lk.y, swap ? t0min : t1min, err);
+            node = swap ? lk.y : lk.x;
+            }
+        else if (h0)
+            node = lk.x;
+        else if (h1)
+            node = lk.y;
+        else
+            {
+            bool found = false;
+            while (st.sp > 0)
+                {
+                uint32_t n;
+                float tm;
+                st.pop(n, tm);
+                if (!(tm > best.t * CULL_SLACK))
+                    {
+                    node = n;
+                    found = true;
+                    break;
+                    }
+                }
+            if (!found)
+                break;
+            }
+        }
+    }
+
+__device__ __forceinline__ uint32_t lane_id()
+    {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    }
+
+// Append slot for lanes with pred set: one atomic per wavefront.  Every lane
+// of the wavefront must call it.
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+    {
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = lane_id();
+    uint32_t base = 0;
+    if (lane == 0 && m)
+        base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, 0, 64);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    }
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+    {
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    return v;
+    }
+
+__device__ __forceinline__ uint32_t find_job(const JobDev* jobs, uint32_t n_jobs, uint32_t i)
+    {
+    // last job with ray_base <= i
+    uint32_t lo = 0, hi = n_jobs;
+    while (hi - lo > 1)
+        {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].ray_base <= i)
+            lo = mid;
+        else
+            hi = mid;
+        }
+    return lo;
+    }
+
+__device__ __forceinline__ uint4 make_rec(v3 c, uint32_t code)
+    {
+    return make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), code);
+    }
+
+__device__ __forceinline__ uint64_t rec_index(const JobDev& J, uint32_t k, uint32_t pixel, uint32_t ls)
+    {
+    return J.rec_base + ((uint64_t)k * J.npix + pixel) * J.L + ls;
+    }
+
+struct Queue
+    {
+    float4* o;   // origin, tnear
+    float4* d;   // direction, job (bits)
+    float4* att; // attenuation
+    uint4* meta; // job, pixel, sample-in-batch | light sample << 16, depth
+    };
+
+// ---------------------------------------------------------------- camgen
+// Primary ray of camera sample n_base + k + 1 for every pixel of every job
+// (fresnel/cpu/TracerPath.cc:100-136: tnear 1e-3).
+__global__ void k_camgen(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t total, Queue q)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total)
+        return;
+    const uint32_t j = find_job(jobs, n_jobs, i);
+    const JobDev& J = jobs[j];
+    const uint32_t local = i - J.ray_base;
+    const uint32_t k = local / J.npix;
+    const uint32_t pixel = local - k * J.npix;
+    const uint32_t py = pixel / J.W;
+    const uint32_t px = pixel - py * J.W;
+    v3 o, d;
+    camera_ray(J.cam, J.W, J.H, J.seed, px, py, J.n_base + k + 1, true, o, d);
+    q.o[i] = make_float4(o.x, o.y, o.z, 1e-3f);
+    q.d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(j));
+    q.att[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    q.meta[i] = make_uint4(j, pixel, k, 0);
+    }
+
+// ---------------------------------------------------------------- traverse
+// Persistent threads: each wavefront takes 64 rays at a time from a counter
+// until the queue is exhausted.
+template<int BLOCK, int SS, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ jobs, Queue q, uint32_t n, uint4* hits,
+                                                    uint32_t* work, uint2* gstack, uint32_t gstride,
+                                                    unsigned long long* counters)
+    {
+    __shared__ uint2 lds[SS * BLOCK];
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    LdsStack<SS> st;
+    st.lds = lds + threadIdx.x;
+    st.lstride = BLOCK;
+    st.glob = gstack + gtid;
+    st.gstride = gstride;
+    st.sp = 0;
+    const uint32_t lane = lane_id();
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    for (;;)
+        {
+        uint32_t base = 0;
+        if (lane == 0)
+            base = atomicAdd(work, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n)
+            break;
+        const uint32_t i = base + lane;
+        if (i < n)
+            {
+            const float4 a = q.o[i];
+            const float4 b = q.d[i];
+            const SceneDev& sc = jobs[__float_as_uint(b.w)].scene;
+            HitT h;
+            trace_closest<LdsStack<SS>, COUNT>(sc, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, h, st, n_nodes,
+                                               n_prims, err);
+            hits[i] = make_uint4(__float_as_uint(h.t), h.geom, h.prim, 0);
+            if (COUNT)
+                n_rays++;
+            }
+        }
+    if (COUNT)
+        {
+        atomicAdd(counters + 0, (unsigned long long)n_rays);
+        atomicAdd(counters + 1, (unsigned long long)n_nodes);
+        atomicAdd(counters + 2, (unsigned long long)n_prims);
+        }
+    if (err)
+        atomicOr((unsigned int*)(counters + 7), 1u);
+    }
+
+// ---------------------------------------------------------------- shading
+// Shading attributes of the closest hit: re-evaluate the winning primitive.
+__device__ __forceinline__ void hit_attributes(const SceneDev& sc, const HitT& h, v3 o, v3 d, float tnear, Attr& at,
+                                               const GeomDev*& g)
+    {
+    g = sc.geoms + h.geom;
+    float t2;
+    intersect_prim<true>(*g, h.prim, o, d, tnear, t2, &at);
+    }
+
+// Depth-0 shading: one primary hit spawns the L light-sample paths
+// (fresnel/cpu/TracerPath.cc:141-155 reuse the cached primary hit).
+__global__ void k_shade_primary(const JobDev* __restrict__ jobs, Queue in, const uint4* __restrict__ hits, uint32_t n,
+                                Queue out, uint32_t* out_count, uint4* records)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t L = 0;
+    uint32_t job = 0, pixel = 0, k = 0;
+    HitT h;
+    h.geom = EMPTY;
+    v3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
+    float tnear = 0;
+    if (valid)
+        {
+        const uint4 mt = in.meta[i];
+        job = mt.x;
+        pixel = mt.y;
+        k = mt.z;
+        const uint4 hh = hits[i];
+        h.t = __uint_as_float(hh.x);
+        h.geom = hh.y;
+        h.prim = hh.z;
+        const float4 a = in.o[i];
+        const float4 b = in.d[i];
+        o = mk3(a.x, a.y, a.z);
+        d = mk3(b.x, b.y, b.z);
+        tnear = a.w;
+        }
+    const JobDev& J = jobs[job];
+    if (valid)
+        {
+        L = J.L;
+        if (h.geom == EMPTY)
+            {
+            // primary miss: background for the whole camera sample (TracerPathMethods.h:44-49)
+            records[rec_index(J, k, pixel, 0)] = make_rec(mk3(0, 0, 0), REC_BG);
+            L = 0;
+            }
+        }
+    Attr at;
+    const GeomDev* g = nullptr;
+    const Mat* m = nullptr;
+    if (L)
+        {
+        hit_attributes(J.scene, h, o, d, tnear, at, g);
+        m = (at.d >= g->outline_width) ? &g->mat : &g->outline;
+        }
+    const uint32_t Lmax = wave_max(L);
+    const uint32_t n_sample = valid ? J.n_base + k + 1 : 0;
+    for (uint32_t ls = 0; ls < Lmax; ls++)
+        {
+        bool cont = false;
+        v3 att = mk3(1.0f, 1.0f, 1.0f), no, nd, add;
+        if (ls < L)
+            {
+            bool addv;
+            cont = path_hit(*m, at, o, d, h.t, pixel, J.seed, 0, (n_sample - 1) * J.L + ls, att, no, nd, add, addv);
+            if (!cont)
+                records[rec_index(J, k, pixel, ls)] = make_rec(addv ? add : mk3(0, 0, 0), addv ? REC_ADD : REC_NONE);
+            }
+        const uint32_t slot = wave_append(cont, out_count);
+        if (cont)
+            {
+            out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
+            out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(job));
+            out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
+            out.meta[slot] = make_uint4(job, pixel, k | (ls << 16), 1);
+            }
+        }
+    }
+
+// Shade one hit or miss of a path at depth >= 1.  Returns true when the path
+// continues with (no, nd, att); otherwise rec holds its termination record.
+__device__ inline bool shade_bounce(const JobDev& J, const HitT& h, v3 o, v3 d, float tnear, uint32_t pixel,
+                                    uint32_t k, uint32_t ls, uint32_t depth, v3& att, v3& no, v3& nd, uint4& rec)
+    {
+    if (h.geom == EMPTY)
+        {
+        // miss: lights whose cone contains the direction (TracerPathMethods.h:50-73)
+        rec = make_rec(att, REC_LIGHTS | (light_mask(J.lights, d) << 2));
+        return false;
+        }
+    Attr at;
+    const GeomDev* g;
+    hit_attributes(J.scene, h, o, d, tnear, at, g);
+    const Mat& m = (at.d >= g->outline_width) ? g->mat : g->outline;
+    const uint32_t n_sample = J.n_base + k + 1;
+    v3 add;
+    bool addv;
+    const bool cont = path_hit(m, at, o, d, h.t, pixel, J.seed, depth, (n_sample - 1) * J.L + ls, att, no, nd, add,
+                               addv);
+    if (!cont)
+        rec = make_rec(addv ? add : mk3(0, 0, 0), addv ? REC_ADD : REC_NONE);
+    return cont;
+    }
+
+__global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* __restrict__ hits, uint32_t n,
+                        Queue out, uint32_t* out_count, uint4* records, uint32_t max_depth,
+                        unsigned long long* counters)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cont = false;
+    v3 att, no, nd;
+    uint4 mt = make_uint4(0, 0, 0, 0);
+    if (i < n)
+        {
+        mt = in.meta[i];
+        const JobDev& J = jobs[mt.x];
+        const uint4 hh = hits[i];
+        HitT h;
+        h.t = __uint_as_float(hh.x);
+        h.geom = hh.y;
+        h.prim = hh.z;
+        const float4 a = in.o[i];
+        const float4 b = in.d[i];
+        const float4 at4 = in.att[i];
+        att = mk3(at4.x, at4.y, at4.z);
+        const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
+        uint4 rec;
+        cont = shade_bounce(J, h, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, mt.y, k, ls, mt.w, att, no, nd, rec);
+        if (cont && mt.w + 1 >= max_depth)
+            {
+            // depth cap (the reference has none): stop without contribution
+            cont = false;
+            rec = make_rec(mk3(0, 0, 0), REC_NONE);
+            atomicAdd(counters + 4, 1ull);
+            }
+        if (!cont)
+            records[rec_index(J, k, mt.y, ls)] = rec;
+        }
+    const uint32_t slot = wave_append(cont, out_count);
+    if (cont)
+        {
+        out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
+        out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
+        out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
+        out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
+        }
+    }
+
+// Tail: each thread finishes its path (trace + shade until termination).
+template<int BLOCK, int SS>
+__global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ jobs, Queue in, uint32_t n,
+                                                  uint4* records, uint32_t max_depth, uint2* gstack,
+                                                  uint32_t gstride, unsigned long long* counters)
+    {
+    __shared__ uint2 lds[SS * BLOCK];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    LdsStack<SS> st;
+    st.lds = lds + threadIdx.x;
+    st.lstride = BLOCK;
+    st.glob = gstack + i;
+    st.gstride = gstride;
+    st.sp = 0;
+    const uint4 mt = in.meta[i];
+    const JobDev& J = jobs[mt.x];
+    const float4 a = in.o[i];
+    const float4 b = in.d[i];
+    const float4 at4 = in.att[i];
+    v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
+    const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
+    uint32_t depth = mt.w;
+    uint32_t dummy = 0, err = 0;
+    uint4 rec;
+    for (;;)
+        {
+        HitT h;
+        trace_closest<LdsStack<SS>, false>(J.scene, o, d, 1e-3f, h, st, dummy, dummy, err);
+        v3 no, nd;
+        if (!shade_bounce(J, h, o, d, 1e-3f, mt.y, k, ls, depth, att, no, nd, rec))
+            break;
+        if (depth + 1 >= max_depth)
+            {
+            rec = make_rec(mk3(0, 0, 0), REC_NONE);
+            atomicAdd(counters + 4, 1ull);
+            break;
+            }
+        o = no;
+        d = nd;
+        depth++;
+        }
+    records[rec_index(J, k, mt.y, ls)] = rec;
+    if (err)
+        atomicOr((unsigned int*)(counters + 7), 1u);
+    }
+
+// ---------------------------------------------------------------- accumulate
+// Replay each camera sample's records in light-sample order, then the Welford
+// mean and sRGB of fresnel/cpu/TracerPath.cc:229-249.
+__global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __restrict__ pix_base, uint32_t n_jobs,
+                             uint32_t total, const uint4* __restrict__ records)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total)
+        return;
+    uint32_t lo = 0, hi = n_jobs;
+    while (hi - lo > 1)
+        {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pix_base[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+        }
+    const JobDev& J = jobs[lo];
+    const uint32_t pixel = i - pix_base[lo];
+    float4 mean = J.linear[pixel];
+    for (uint32_t k = 0; k < J.K; k++)
+        {
+        v3 result = mk3(0, 0, 0);
+        float a = 1.0f;
+        const uint4 r0 = records[rec_index(J, k, pixel, 0)];
+        if ((r0.w & 3u) == REC_BG)
+            {
+            result = J.bg * float(J.L);
+            a = J.bg_alpha;
+            }
+        else
+            {
+            for (uint32_t ls = 0; ls < J.L; ls++)
+                {
+                const uint4 r = ls ? records[rec_index(J, k, pixel, ls)] : r0;
+                const v3 val = mk3(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z));
+                const uint32_t kind = r.w & 3u;
+                if (kind == REC_ADD)
+                    result = result + val;
+                else if (kind == REC_LIGHTS)
+                    {
+                    const uint32_t mask = r.w >> 2;
+                    for (uint32_t li = 0; li < J.lights.N; li++)
+                        if (mask & (1u << li))
+                            result = result + (val * J.lights.color[li]) * J.lights.inv_sin[li];
+                    }
+                }
+            }
+        const v3 s = div_rcp(result, float(J.L));
+        const float q = 1.0f / float(J.n_base + k + 1);
+        mean.x = mean.x + (s.x - mean.x) * q;
+        mean.y = mean.y + (s.y - mean.y) * q;
+        mean.z = mean.z + (s.z - mean.z) * q;
+        mean.w = mean.w + (a - mean.w) * q;
+        }
+    J.linear[pixel] = mean;
+    if (!J.highlight || (mean.x <= 1.0f && mean.y <= 1.0f && mean.z <= 1.0f))
+        J.srgb[pixel] = to_srgb(mean);
+    else
+        J.srgb[pixel] = to_srgb(make_float4(J.hw_color.x, J.hw_color.y, J.hw_color.z, mean.w));
+    }
+
+// ---------------------------------------------------------------- preview
+// fresnel/cpu/TracerDirect.cc:77-218: aa_n^2 primary rays with tnear 0 and
+// representative-point direct lighting, plain mean.
+__global__ void k_preview(const JobDev* __restrict__ jobp, uint32_t aa_n)
+    {
+    const JobDev& J = *jobp;
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= J.npix)
+        return;
+    const uint32_t py = pixel / J.W, px = pixel - py * J.W;
+    LocalStack st;
+    uint32_t dummy = 0, err = 0;
+    float4 avg = make_float4(0, 0, 0, 0);
+    for (uint32_t sample = 0; sample < aa_n * aa_n; sample++)
+        {
+        v3 o, d;
+        camera_ray(J.cam, J.W, J.H, J.seed, px, py, sample, aa_n != 1, o, d);
+        HitT h;
+        trace_closest<LocalStack, false>(J.scene, o, d, 0.0f, h, st, dummy, dummy, err);
+        v3 c = J.bg;
+        float a = J.bg_alpha;
+        if (h.geom != EMPTY)
+            {
+            Attr at;
+            const GeomDev* g;
+            hit_attributes(J.scene, h, o, d, 0.0f, at, g);
+            const v3 n = div_true(at.Ng, sqrtf(dot(at.Ng, at.Ng)));
+            const v3 v = div_rcp(-d, sqrtf(dot(d, d)));
+            const Mat& m = (at.d >= g->outline_width) ? g->mat : g->outline;
+            if (m.solid > 0.5f)
+                c = get_color(m, at.color);
+            else
+                {
+                c = mk3(0, 0, 0);
+                for (uint32_t li = 0; li < J.lights.N; li++)
+                    {
+                    const v3 l = J.lights.dir[li];
+                    v3 r = -v + (2.0f * n * dot(n, v));
+                    const float half = J.lights.theta[li];
+                    const float ldotr = dot(l, r);
+                    if (ldotr < J.lights.cos_theta[li])
+                        {
+                        v3 ax = cross(l, r);
+                        ax = div_rcp(ax, sqrtf(dot(ax, ax)));
+                        const float ang = -acosf(ldotr) + half;
+                        quat qr;
+                        qr.s = cosf(ang / 2.0f);
+                        qr.v = sinf(ang / 2.0f) * ax;
+                        r = rotate(qr, r);
+                        }
+                    const float ndotl = dot(n, l);
+                    const v3 f_d = (ndotl >= 0.0f) ? brdf_diffuse(m, l, v, n, at.color) * ndotl : mk3(0, 0, 0);
+                    const v3 f_s = (dot(n, r) >= 0.0f)
+                                       ? brdf_specular(m, r, v, n, at.color, half, J.lights.tan_clamped[li]) * dot(n, r)
+                                       : mk3(0, 0, 0);
+                    c = c + (f_d + f_s) * PI_F * J.lights.color[li];
+                    }
+                }
+            a = 1.0f;
+            }
+        avg.x += c.x;
+        avg.y += c.y;
+        avg.z += c.z;
+        avg.w += a;
+        }
+    const float q = 1.0f / float(aa_n * aa_n);
+    const float4 outp = make_float4(avg.x * q, avg.y * q, avg.z * q, avg.w * q);
+    J.linear[pixel] = outp;
+    if (!J.highlight || (outp.x <= 1.0f && outp.y <= 1.0f && outp.z <= 1.0f))
+        J.srgb[pixel] = to_srgb(outp);
+    else
+        J.srgb[pixel] = to_srgb(make_float4(J.hw_color.x, J.hw_color.y, J.hw_color.z, outp.w));
+    }
+
+// ---------------------------------------------------------------- diagnostics
+__global__ void k_debug_philox(const uint32_t* in, uint32_t n, uint32_t* out)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t* c = in + 6 * i;
+    const u4 r = philox(c[0], c[1], c[2], c[3], c[4], c[5]);
+    for (int q = 0; q < 4; q++)
+        out[4 * i + q] = r.v[q];
+    }
+
+// Primary ray and closest hit per pixel for camera sample `sample`.
+__global__ void k_debug_primary(const JobDev* __restrict__ jobp, uint32_t sample, float* rays, uint4* hits)
+    {
+    const JobDev& J = *jobp;
+    const uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pixel >= J.npix)
+        return;
+    const uint32_t py = pixel / J.W, px = pixel - py * J.W;
+    v3 o, d;
+    camera_ray(J.cam, J.W, J.H, J.seed, px, py, sample, true, o, d);
+    LocalStack st;
+    uint32_t dummy = 0, err = 0;
+    HitT h;
+    trace_closest<LocalStack, false>(J.scene, o, d, 1e-3f, h, st, dummy, dummy, err);
+    float* r = rays + 6 * (size_t)pixel;
+    r[0] = o.x;
+    r[1] = o.y;
+    r[2] = o.z;
+    r[3] = d.x;
+    r[4] = d.y;
+    r[5] = d.z;
+    hits[pixel] = make_uint4(__float_as_uint(h.t), h.geom, h.prim, err);
+    }
+
+__global__ void k_debug_trace(const SceneDev sc, const float* rays, uint32_t n, float tnear, uint4* hits)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float* r = rays + 6 * (size_t)i;
+    LocalStack st;
+    uint32_t dummy = 0, err = 0;
+    HitT h;
+    trace_closest<LocalStack, false>(sc, mk3(r[0], r[1], r[2]), mk3(r[3], r[4], r[5]), tnear, h, st, dummy, dummy,
+                                     err);
+    hits[i] = make_uint4(__float_as_uint(h.t), h.geom, h.prim, err);
+    }
+    } // namespace fh

@@ -1,0 +1,133 @@
+// scene_dev.h -- device-resident scene, BVH and job records.
+#pragma once
+
+#include "dmath.h"
+#include "fresnel_hip.h"
+
+namespace fh
+{
+enum GeomType : uint32_t
+    {
+    G_SPHERE = FH_GEOM_SPHERE,
+    G_CYLINDER = FH_GEOM_CYLINDER,
+    G_POLYGON = FH_GEOM_POLYGON,
+    G_POLYHEDRON = FH_GEOM_CONVEX_POLYHEDRON,
+    G_MESH = FH_GEOM_MESH
+    };
+
+// Material record (fresnel/common/Material.h:47-56 field order)
+struct Mat
+    {
+    float solid;
+    float cr, cg, cb;
+    float mix;
+    float roughness;
+    float specular;
+    float metal;
+    float spec_trans;
+    };
+
+// One geometry slot of a scene.  The slot index is the geometry id: it
+// orders exact-t ties and selects the materials, as Embree's geomID does in
+// the reference (fresnel/cpu/Scene.h:55-107).
+struct GeomDev
+    {
+    uint32_t type;
+    uint32_t n_prims;     // sphere/cyl/polyhedron/polygon: N; mesh: instances * faces
+    uint32_t n_aux;       // polyhedron: planes; mesh: faces; polygon: vertices
+    uint32_t pad0;
+    float outline_width;
+    float color_by_face;  // polyhedron
+    float rounding;       // polygon rounding radius
+    float xy_radius;      // polygon bounds radius (vertex radius + rounding)
+    Mat mat;
+    Mat outline;
+    float pad1[2];
+    // primitive attributes (layout per type, packed on upload)
+    //   sphere:     a[i] = (x, y, z, r)
+    //   cylinder:   a[i] = (Ax, Ay, Az, r), b[i] = (Bx, By, Bz, 0)
+    //   polyhedron: a[i] = (x, y, z, 0),    b[i] = quaternion (s, x, y, z)
+    //   mesh:       a[inst] = (x, y, z, 0), b[inst] = quaternion
+    //   polygon:    a[i] = (x, y, 0, 0),    b[i] = quaternion about +z from the angle
+    const float4* a;
+    const float4* b;
+    const float* color;    // the reference's color array layout
+    const float4* aux0;    // polyhedron plane (nx, ny, nz, -dot(n, p)); mesh vertices; polygon vertices
+    const float4* aux1;    // polyhedron plane origins
+    const float* aux_color; // polyhedron plane colors
+    };
+
+// BVH2 node with both children's boxes (64 bytes).  A child is an inner node
+// index, a leaf (LEAF bit | local primitive index, geometry slot in g), or
+// EMPTY with a NaN box that every slab test rejects.
+constexpr uint32_t LEAF = 0x80000000u;
+constexpr uint32_t EMPTY = 0xffffffffu;
+struct __attribute__((aligned(64))) Node
+    {
+    float4 a_lo;  // child 0: lo.xyz, hi.x
+    float4 a_hi;  // child 0: hi.y, hi.z | child 1: lo.x, lo.y
+    float4 b_box; // child 1: lo.z, hi.x, hi.y, hi.z
+    uint4 link;   // c0, c1, g0, g1
+    };
+
+struct SceneDev
+    {
+    const Node* nodes;
+    const GeomDev* geoms;
+    uint32_t root;      // EMPTY when the scene has no primitive
+    uint32_t n_geoms;
+    uint32_t n_prims;
+    uint32_t pad;
+    };
+
+// Camera constants derived on the host from the UserCamera
+// (fresnel/common/Camera.h:92-159).
+struct CamDev
+    {
+    v3 p, u, v, w;
+    float focal_h, focal_d, a;
+    int32_t model;
+    };
+
+// Lights in scene space plus per-light constants evaluated once on the host:
+// cos(theta) and 1/sin(min(theta, pi/2)) (fresnel/common/TracerPathMethods.h:57-71),
+// tan(min(theta, 0.99 pi/2)) for the preview (fresnel/common/Material.h:187-192).
+struct LightsDev
+    {
+    v3 dir[4];
+    v3 color[4];
+    float theta[4];
+    float cos_theta[4];
+    float inv_sin[4];
+    float tan_clamped[4];
+    uint32_t N;
+    };
+
+// One render job: one (scene, tracer) pair for one batch of camera samples.
+struct JobDev
+    {
+    SceneDev scene;
+    CamDev cam;
+    LightsDev lights;
+    v3 bg;
+    float bg_alpha;
+    uint32_t W, H, npix;
+    uint32_t seed;
+    uint32_t n_base;      // passes accumulated before this batch
+    uint32_t K;           // camera samples in this batch
+    uint32_t L;           // light samples per camera sample
+    uint32_t highlight;
+    v3 hw_color;
+    uint32_t ray_base;    // first primary ray of this job in the batch
+    uint64_t rec_base;    // first termination record of this job
+    float4* linear;       // running mean (h, w) RGBA
+    uchar4* srgb;         // sRGB output (h, w) RGBA
+    };
+
+// Termination record of one light-sample path.
+//   kind 0: no contribution
+//   kind 1: add (x, y, z)             solid emitter, TracerPathMethods.h:128-134
+//   kind 2: lights in bits 2..5 with attenuation (x, y, z), TracerPathMethods.h:55-73
+//   kind 3: primary miss (the whole camera sample is background), :44-49
+constexpr uint32_t REC_NONE = 0, REC_ADD = 1, REC_LIGHTS = 2, REC_BG = 3;
+    } // namespace fh

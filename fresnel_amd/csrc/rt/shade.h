@@ -1,0 +1,308 @@
+// shade.h -- camera rays, Disney BRDF, lobe sampling and path events on the device.
+#pragma once
+
+#include "scene_dev.h"
+
+namespace fh
+{
+__device__ __forceinline__ v3 mat_color(const Mat& m)
+    {
+    return mk3(m.cr, m.cg, m.cb);
+    }
+// Material::getColor, fresnel/common/Material.h:225-228
+__device__ __forceinline__ v3 get_color(const Mat& m, v3 sc)
+    {
+    return lerp(m.mix, mat_color(m), sc);
+    }
+// schlick, Material.h:26-33
+__device__ __forceinline__ float schlick(float x)
+    {
+    const float v = 1.0f - x;
+    const float v_sq = v * v;
+    return v_sq * v_sq * v;
+    }
+
+// Disney diffuse term shared by brdf and brdf_diffuse (Material.h:88-93)
+__device__ __forceinline__ v3 disney_diffuse(const Mat& m, v3 base_color, float ndotl, float ndotv, float ldoth)
+    {
+    const float FL = schlick(ndotl);
+    const float FV = schlick(ndotv);
+    const float RR = 2 * m.roughness * ldoth * ldoth;
+    return div_rcp(base_color, PI_F)
+           * ((1.0f - 0.5f * FL) * (1.0f - 0.5f * FV) + RR * (FL + FV + FL * FV * (RR - 1.0f)));
+    }
+
+// Material::brdf, Material.h:67-122
+__device__ inline v3 brdf(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
+    {
+    const float ndotv = dot(n, v);
+    if (ndotv <= 0)
+        return mk3(0, 0, 0);
+    v3 h = l + v;
+    h = div_true(h, sqrtf(dot(h, h)));
+    const float ndotl = dot(n, l);
+    const float ldoth = dot(l, h);
+    const float ndoth = dot(n, h);
+    const v3 base_color = get_color(m, sc);
+    const v3 f_d = disney_diffuse(m, base_color, ndotl, ndotv, ldoth);
+    const float alpha = m.roughness * m.roughness;
+    const float alpha_sq = alpha * alpha;
+    const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
+    const float D = alpha_sq / (PI_F * denom_rt * denom_rt);
+    const float f0 = 0.08f * m.specular;
+    const v3 F0 = lerp(m.metal, mk3(f0, f0, f0), base_color);
+    const v3 F = F0 + (mk3(1.0f, 1.0f, 1.0f) - F0) * schlick(ldoth);
+    const float ndotv_sq = ndotv * ndotv;
+    const float ndotl_sq = ndotl * ndotl;
+    const float V1 = 1.0f / (ndotv + sqrtf(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
+    const float V2 = 1.0f / (ndotl + sqrtf(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
+    const float V = V1 * V2;
+    const v3 f_s = D * F * V;
+    return f_d * (1.0f - m.metal) + f_s;
+    }
+
+// Material::brdf_diffuse, Material.h:124-153
+__device__ inline v3 brdf_diffuse(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
+    {
+    const float ndotv = dot(n, v);
+    if (ndotv <= 0 || m.metal == 1.0f)
+        return mk3(0, 0, 0);
+    v3 h = l + v;
+    h = div_true(h, sqrtf(dot(h, h)));
+    const float ndotl = dot(n, l);
+    const float ldoth = dot(l, h);
+    return disney_diffuse(m, get_color(m, sc), ndotl, ndotv, ldoth) * (1.0f - m.metal);
+    }
+
+// Material::brdf_specular, Material.h:155-218; tan_clamped is tanf of the
+// light half angle after the 0.99 pi/2 clamp (a per-light host constant).
+__device__ inline v3 brdf_specular(const Mat& m, v3 l, v3 v, v3 n, v3 sc, float light_half_angle,
+                                   float tan_clamped)
+    {
+    const float ndotv = dot(n, v);
+    if (ndotv <= 0)
+        return mk3(0, 0, 0);
+    v3 h = l + v;
+    h = div_true(h, sqrtf(dot(h, h)));
+    const float ndotl = dot(n, l);
+    const float ldoth = dot(l, h);
+    const float ndoth = dot(n, h);
+    const v3 base_color = get_color(m, sc);
+    const float alpha = m.roughness * m.roughness;
+    const float alpha_sq = alpha * alpha;
+    const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
+    float D = alpha_sq / (PI_F * denom_rt * denom_rt);
+    if (light_half_angle > 0.99f * PI_F / 2.0f)
+        light_half_angle = 0.99f * PI_F / 2.0f;
+    if (light_half_angle > 0.0f)
+        {
+        float alpha_prime = alpha + 0.5f * tan_clamped;
+        if (alpha_prime > 1.0f)
+            alpha_prime = 1.0f;
+        D = D * alpha_sq / (alpha_prime * alpha_prime);
+        }
+    const float f0 = 0.08f * m.specular;
+    const v3 F0 = lerp(m.metal, mk3(f0, f0, f0), base_color);
+    const v3 F = F0 + (mk3(1.0f, 1.0f, 1.0f) - F0) * schlick(ldoth);
+    const float ndotv_sq = ndotv * ndotv;
+    const float ndotl_sq = ndotl * ndotl;
+    const float V1 = 1.0f / (ndotv + sqrtf(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
+    const float V2 = 1.0f / (ndotl + sqrtf(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
+    return D * F * (V1 * V2);
+    }
+
+// tangent frame of Material.h:243-249 / 293-299.  `fabs(n.z) > 0.999` is a
+// double comparison; float(0.999) lies above 0.999, so it is `>= 0.999f`.
+__device__ __forceinline__ void tangent_frame(v3 n, v3& t_x, v3& t_y)
+    {
+    v3 up = mk3(0, 0, 1.0f);
+    if (fabsf(n.z) >= 0.999f)
+        up = mk3(1.0f, 0, 0);
+    t_x = cross(up, n);
+    t_x = div_rcp(t_x, sqrtf(dot(t_x, t_x)));
+    t_y = cross(n, t_x);
+    }
+
+// Material::importanceSampleGGX, Material.h:230-256
+__device__ inline v3 sample_ggx(const Mat& m, float xi_x, float xi_y, v3 v, v3 n)
+    {
+    const float alpha = m.roughness * m.roughness;
+    const float phi = 2.0f * PI_F * xi_x;
+    const float cos_theta = sqrtf((1.0f - xi_y) / (1.0f + (alpha * alpha - 1.0f) * xi_y));
+    const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    const v3 h_t = mk3(sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta);
+    v3 t_x, t_y;
+    tangent_frame(n, t_x, t_y);
+    const v3 h = t_x * h_t.x + t_y * h_t.y + n * h_t.z;
+    return 2.0f * dot(v, h) * h - v;
+    }
+// Material::pdfGGX, Material.h:258-280
+__device__ inline float pdf_ggx(const Mat& m, v3 l, v3 v, v3 n)
+    {
+    v3 h = l + v;
+    h = div_true(h, sqrtf(dot(h, h)));
+    const float ldoth = dot(l, h);
+    const float ndoth = dot(n, h);
+    const float alpha = m.roughness * m.roughness;
+    const float alpha_sq = alpha * alpha;
+    const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
+    const float D = alpha_sq / (PI_F * denom_rt * denom_rt);
+    const float pdf_l = D * ndoth / (4.0f * ldoth);
+    return (pdf_l > 0.0f) ? pdf_l : 0.0f;
+    }
+// Material::importanceSampleDiffuse, Material.h:282-304
+__device__ inline v3 sample_diffuse(float xi_x, float xi_y, v3 n)
+    {
+    const float r = sqrtf(xi_x);
+    const float theta = 2 * PI_F * xi_y;
+    const v3 v_t = mk3(r * cosf(theta), r * sinf(theta), sqrtf(1.0f - xi_x));
+    v3 t_x, t_y;
+    tangent_frame(n, t_x, t_y);
+    return t_x * v_t.x + t_y * v_t.y + n * v_t.z;
+    }
+// Material::pdfDiffuse, Material.h:306-313
+__device__ __forceinline__ float pdf_diffuse(v3 l, v3 n)
+    {
+    const float ndotl = dot(n, l);
+    return (ndotl > 0.0f) ? ndotl / PI_F : 0.0f;
+    }
+
+// ---------------------------------------------------------------- camera
+// Camera::importanceSampleAA + generateRay, fresnel/common/Camera.h:169-269
+__device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, uint32_t seed, uint32_t i, uint32_t j,
+                                  uint32_t sample, bool aa, v3& o, v3& d)
+    {
+    const uint32_t key0 = j * W + i;
+    float i_f, j_f;
+    if (aa)
+        {
+        const u4 r = philox(0, 0, sample, RNG_AA, key0, seed);
+        const float r1 = u01(r.v[0]) * 2.0f;
+        const float r2 = u01(r.v[1]) * 2.0f;
+        const float dx = (r1 < 1.0f) ? (sqrtf(r1) - 1.0f) : (1.0f - sqrtf(2.0f - r1));
+        const float dy = (r2 < 1.0f) ? (sqrtf(r2) - 1.0f) : (1.0f - sqrtf(2.0f - r2));
+        const float aa_w = 0.707106781f;
+        i_f = float(i) + 0.5f + dx * aa_w;
+        j_f = float(j) + 0.5f + dy * aa_w;
+        }
+    else
+        {
+        i_f = float(i) + 0.5f;
+        j_f = float(j) + 0.5f;
+        }
+    const float ys = -1.0f * (j_f / float(H) - 0.5f);
+    const float xs = i_f / float(H) - 0.5f * float(W) / float(H);
+    if (c.model == 0)
+        {
+        o = c.p + (ys * c.v + xs * c.u) * c.focal_h;
+        d = -c.w;
+        return;
+        }
+    const v3 F = c.p - c.w * c.focal_d;
+    const v3 C = F + (ys * c.v + xs * c.u) * c.focal_h;
+    const u4 r = philox(0, 0, sample, RNG_APERTURE, key0, seed);
+    const float theta = u01(r.v[0]) * 2.0f * PI_F;
+    const float rr = u01(r.v[1]) * c.a * 0.5f;
+    const v3 offset = rr * cosf(theta) * c.u + rr * sinf(theta) * c.v;
+    o = c.p + offset;
+    d = C - o;
+    d = d * (1.0f / sqrtf(dot(d, d)));
+    }
+
+// ---------------------------------------------------------------- path events
+// path_tracer_hit after the outline choice: fresnel/common/TracerPathMethods.h:79-168
+// with RayGen::MISReflectionTransmission and shouldTerminatePath
+// (fresnel/common/RayGen.h:110-186).  Returns true when the path continues;
+// `add` receives a solid emitter's contribution (add_valid).
+__device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, float t_hit, uint32_t key0, uint32_t seed,
+                                uint32_t depth, uint32_t sidx, v3& att, v3& next_o, v3& next_d, v3& add,
+                                bool& add_valid)
+    {
+    add_valid = false;
+    const v3 Ng = at.Ng;
+    const v3 n = Ng * (1.0f / sqrtf(dot(Ng, Ng)));
+    const v3 v = -rd;
+    if (m.solid > 0.5f)
+        {
+        add = att * get_color(m, at.color);
+        add_valid = true;
+        return false;
+        }
+    bool done = false;
+    const u4 ru = philox(0, depth, sidx, RNG_MIS, key0, seed);
+    const float xi_x = u01(ru.v[0]);
+    const float xi_y = u01(ru.v[1]);
+    const float choice_mis = u01(ru.v[2]);
+    const float choice_trans = u01(ru.v[3]);
+    float factor = 1.0f;
+    v3 l;
+    const bool transmit = (choice_trans <= m.spec_trans);
+    if (transmit)
+        l = -v;
+    else if (choice_mis <= 0.5f)
+        {
+        l = sample_diffuse(xi_x, xi_y, n);
+        const float pd = pdf_diffuse(l, n);
+        const float pg = pdf_ggx(m, l, v, n);
+        const float w_d = pd / (pd + pg);
+        factor = w_d / (0.5f * pd);
+        }
+    else
+        {
+        l = sample_ggx(m, xi_x, xi_y, v, n);
+        const float pd = pdf_diffuse(l, n);
+        const float pg = pdf_ggx(m, l, v, n);
+        const float w_g = pg / (pd + pg);
+        factor = w_g / (0.5f * pg);
+        }
+    if (transmit)
+        {
+        const v3 tc = get_color(m, at.color);
+        att = att * mk3(sqrtf(tc.x), sqrtf(tc.y), sqrtf(tc.z));
+        }
+    else
+        {
+        const float ndotl = dot(n, l);
+        if (dot(n, v) > 0.0f && ndotl > 0.0f)
+            att = att * (brdf(m, l, v, n, at.color) * ndotl * factor);
+        else
+            {
+            att = mk3(0, 0, 0);
+            done = true;
+            }
+        }
+    next_o = ro + rd * t_hit;
+    next_d = l;
+    const u4 rr = philox(0, depth, sidx, RNG_RR, key0, seed);
+    float p_continue = fmaxf(att.x, fmaxf(att.y, att.z));
+    p_continue = fminf(p_continue, 1.0f);
+    if (u01(rr.v[0]) > p_continue)
+        done = true;
+    else
+        att = div_true(att, p_continue);
+    return !done;
+    }
+
+// Lights whose cone contains an escaping direction (TracerPathMethods.h:55-73).
+__device__ __forceinline__ uint32_t light_mask(const LightsDev& L, v3 rd)
+    {
+    uint32_t mask = 0;
+    for (uint32_t li = 0; li < L.N; li++)
+        if (dot(L.dir[li], rd) >= L.cos_theta[li])
+            mask |= 1u << li;
+    return mask;
+    }
+
+// sRGB of fresnel/common/ColorMath.h:514-546
+__device__ __forceinline__ unsigned char srgb_channel(float c)
+    {
+    float t = (c < 0.0031308f) ? 12.92f * c : (1.0f + 0.055f) * powf(c, 1.0f / 2.4f) - 0.055f;
+    if (t > 1.0f)
+        t = 1.0f;
+    return (unsigned char)(t * 255.0f + 0.5f);
+    }
+__device__ __forceinline__ uchar4 to_srgb(float4 c)
+    {
+    return make_uchar4(srgb_channel(c.x), srgb_channel(c.y), srgb_channel(c.z), (unsigned char)(c.w * 255.0f + 0.5f));
+    }
+    } // namespace fh

@@ -1,0 +1,1885 @@
+// runtime.hip -- host runtime and C ABI of libfresnel_hip.so.
+//
+// Objects mirror the reference's backend classes (fresnel/cpu/*.h): a Device
+// owns a HIP stream and the render scratch; a Scene owns its geometries,
+// their device copies and the LBVH; a Tracer owns the device output buffers
+// and the sampling state (seed, sample count).  Every C entry point catches
+// exceptions and returns a status (include/fresnel_hip.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fresnel_hip.h"
+#include "kernels/bvh.hip"
+#include "kernels/trace.hip"
+
+namespace fh
+{
+// ====================================================================== errors
+static thread_local std::string g_last_error;
+
+struct Error : public std::runtime_error
+    {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) { }
+    };
+
+#define HIPCHECK(expr)                                                                               \
+    do                                                                                               \
+        {                                                                                            \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw Error(FH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));              \
+        } while (0)
+
+#define REQUIRE(cond, msg)                                                                           \
+    do                                                                                               \
+        {                                                                                            \
+        if (!(cond))                                                                                 \
+            throw Error(FH_ERR_INVALID, msg);                                                        \
+        } while (0)
+
+template<class F> static int guard(F f)
+    {
+    try
+        {
+        f();
+        g_last_error.clear();
+        return FH_OK;
+        }
+    catch (const Error& e)
+        {
+        g_last_error = e.what();
+        return e.code;
+        }
+    catch (const std::bad_alloc&)
+        {
+        g_last_error = "out of host memory";
+        return FH_ERR_NOMEM;
+        }
+    catch (const std::exception& e)
+        {
+        g_last_error = e.what();
+        return FH_ERR_INVALID;
+        }
+    }
+
+// ====================================================================== device memory
+template<class T> struct DBuf
+    {
+    T* p = nullptr;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf()
+        {
+        release();
+        }
+    void release()
+        {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        }
+    // grow-only; contents are not preserved on growth
+    void ensure(size_t n)
+        {
+        if (n <= cap && p)
+            return;
+        release();
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess)
+            {
+            p = nullptr;
+            throw Error(FH_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: "
+                                          + hipGetErrorString(e));
+            }
+        cap = std::max<size_t>(n, 1);
+        }
+    size_t bytes() const
+        {
+        return cap * sizeof(T);
+        }
+    };
+
+static unsigned blocks(size_t n, unsigned b)
+    {
+    return (unsigned)((n + b - 1) / b);
+    }
+
+// ====================================================================== device
+constexpr int TRAV_BLOCK = 128;
+constexpr int TRAV_SS = 16;
+constexpr uint32_t TAIL_RAYS = 32768;   // finish the remaining paths in one kernel below this
+constexpr uint32_t MAX_DEPTH = 4096;    // safety cap on path length (the reference has none)
+
+struct KernelTime
+    {
+    std::string name;
+    hipEvent_t a, b;
+    };
+
+struct RenderScratch
+    {
+    DBuf<float4> qo[2], qd[2], qatt[2];
+    DBuf<uint4> qmeta[2];
+    DBuf<uint4> hits;
+    DBuf<uint4> records;
+    DBuf<uint32_t> counters32; // [0] traverse work, [1] queue A count, [2] queue B count
+    DBuf<uint2> gstack;        // traversal stack overflow (persistent grid)
+    DBuf<uint2> tail_stack;    // stack overflow of the tail kernel
+    DBuf<JobDev> jobs;
+    DBuf<uint32_t> pix_base;
+    };
+
+struct DeviceObj
+    {
+    int hip_dev = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+    std::string name;
+    int limit = -1;
+    uint32_t trav_grid = 0;
+    RenderScratch rs;
+    DBuf<unsigned long long> counters;
+    uint32_t* h_pinned = nullptr; // 16 uint32 of pinned host memory for counts
+    bool stats_on = false;
+    bool count_on = false;
+    std::vector<KernelTime> times;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::pair<std::string, std::pair<double, uint32_t>>> last_stats;
+    uint64_t last_counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::mutex mu;
+
+    hipEvent_t event()
+        {
+        if (!event_pool.empty())
+            {
+            hipEvent_t e = event_pool.back();
+            event_pool.pop_back();
+            return e;
+            }
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        return e;
+        }
+    template<class F> void launch(const char* nm, F f)
+        {
+        if (!stats_on)
+            {
+            f();
+            HIPCHECK(hipGetLastError());
+            return;
+            }
+        KernelTime kt;
+        kt.name = nm;
+        kt.a = event();
+        kt.b = event();
+        HIPCHECK(hipEventRecord(kt.a, stream));
+        f();
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(kt.b, stream));
+        times.push_back(kt);
+        }
+    void begin_stats()
+        {
+        for (auto& t : times)
+            {
+            event_pool.push_back(t.a);
+            event_pool.push_back(t.b);
+            }
+        times.clear();
+        if (count_on || true)
+            {
+            counters.ensure(8);
+            HIPCHECK(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), stream));
+            }
+        }
+    void end_stats()
+        {
+        HIPCHECK(hipStreamSynchronize(stream));
+        last_stats.clear();
+        for (auto& t : times)
+            {
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, t.a, t.b));
+            bool found = false;
+            for (auto& s : last_stats)
+                if (s.first == t.name)
+                    {
+                    s.second.first += ms;
+                    s.second.second += 1;
+                    found = true;
+                    }
+            if (!found)
+                last_stats.push_back({t.name, {ms, 1u}});
+            }
+        unsigned long long c[8];
+        HIPCHECK(hipMemcpy(c, counters.p, sizeof(c), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 8; k++)
+            last_counters[k] = c[k];
+        }
+    };
+
+// ====================================================================== geometry
+struct SceneObj;
+
+struct GeometryObj
+    {
+    SceneObj* scene = nullptr;
+    uint32_t type = 0;
+    uint32_t slot = 0;
+    bool valid = true;
+    bool enabled = true;
+    uint32_t N = 0;       // primitives (mesh: instances)
+    uint32_t n_faces = 0; // mesh
+    fh_material mat {}, outline {};
+    float outline_width = 0;
+    float color_by_face = 0;
+    float radius = 0;   // polyhedron bounding radius / polygon xy radius
+    float rounding = 0; // polygon
+    // host buffers (authoritative)
+    std::vector<float> position, radius_buf, color, points, orientation, angle;
+    std::vector<float> plane_n, plane_p, plane_color; // polyhedron (normalized normals)
+    std::vector<float> verts;                         // mesh (3F x 3) / polygon (V x 2)
+    // device copies
+    DBuf<float4> a, b, aux0, aux1;
+    DBuf<float> dcolor, daux_color;
+    bool dirty = true;
+    bool aux_uploaded = false;
+    };
+
+static fh_material make_material(float r, float g, float b, float solid)
+    {
+    // Material(RGB, solid), fresnel/common/Material.h:61-65
+    fh_material m;
+    m.solid = solid;
+    m.color = fh_rgb {r, g, b};
+    m.primitive_color_mix = 0.0f;
+    m.roughness = 0.1f;
+    m.specular = 0.5f;
+    m.metal = 0.0f;
+    m.spec_trans = 0.0f;
+    return m;
+    }
+
+static Mat to_mat(const fh_material& m)
+    {
+    Mat r;
+    r.solid = m.solid;
+    r.cr = m.color.r;
+    r.cg = m.color.g;
+    r.cb = m.color.b;
+    r.mix = m.primitive_color_mix;
+    r.roughness = m.roughness;
+    r.specular = m.specular;
+    r.metal = m.metal;
+    r.spec_trans = m.spec_trans;
+    return r;
+    }
+
+// ====================================================================== scene
+struct SceneObj
+    {
+    DeviceObj* dev = nullptr;
+    fh_user_camera camera {};
+    fh_rgb bg {0, 0, 0};
+    float bg_alpha = 0;
+    fh_lights lights {};
+    std::map<uint32_t, GeometryObj*> geoms; // slot -> geometry
+    // device scene
+    DBuf<GeomDev> dgeoms;
+    uint32_t n_slots = 0;
+    DBuf<float4> blo, bhi;
+    DBuf<uint32_t> keys, vals, keys2, vals2, hist, sums, parent, flags, prefix, slots;
+    DBuf<float> partial, cb, root_box;
+    DBuf<Node> nodes;
+    uint32_t n_prims = 0;
+    bool bvh_valid = false;
+    std::vector<uint32_t> topo; // (slot, prim count) of the enabled geometries in the last build
+    uint64_t builds = 0, refits = 0;
+    bool allow_refit = false;   // refit instead of rebuilding when only positions moved
+
+    uint32_t alloc_slot()
+        {
+        uint32_t s = 0;
+        while (geoms.count(s))
+            s++;
+        return s;
+        }
+    };
+
+static void upload(DBuf<float4>& d, const std::vector<float4>& h, hipStream_t s)
+    {
+    d.ensure(h.size());
+    if (!h.empty())
+        HIPCHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice, s));
+    }
+static void upload(DBuf<float>& d, const std::vector<float>& h, hipStream_t s)
+    {
+    d.ensure(h.size());
+    if (!h.empty())
+        HIPCHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+
+// Pack one geometry's host buffers into the device layout of GeomDev.
+static void upload_geometry(GeometryObj& g, hipStream_t s)
+    {
+    std::vector<float4> a, b;
+    const uint32_t N = g.N;
+    switch (g.type)
+        {
+    case G_SPHERE:
+        a.resize(N);
+        for (uint32_t i = 0; i < N; i++)
+            a[i] = make_float4(g.position[3 * i], g.position[3 * i + 1], g.position[3 * i + 2], g.radius_buf[i]);
+        break;
+    case G_CYLINDER:
+        a.resize(N);
+        b.resize(N);
+        for (uint32_t i = 0; i < N; i++)
+            {
+            const float* p = &g.points[6 * i];
+            a[i] = make_float4(p[0], p[1], p[2], g.radius_buf[i]);
+            b[i] = make_float4(p[3], p[4], p[5], 0.0f);
+            }
+        break;
+    case G_POLYHEDRON:
+    case G_MESH:
+        a.resize(N);
+        b.resize(N);
+        for (uint32_t i = 0; i < N; i++)
+            {
+            a[i] = make_float4(g.position[3 * i], g.position[3 * i + 1], g.position[3 * i + 2], 0.0f);
+            const float* q = &g.orientation[4 * i];
+            b[i] = make_float4(q[0], q[1], q[2], q[3]);
+            }
+        break;
+    case G_POLYGON:
+        a.resize(N);
+        b.resize(N);
+        for (uint32_t i = 0; i < N; i++)
+            {
+            a[i] = make_float4(g.position[2 * i], g.position[2 * i + 1], 0.0f, 0.0f);
+            // quat::fromAxisAngle((0,0,1), angle), fresnel/common/VectorMath.h:920-925,
+            // evaluated on the host with the C library's cosf/sinf
+            const float th = g.angle[i];
+            const float c = cosf(th / 2.0f), sn = sinf(th / 2.0f);
+            b[i] = make_float4(c, sn * 0.0f, sn * 0.0f, sn * 1.0f);
+            }
+        break;
+        }
+    upload(g.a, a, s);
+    upload(g.b, b, s);
+    upload(g.dcolor, g.color, s);
+    if (!g.aux_uploaded)
+        {
+        std::vector<float4> x0, x1;
+        if (g.type == G_POLYHEDRON)
+            {
+            const size_t P = g.plane_n.size() / 3;
+            for (size_t k = 0; k < P; k++)
+                {
+                const v3 n = mk3(g.plane_n[3 * k], g.plane_n[3 * k + 1], g.plane_n[3 * k + 2]);
+                const v3 p = mk3(g.plane_p[3 * k], g.plane_p[3 * k + 1], g.plane_p[3 * k + 2]);
+                x0.push_back(make_float4(n.x, n.y, n.z, -dot(n, p)));
+                x1.push_back(make_float4(p.x, p.y, p.z, 0.0f));
+                }
+            upload(g.daux_color, g.plane_color, s);
+            }
+        else if (g.type == G_MESH)
+            {
+            for (size_t k = 0; k < g.verts.size() / 3; k++)
+                x0.push_back(make_float4(g.verts[3 * k], g.verts[3 * k + 1], g.verts[3 * k + 2], 0.0f));
+            }
+        else if (g.type == G_POLYGON)
+            {
+            for (size_t k = 0; k < g.verts.size() / 2; k++)
+                x0.push_back(make_float4(g.verts[2 * k], g.verts[2 * k + 1], 0.0f, 0.0f));
+            }
+        upload(g.aux0, x0, s);
+        upload(g.aux1, x1, s);
+        g.aux_uploaded = true;
+        }
+    g.dirty = false;
+    }
+
+static uint32_t prim_count(const GeometryObj& g)
+    {
+    return g.type == G_MESH ? g.N * g.n_faces : g.N;
+    }
+
+// Build or refit the LBVH over the enabled geometries.
+static void build_bvh(SceneObj& sc, bool refit_only)
+    {
+    DeviceObj& D = *sc.dev;
+    hipStream_t s = D.stream;
+    std::vector<uint32_t> prefix, slots;
+    uint32_t N = 0;
+    for (auto& kv : sc.geoms)
+        {
+        const GeometryObj& g = *kv.second;
+        if (!g.valid || !g.enabled)
+            continue;
+        const uint32_t n = prim_count(g);
+        if (n == 0)
+            continue;
+        prefix.push_back(N);
+        slots.push_back(kv.first);
+        N += n;
+        }
+    sc.n_prims = N;
+    if (N == 0)
+        {
+        sc.bvh_valid = true;
+        return;
+        }
+    REQUIRE(N < (1u << 30), "too many primitives in one scene");
+    sc.prefix.ensure(prefix.size());
+    sc.slots.ensure(slots.size());
+    HIPCHECK(hipMemcpyAsync(sc.prefix.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(sc.slots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s));
+    sc.blo.ensure(N);
+    sc.bhi.ensure(N);
+    const uint32_t na = (uint32_t)slots.size();
+    D.launch("lbvh_bounds",
+             [&]
+             {
+                 k_prim_bounds<<<blocks(N, 256), 256, 0, s>>>(sc.dgeoms.p, sc.prefix.p, sc.slots.p, na, N,
+                                                                sc.blo.p, sc.bhi.p);
+             });
+    sc.nodes.ensure(std::max<uint32_t>(N - 1, 1));
+    sc.root_box.ensure(6);
+    if (N == 1)
+        {
+        D.launch("lbvh_single", [&] { k_single_leaf<<<1, 1, 0, s>>>(sc.blo.p, sc.bhi.p, sc.nodes.p); });
+        sc.bvh_valid = true;
+        sc.builds++;
+        return;
+        }
+    if (!refit_only)
+        {
+        // centroid bounds -> morton -> sort -> hierarchy
+        const uint32_t n_part = std::min<uint32_t>(blocks(N, 256), 1024);
+        sc.partial.ensure(6 * n_part);
+        sc.cb.ensure(6);
+        D.launch("lbvh_centroid",
+                 [&]
+                 {
+                     k_centroid_partial<<<n_part, 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, sc.partial.p);
+                     k_centroid_final<<<1, 256, 0, s>>>(sc.partial.p, n_part, sc.cb.p);
+                 });
+        sc.keys.ensure(N);
+        sc.vals.ensure(N);
+        sc.keys2.ensure(N);
+        sc.vals2.ensure(N);
+        D.launch("lbvh_morton",
+                 [&] { k_morton<<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, sc.cb.p, sc.keys.p, sc.vals.p); });
+        const uint32_t nb = blocks(N, SORT_TILE);
+        const uint32_t nh = 256 * nb;
+        const uint32_t n_sums = blocks(nh, 2048);
+        REQUIRE(n_sums <= 2048, "scene too large for the radix sort scan");
+        sc.hist.ensure(nh);
+        sc.sums.ensure(n_sums);
+        uint32_t *kin = sc.keys.p, *vin = sc.vals.p, *kout = sc.keys2.p, *vout = sc.vals2.p;
+        for (uint32_t shift = 0; shift < 32; shift += 8)
+            {
+            D.launch("lbvh_sort",
+                     [&]
+                     {
+                         k_radix_hist<<<nb, SORT_BLOCK, 0, s>>>(kin, N, shift, nb, sc.hist.p);
+                         k_scan_blocks<<<n_sums, 256, 0, s>>>(sc.hist.p, nh, sc.sums.p);
+                         k_scan_top<<<1, 256, 0, s>>>(sc.sums.p, n_sums);
+                         k_scan_add<<<blocks(nh, 256), 256, 0, s>>>(sc.hist.p, nh, sc.sums.p);
+                         k_radix_scatter<<<nb, SORT_BLOCK, 0, s>>>(kin, vin, kout, vout, N, shift, nb, sc.hist.p);
+                     });
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+            }
+        // after 4 passes the sorted data is back in keys/vals
+        sc.parent.ensure(2 * N - 1);
+        D.launch("lbvh_karras",
+                 [&]
+                 {
+                     k_karras<<<blocks(N - 1, 256), 256, 0, s>>>(sc.keys.p, sc.vals.p, sc.blo.p, sc.bhi.p, (int)N,
+                                                                 sc.nodes.p, sc.parent.p);
+                 });
+        sc.builds++;
+        }
+    else
+        sc.refits++;
+    sc.flags.ensure(N - 1);
+    HIPCHECK(hipMemsetAsync(sc.flags.p, 0, (N - 1) * sizeof(uint32_t), s));
+    D.launch("lbvh_refit",
+             [&]
+             {
+                 k_refit<<<blocks(N, 256), 256, 0, s>>>(sc.vals.p, sc.blo.p, sc.bhi.p, (int)N, sc.nodes.p,
+                                                        sc.parent.p, sc.flags.p, sc.root_box.p);
+             });
+    sc.bvh_valid = true;
+    }
+
+// Bring the device copy of the scene up to date: uploads, geometry table, BVH.
+static void sync_scene(SceneObj& sc)
+    {
+    DeviceObj& D = *sc.dev;
+    bool any_dirty = false;
+    std::vector<uint32_t> topo;
+    for (auto& kv : sc.geoms)
+        {
+        GeometryObj& g = *kv.second;
+        if (!g.valid)
+            continue;
+        if (g.dirty)
+            {
+            upload_geometry(g, D.stream);
+            if (g.enabled)
+                any_dirty = true;
+            }
+        if (g.enabled)
+            {
+            topo.push_back(kv.first);
+            topo.push_back(prim_count(g));
+            }
+        }
+    // geometry table (materials change without a rebuild)
+    uint32_t n_slots = sc.geoms.empty() ? 1 : (sc.geoms.rbegin()->first + 1);
+    std::vector<GeomDev> table(n_slots);
+    std::memset((void*)table.data(), 0, table.size() * sizeof(GeomDev));
+    for (auto& kv : sc.geoms)
+        {
+        GeometryObj& g = *kv.second;
+        GeomDev& t = table[kv.first];
+        t.type = g.type;
+        t.n_prims = prim_count(g);
+        t.n_aux = g.type == G_POLYHEDRON ? (uint32_t)(g.plane_n.size() / 3)
+                                         : (g.type == G_MESH ? g.n_faces : (uint32_t)(g.verts.size() / 2));
+        t.outline_width = g.outline_width;
+        t.color_by_face = g.color_by_face;
+        t.rounding = g.rounding;
+        t.xy_radius = g.radius;
+        t.mat = to_mat(g.mat);
+        t.outline = to_mat(g.outline);
+        t.a = g.a.p;
+        t.b = g.b.p;
+        t.color = g.dcolor.p;
+        t.aux0 = g.aux0.p;
+        t.aux1 = g.aux1.p;
+        t.aux_color = g.daux_color.p;
+        }
+    sc.dgeoms.ensure(n_slots);
+    HIPCHECK(hipMemcpyAsync(sc.dgeoms.p, table.data(), n_slots * sizeof(GeomDev), hipMemcpyHostToDevice, D.stream));
+    sc.n_slots = n_slots;
+    const bool topo_same = sc.bvh_valid && topo == sc.topo;
+    if (!sc.bvh_valid || !topo_same)
+        build_bvh(sc, false);
+    else if (any_dirty)
+        build_bvh(sc, sc.allow_refit);
+    sc.topo = topo;
+    // the table upload above must complete before the host vector goes away
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    }
+
+static SceneDev scene_dev(const SceneObj& sc)
+    {
+    SceneDev d;
+    d.nodes = sc.nodes.p;
+    d.geoms = sc.dgeoms.p;
+    d.root = sc.n_prims ? 0u : EMPTY;
+    d.n_geoms = sc.n_slots;
+    d.n_prims = sc.n_prims;
+    d.pad = 0;
+    return d;
+    }
+
+// Camera constants, fresnel/common/Camera.h:92-159 (host float arithmetic)
+static CamDev make_cam(const fh_user_camera& u)
+    {
+    CamDev c;
+    v3 d = mk3(u.look_at.x - u.position.x, u.look_at.y - u.position.y, u.look_at.z - u.position.z);
+    v3 vv = mk3(u.up.x, u.up.y, u.up.z);
+    d = d * (1.0f / sqrtf(dot(d, d)));
+    vv = vv * (1.0f / sqrtf(dot(vv, vv)));
+    v3 uu = cross(d, vv);
+    uu = uu * (1.0f / sqrtf(dot(uu, uu)));
+    vv = cross(uu, d);
+    vv = vv * (1.0f / sqrtf(dot(vv, vv)));
+    c.u = uu;
+    c.v = vv;
+    c.w = -d;
+    c.p = mk3(u.position.x, u.position.y, u.position.z);
+    c.focal_d = u.focus_distance;
+    c.a = u.f / u.f_stop;
+    c.model = u.model;
+    c.focal_h = (u.model == 0) ? u.h : u.h / u.f * c.focal_d;
+    return c;
+    }
+
+// Lights in scene space (fresnel/common/Light.h:47-66) plus per-light constants.
+static LightsDev make_lights(const fh_lights& in, const CamDev& cam)
+    {
+    LightsDev L;
+    std::memset(&L, 0, sizeof(L));
+    L.N = std::min<uint32_t>(in.N, 4);
+    for (uint32_t i = 0; i < L.N; i++)
+        {
+        L.color[i] = mk3(in.color[i].r, in.color[i].g, in.color[i].b);
+        v3 v = mk3(in.direction[i].x, in.direction[i].y, in.direction[i].z);
+        v = v * (1.0f / sqrtf(dot(v, v)));
+        L.dir[i] = v.x * cam.u + v.y * cam.v + v.z * cam.w;
+        const float th = in.theta[i];
+        L.theta[i] = th;
+        L.cos_theta[i] = cosf(th);
+        const float hs = (th > PI_F / 2.0f) ? PI_F / 2.0f : th;
+        L.inv_sin[i] = 1.0f / sinf(hs);
+        const float ht = (th > 0.99f * PI_F / 2.0f) ? 0.99f * PI_F / 2.0f : th;
+        L.tan_clamped[i] = tanf(ht);
+        }
+    return L;
+    }
+
+// ====================================================================== tracer
+enum TracerType
+    {
+    T_PATH,
+    T_DIRECT
+    };
+
+struct TracerObj
+    {
+    DeviceObj* dev = nullptr;
+    TracerType type = T_PATH;
+    uint32_t W = 0, H = 0;
+    uint32_t seed = 0;
+    uint32_t n_samples = 0;
+    uint32_t L = 1;
+    uint32_t aa_n = 8;
+    bool highlight = false;
+    fh_rgb hw_color {1, 0, 1};
+    DBuf<float4> linear;
+    DBuf<uchar4> srgb;
+    std::vector<float> h_linear;
+    std::vector<uint8_t> h_srgb;
+
+    void resize(uint32_t w, uint32_t h)
+        {
+        if (w == 0 || h == 0)
+            throw Error(FH_ERR_INVALID, "Invalid dimensions");
+        W = w;
+        H = h;
+        linear.release();
+        srgb.release();
+        linear.ensure((size_t)w * h);
+        srgb.ensure((size_t)w * h);
+        clear();
+        h_linear.assign((size_t)w * h * 4, 0.0f);
+        h_srgb.assign((size_t)w * h * 4, 0);
+        }
+    void clear()
+        {
+        HIPCHECK(hipMemsetAsync(linear.p, 0, (size_t)W * H * sizeof(float4), dev->stream));
+        HIPCHECK(hipMemsetAsync(srgb.p, 0, (size_t)W * H * sizeof(uchar4), dev->stream));
+        HIPCHECK(hipStreamSynchronize(dev->stream));
+        }
+    };
+
+static JobDev make_job(SceneObj& sc, TracerObj& tr, const fh_user_camera& cam_user)
+    {
+    JobDev J;
+    std::memset(&J, 0, sizeof(J));
+    J.scene = scene_dev(sc);
+    J.cam = make_cam(cam_user);
+    J.lights = make_lights(sc.lights, J.cam);
+    J.bg = mk3(sc.bg.r, sc.bg.g, sc.bg.b);
+    J.bg_alpha = sc.bg_alpha;
+    J.W = tr.W;
+    J.H = tr.H;
+    J.npix = tr.W * tr.H;
+    J.seed = tr.seed;
+    J.n_base = tr.n_samples;
+    J.L = tr.L;
+    J.highlight = tr.highlight ? 1u : 0u;
+    J.hw_color = mk3(tr.hw_color.r, tr.hw_color.g, tr.hw_color.b);
+    J.linear = tr.linear.p;
+    J.srgb = tr.srgb.p;
+    return J;
+    }
+
+// ====================================================================== path rendering
+// Target number of light-sample paths per batch (sizes the queues).
+static uint64_t g_batch_paths = 8ull << 20;
+
+static void read_count(DeviceObj& D, const uint32_t* dptr, uint32_t& out)
+    {
+    HIPCHECK(hipMemcpyAsync(D.h_pinned, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    out = D.h_pinned[0];
+    }
+
+// Run one batch: every job advances by jobs[j].K camera samples.
+static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs)
+    {
+    hipStream_t s = D.stream;
+    RenderScratch& R = D.rs;
+    uint64_t n_pri = 0, n_paths = 0, n_pix = 0;
+    std::vector<uint32_t> pix_base;
+    for (auto& J : jobs)
+        {
+        J.ray_base = (uint32_t)n_pri;
+        J.rec_base = n_paths;
+        pix_base.push_back((uint32_t)n_pix);
+        n_pri += (uint64_t)J.npix * J.K;
+        n_paths += (uint64_t)J.npix * J.K * J.L;
+        n_pix += J.npix;
+        }
+    REQUIRE(n_pri < (1ull << 31) && n_paths < (1ull << 31), "batch too large");
+    const uint64_t cap = std::max(n_pri, n_paths);
+    for (int q = 0; q < 2; q++)
+        {
+        R.qo[q].ensure(cap);
+        R.qd[q].ensure(cap);
+        R.qatt[q].ensure(cap);
+        R.qmeta[q].ensure(cap);
+        }
+    R.hits.ensure(cap);
+    R.records.ensure(n_paths);
+    R.counters32.ensure(4);
+    R.jobs.ensure(jobs.size());
+    R.pix_base.ensure(jobs.size());
+    HIPCHECK(hipMemcpyAsync(R.jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(R.pix_base.p, pix_base.data(), pix_base.size() * 4, hipMemcpyHostToDevice, s));
+    const uint32_t gstride = D.trav_grid * TRAV_BLOCK;
+    R.gstack.ensure((size_t)(STACK_MAX - TRAV_SS) * gstride);
+    R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * TAIL_RAYS);
+    Queue Q[2];
+    for (int q = 0; q < 2; q++)
+        Q[q] = Queue {R.qo[q].p, R.qd[q].p, R.qatt[q].p, R.qmeta[q].p};
+    const uint32_t nj = (uint32_t)jobs.size();
+    uint32_t* work = R.counters32.p;
+    uint32_t* qcount = R.counters32.p + 1;
+    unsigned long long* ctr = D.counters.p;
+
+    auto traverse = [&](const Queue& q, uint32_t n)
+    {
+        HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
+        if (D.count_on)
+            D.launch("traverse",
+                     [&]
+                     {
+                         k_traverse<TRAV_BLOCK, TRAV_SS, true><<<D.trav_grid, TRAV_BLOCK, 0, s>>>(
+                             R.jobs.p, q, n, R.hits.p, work, R.gstack.p, gstride, ctr);
+                     });
+        else
+            D.launch("traverse",
+                     [&]
+                     {
+                         k_traverse<TRAV_BLOCK, TRAV_SS, false><<<D.trav_grid, TRAV_BLOCK, 0, s>>>(
+                             R.jobs.p, q, n, R.hits.p, work, R.gstack.p, gstride, ctr);
+                     });
+    };
+
+    D.launch("camgen", [&] { k_camgen<<<blocks(n_pri, 256), 256, 0, s>>>(R.jobs.p, nj, (uint32_t)n_pri, Q[0]); });
+    traverse(Q[0], (uint32_t)n_pri);
+    HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
+    D.launch("shade_primary",
+             [&]
+             {
+                 k_shade_primary<<<blocks(n_pri, 256), 256, 0, s>>>(R.jobs.p, Q[0], R.hits.p, (uint32_t)n_pri, Q[1],
+                                                                    qcount, R.records.p);
+             });
+    uint32_t n = 0;
+    read_count(D, qcount, n);
+    int cur = 1;
+    while (n > 0)
+        {
+        if (n <= TAIL_RAYS)
+            {
+            D.launch("finish",
+                     [&]
+                     {
+                         k_finish<TRAV_BLOCK, TRAV_SS><<<blocks(n, TRAV_BLOCK), TRAV_BLOCK, 0, s>>>(
+                             R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p, TAIL_RAYS, ctr);
+                     });
+            break;
+            }
+        traverse(Q[cur], n);
+        HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
+        D.launch("shade",
+                 [&]
+                 {
+                     k_shade<<<blocks(n, 256), 256, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, n, Q[1 - cur], qcount,
+                                                            R.records.p, MAX_DEPTH, ctr);
+                 });
+        read_count(D, qcount, n);
+        cur = 1 - cur;
+        }
+    D.launch("accumulate",
+             [&]
+             {
+                 k_accumulate<<<blocks(n_pix, 256), 256, 0, s>>>(R.jobs.p, R.pix_base.p, nj, (uint32_t)n_pix,
+                                                                 R.records.p);
+             });
+    HIPCHECK(hipStreamSynchronize(s));
+    }
+
+// Render `samples` Path passes for each (scene, tracer) pair.
+static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, const std::vector<TracerObj*>& tracers,
+                        uint32_t samples)
+    {
+    std::lock_guard<std::mutex> lock(D.mu);
+    D.begin_stats();
+    for (size_t k = 0; k < scenes.size(); k++)
+        {
+        REQUIRE(scenes[k]->dev == tracers[k]->dev, "Scene and Tracer devices do not match");
+        REQUIRE(tracers[k]->type == T_PATH, "not a Path tracer");
+        }
+    // a scene appearing twice is synchronized once
+    std::vector<SceneObj*> uniq;
+    for (auto* sc : scenes)
+        if (std::find(uniq.begin(), uniq.end(), sc) == uniq.end())
+            uniq.push_back(sc);
+    for (auto* sc : uniq)
+        sync_scene(*sc);
+    uint64_t per_sample = 0;
+    for (auto* tr : tracers)
+        per_sample += (uint64_t)tr->W * tr->H * tr->L;
+    uint32_t K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(samples, g_batch_paths / std::max<uint64_t>(per_sample, 1)));
+    uint32_t done = 0;
+    while (done < samples)
+        {
+        const uint32_t k = std::min(K, samples - done);
+        std::vector<JobDev> jobs;
+        for (size_t j = 0; j < scenes.size(); j++)
+            {
+            JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
+            J.K = k;
+            jobs.push_back(J);
+            }
+        run_batch(D, jobs);
+        for (auto* tr : tracers)
+            tr->n_samples += k;
+        done += k;
+        }
+    D.end_stats();
+    if (D.last_counters[7])
+        throw Error(FH_ERR_STATE, "traversal stack overflow");
+    }
+
+static void render_preview(DeviceObj& D, SceneObj& sc, TracerObj& tr)
+    {
+    std::lock_guard<std::mutex> lock(D.mu);
+    REQUIRE(sc.dev == tr.dev, "Scene and Tracer devices do not match");
+    D.begin_stats();
+    sync_scene(sc);
+    // direct tracers do not support depth of field (fresnel/cpu/TracerDirect.cc:38-40)
+    fh_user_camera cam = sc.camera;
+    cam.f_stop = __builtin_huge_valf();
+    JobDev J = make_job(sc, tr, cam);
+    J.K = 1;
+    RenderScratch& R = D.rs;
+    R.jobs.ensure(1);
+    HIPCHECK(hipMemcpyAsync(R.jobs.p, &J, sizeof(J), hipMemcpyHostToDevice, D.stream));
+    D.launch("preview", [&] { k_preview<<<blocks(J.npix, 64), 64, 0, D.stream>>>(R.jobs.p, tr.aa_n); });
+    D.end_stats();
+    }
+    } // namespace fh
+
+// ====================================================================== C ABI
+using namespace fh;
+
+struct fh_device_s
+    {
+    DeviceObj obj;
+    };
+struct fh_scene_s
+    {
+    SceneObj obj;
+    };
+struct fh_geometry_s
+    {
+    GeometryObj obj;
+    };
+struct fh_tracer_s
+    {
+    TracerObj obj;
+    };
+
+#define NONNULL(p)                                                                                   \
+    do                                                                                               \
+        {                                                                                            \
+        if (!(p))                                                                                    \
+            throw Error(FH_ERR_INVALID, "null handle or pointer: " #p);                              \
+        } while (0)
+
+static void copy_str(const std::string& s, char* buf, size_t cap)
+    {
+    if (!buf || cap == 0)
+        return;
+    size_t n = std::min(cap - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+    }
+
+static std::string gpu_list()
+    {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return "";
+    std::string out;
+    for (int i = 0; i < n; i++)
+        {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, i) != hipSuccess)
+            continue;
+        out += "[" + std::to_string(i) + "]: " + p.name + " (" + p.gcnArchName + ") "
+               + std::to_string(p.multiProcessorCount) + " CUs, "
+               + std::to_string((unsigned long long)(p.totalGlobalMem >> 20)) + " MiB\n";
+        }
+    return out;
+    }
+
+extern "C"
+    {
+    const char* fh_last_error(void)
+        {
+        return g_last_error.c_str();
+        }
+
+    int fh_get_num_available_devices(int* n)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(n);
+                int c = 0;
+                if (hipGetDeviceCount(&c) != hipSuccess)
+                    c = 0;
+                *n = c;
+            });
+        }
+
+    int fh_device_create(int limit, fh_device* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                int c = 0;
+                if (hipGetDeviceCount(&c) != hipSuccess || c == 0)
+                    throw Error(FH_ERR_HIP, "No GPUs are available");
+                auto d = std::make_unique<fh_device_s>();
+                DeviceObj& D = d->obj;
+                D.limit = limit;
+                D.hip_dev = 0;
+                HIPCHECK(hipSetDevice(D.hip_dev));
+                hipDeviceProp_t p;
+                HIPCHECK(hipGetDeviceProperties(&p, D.hip_dev));
+                D.n_cu = p.multiProcessorCount;
+                D.name = std::string(p.name) + " (" + p.gcnArchName + ")";
+                HIPCHECK(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
+                int per_cu = 0;
+                HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, TRAV_SS, false>), TRAV_BLOCK, 0));
+                D.trav_grid = (uint32_t)std::max(1, per_cu) * (uint32_t)D.n_cu;
+                void* hp = nullptr;
+                HIPCHECK(hipHostMalloc(&hp, 64, hipHostMallocDefault));
+                D.h_pinned = (uint32_t*)hp;
+                D.counters.ensure(8);
+                HIPCHECK(hipMemset(D.counters.p, 0, 8 * sizeof(unsigned long long)));
+                *out = d.release();
+            });
+        }
+
+    int fh_device_destroy(fh_device dev)
+        {
+        return guard(
+            [&]
+            {
+                if (!dev)
+                    return;
+                (void)hipStreamSynchronize(dev->obj.stream);
+                for (auto& t : dev->obj.times)
+                    {
+                    (void)hipEventDestroy(t.a);
+                    (void)hipEventDestroy(t.b);
+                    }
+                for (auto e : dev->obj.event_pool)
+                    (void)hipEventDestroy(e);
+                if (dev->obj.h_pinned)
+                    (void)hipHostFree(dev->obj.h_pinned);
+                hipStream_t s = dev->obj.stream;
+                delete dev;
+                if (s)
+                    (void)hipStreamDestroy(s);
+            });
+        }
+
+    int fh_device_describe(fh_device dev, char* buf, size_t cap)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                copy_str("Enabled HIP devices:\n[0]: " + dev->obj.name + ", " + std::to_string(dev->obj.n_cu) + " CUs\n",
+                         buf, cap);
+            });
+        }
+
+    int fh_device_get_all_gpus(char* buf, size_t cap)
+        {
+        return guard([&] { copy_str(gpu_list(), buf, cap); });
+        }
+
+    // ---------------------------------------------------------------- scene
+    int fh_scene_create(fh_device dev, fh_scene* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(out);
+                auto s = std::make_unique<fh_scene_s>();
+                SceneObj& S = s->obj;
+                S.dev = &dev->obj;
+                // UserCamera() and the default lights of fresnel/cpu/Scene.cc:15-27
+                S.camera.position = {0, 0, 0};
+                S.camera.look_at = {0, 0, 1};
+                S.camera.up = {0, 1, 0};
+                S.camera.h = 1;
+                S.camera.f = 1;
+                S.camera.f_stop = 1;
+                S.camera.focus_distance = 1;
+                S.camera.model = 0;
+                std::memset(&S.lights, 0, sizeof(S.lights));
+                S.lights.N = 2;
+                S.lights.direction[0] = {-1, 0.3f, 1};
+                S.lights.color[0] = {1, 1, 1};
+                S.lights.direction[1] = {1, 0, 1};
+                S.lights.color[1] = {0.1f, 0.1f, 0.1f};
+                *out = s.release();
+            });
+        }
+    int fh_scene_destroy(fh_scene scene)
+        {
+        return guard(
+            [&]
+            {
+                if (!scene)
+                    return;
+                for (auto& kv : scene->obj.geoms)
+                    kv.second->scene = nullptr;
+                (void)hipStreamSynchronize(scene->obj.dev->stream);
+                delete scene;
+            });
+        }
+    int fh_scene_set_camera(fh_scene scene, const fh_user_camera* cam)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                NONNULL(cam);
+                scene->obj.camera = *cam;
+            });
+        }
+    int fh_scene_get_camera(fh_scene scene, fh_user_camera* cam)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                NONNULL(cam);
+                *cam = scene->obj.camera;
+            });
+        }
+    int fh_scene_set_background(fh_scene scene, fh_rgb color, float alpha)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                scene->obj.bg = color;
+                scene->obj.bg_alpha = alpha;
+            });
+        }
+    int fh_scene_get_background(fh_scene scene, fh_rgb* color, float* alpha)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                if (color)
+                    *color = scene->obj.bg;
+                if (alpha)
+                    *alpha = scene->obj.bg_alpha;
+            });
+        }
+    int fh_scene_set_lights(fh_scene scene, const fh_lights* lights)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                NONNULL(lights);
+                scene->obj.lights = *lights;
+            });
+        }
+    int fh_scene_get_lights(fh_scene scene, fh_lights* lights)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                NONNULL(lights);
+                *lights = scene->obj.lights;
+            });
+        }
+
+    // ---------------------------------------------------------------- geometry
+    static GeometryObj& new_geometry(fh_scene scene, uint32_t type, uint32_t N, std::unique_ptr<fh_geometry_s>& h)
+        {
+        NONNULL(scene);
+        h = std::make_unique<fh_geometry_s>();
+        GeometryObj& g = h->obj;
+        g.scene = &scene->obj;
+        g.type = type;
+        g.N = N;
+        g.slot = scene->obj.alloc_slot();
+        // defaults of the reference geometry constructors (e.g. cpu/GeometrySphere.cc:29-31)
+        g.mat = make_material(1, 0, 1, 0.0f);
+        g.outline = make_material(0, 0, 0, 1.0f);
+        g.outline_width = 0;
+        return g;
+        }
+    static void attach(fh_scene scene, std::unique_ptr<fh_geometry_s>& h, fh_geometry* out)
+        {
+        scene->obj.geoms[h->obj.slot] = &h->obj;
+        scene->obj.bvh_valid = false;
+        *out = h.release();
+        }
+
+    int fh_geometry_sphere_create(fh_scene scene, uint32_t N, fh_geometry* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                std::unique_ptr<fh_geometry_s> h;
+                GeometryObj& g = new_geometry(scene, G_SPHERE, N, h);
+                g.position.assign(3 * (size_t)N, 0.0f);
+                g.radius_buf.assign(N, 0.0f);
+                g.color.assign(3 * (size_t)N, 0.0f);
+                attach(scene, h, out);
+            });
+        }
+    int fh_geometry_cylinder_create(fh_scene scene, uint32_t N, fh_geometry* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                std::unique_ptr<fh_geometry_s> h;
+                GeometryObj& g = new_geometry(scene, G_CYLINDER, N, h);
+                g.points.assign(6 * (size_t)N, 0.0f);
+                g.radius_buf.assign(N, 0.0f);
+                g.color.assign(6 * (size_t)N, 0.0f);
+                attach(scene, h, out);
+            });
+        }
+    int fh_geometry_polygon_create(fh_scene scene, const float* vertices, uint32_t n_verts, float rounding_radius,
+                                   uint32_t N, fh_geometry* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                REQUIRE(n_verts >= 1 && vertices, "vertices must be a Nvert by 2 array");
+                std::unique_ptr<fh_geometry_s> h;
+                GeometryObj& g = new_geometry(scene, G_POLYGON, N, h);
+                g.verts.assign(vertices, vertices + 2 * (size_t)n_verts);
+                g.rounding = rounding_radius;
+                // xy radius, fresnel/cpu/GeometryPolygon.cc:55-65
+                float r = 0;
+                for (uint32_t k = 0; k < n_verts; k++)
+                    {
+                    v2 p0;
+                    p0.x = vertices[2 * k];
+                    p0.y = vertices[2 * k + 1];
+                    r = std::max(r, sqrtf(dot(p0, p0)));
+                    }
+                g.radius = r + rounding_radius;
+                g.position.assign(2 * (size_t)N, 0.0f);
+                g.angle.assign(N, 0.0f);
+                g.color.assign(3 * (size_t)N, 0.0f);
+                attach(scene, h, out);
+            });
+        }
+    int fh_geometry_convex_polyhedron_create(fh_scene scene, const float* origins, const float* normals,
+                                             const float* colors, uint32_t n_planes, uint32_t N, float radius,
+                                             fh_geometry* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                REQUIRE(origins && normals && colors && n_planes >= 1, "polyhedron planes are required");
+                std::unique_ptr<fh_geometry_s> h;
+                GeometryObj& g = new_geometry(scene, G_POLYHEDRON, N, h);
+                for (uint32_t k = 0; k < n_planes; k++)
+                    {
+                    // n = n / sqrtf(dot(n, n)), cpu/GeometryConvexPolyhedron.cc:85-86
+                    v3 n = mk3(normals[3 * k], normals[3 * k + 1], normals[3 * k + 2]);
+                    n = div_rcp(n, sqrtf(dot(n, n)));
+                    g.plane_n.insert(g.plane_n.end(), {n.x, n.y, n.z});
+                    g.plane_p.insert(g.plane_p.end(), {origins[3 * k], origins[3 * k + 1], origins[3 * k + 2]});
+                    g.plane_color.insert(g.plane_color.end(), {colors[3 * k], colors[3 * k + 1], colors[3 * k + 2]});
+                    }
+                g.radius = radius;
+                g.position.assign(3 * (size_t)N, 0.0f);
+                g.orientation.assign(4 * (size_t)N, 0.0f);
+                g.color.assign(3 * (size_t)N, 0.0f);
+                attach(scene, h, out);
+            });
+        }
+    int fh_geometry_mesh_create(fh_scene scene, const float* vertices, uint32_t n_verts, uint32_t N, fh_geometry* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                REQUIRE(vertices && n_verts >= 3, "vertices must be a Nvert by 3 array");
+                REQUIRE(n_verts % 3 == 0, "the number of triangle vertices must be a multiple of three.");
+                std::unique_ptr<fh_geometry_s> h;
+                GeometryObj& g = new_geometry(scene, G_MESH, N, h);
+                g.verts.assign(vertices, vertices + 3 * (size_t)n_verts);
+                g.n_faces = n_verts / 3;
+                g.position.assign(3 * (size_t)N, 0.0f);
+                g.orientation.assign(4 * (size_t)N, 0.0f);
+                g.color.assign(3 * (size_t)n_verts, 0.0f);
+                attach(scene, h, out);
+            });
+        }
+    int fh_geometry_destroy(fh_geometry geom)
+        {
+        return guard(
+            [&]
+            {
+                if (!geom)
+                    return;
+                GeometryObj& g = geom->obj;
+                if (g.scene)
+                    {
+                    (void)hipStreamSynchronize(g.scene->dev->stream);
+                    if (g.valid)
+                        {
+                        g.scene->geoms.erase(g.slot);
+                        g.scene->bvh_valid = false;
+                        }
+                    }
+                delete geom;
+            });
+        }
+
+    int fh_geometry_buffer(fh_geometry geom, int which, float** data, uint32_t* ndim, uint64_t shape[3])
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(data);
+                NONNULL(ndim);
+                NONNULL(shape);
+                GeometryObj& g = geom->obj;
+                std::vector<float>* v = nullptr;
+                uint32_t nd = 2;
+                uint64_t s0 = g.N, s1 = 3, s2 = 0;
+                switch (which)
+                    {
+                case FH_BUF_POSITION:
+                    REQUIRE(g.type != G_CYLINDER, "geometry has no position buffer");
+                    v = &g.position;
+                    s1 = (g.type == G_POLYGON) ? 2 : 3;
+                    break;
+                case FH_BUF_RADIUS:
+                    REQUIRE(g.type == G_SPHERE || g.type == G_CYLINDER, "geometry has no radius buffer");
+                    v = &g.radius_buf;
+                    nd = 1;
+                    break;
+                case FH_BUF_COLOR:
+                    v = &g.color;
+                    if (g.type == G_CYLINDER)
+                        {
+                        nd = 3;
+                        s1 = 2;
+                        s2 = 3;
+                        }
+                    else if (g.type == G_MESH)
+                        s0 = (uint64_t)g.n_faces * 3;
+                    break;
+                case FH_BUF_POINTS:
+                    REQUIRE(g.type == G_CYLINDER, "geometry has no points buffer");
+                    v = &g.points;
+                    nd = 3;
+                    s1 = 2;
+                    s2 = 3;
+                    break;
+                case FH_BUF_ORIENTATION:
+                    REQUIRE(g.type == G_POLYHEDRON || g.type == G_MESH, "geometry has no orientation buffer");
+                    v = &g.orientation;
+                    s1 = 4;
+                    break;
+                case FH_BUF_ANGLE:
+                    REQUIRE(g.type == G_POLYGON, "geometry has no angle buffer");
+                    v = &g.angle;
+                    nd = 1;
+                    break;
+                default:
+                    throw Error(FH_ERR_INVALID, "unknown buffer id");
+                    }
+                *data = v->data();
+                *ndim = nd;
+                shape[0] = s0;
+                shape[1] = s1;
+                shape[2] = s2;
+            });
+        }
+    int fh_geometry_update(fh_geometry geom)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                geom->obj.dirty = true;
+            });
+        }
+#define GEOM_VALID(g)                                                                                \
+    do                                                                                               \
+        {                                                                                            \
+        if (!(g).valid || !(g).scene)                                                                \
+            throw Error(FH_ERR_STATE, "Cannot modify a removed Geometry");                           \
+        } while (0)
+
+    int fh_geometry_set_material(fh_geometry geom, const fh_material* m)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(m);
+                geom->obj.mat = *m;
+            });
+        }
+    int fh_geometry_get_material(fh_geometry geom, fh_material* m)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(m);
+                *m = geom->obj.mat;
+            });
+        }
+    int fh_geometry_set_outline_material(fh_geometry geom, const fh_material* m)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(m);
+                geom->obj.outline = *m;
+            });
+        }
+    int fh_geometry_get_outline_material(fh_geometry geom, fh_material* m)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(m);
+                *m = geom->obj.outline;
+            });
+        }
+    int fh_geometry_set_outline_width(fh_geometry geom, float w)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                geom->obj.outline_width = w;
+            });
+        }
+    int fh_geometry_get_outline_width(fh_geometry geom, float* w)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(w);
+                *w = geom->obj.outline_width;
+            });
+        }
+    int fh_geometry_enable(fh_geometry geom)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                if (!geom->obj.valid || !geom->obj.scene)
+                    throw Error(FH_ERR_STATE, "Cannot enable inactive Geometry");
+                geom->obj.enabled = true;
+            });
+        }
+    int fh_geometry_disable(fh_geometry geom)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                if (!geom->obj.valid || !geom->obj.scene)
+                    throw Error(FH_ERR_STATE, "Cannot disable inactive Geometry");
+                geom->obj.enabled = false;
+            });
+        }
+    int fh_geometry_remove(fh_geometry geom)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                GeometryObj& g = geom->obj;
+                if (g.valid && g.scene)
+                    {
+                    (void)hipStreamSynchronize(g.scene->dev->stream);
+                    g.scene->geoms.erase(g.slot);
+                    g.scene->bvh_valid = false;
+                    }
+                g.valid = false;
+            });
+        }
+    int fh_geometry_set_color_by_face(fh_geometry geom, float f)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                geom->obj.color_by_face = f;
+            });
+        }
+    int fh_geometry_get_color_by_face(fh_geometry geom, float* f)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(f);
+                *f = geom->obj.color_by_face;
+            });
+        }
+    int fh_geometry_get_radius(fh_geometry geom, float* r)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                NONNULL(r);
+                *r = geom->obj.radius;
+            });
+        }
+
+    // ---------------------------------------------------------------- tracers
+    int fh_tracer_path_create(fh_device dev, uint32_t w, uint32_t h, uint32_t light_samples, fh_tracer* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(out);
+                auto t = std::make_unique<fh_tracer_s>();
+                TracerObj& T = t->obj;
+                T.dev = &dev->obj;
+                T.type = T_PATH;
+                T.L = light_samples;
+                T.resize(w, h);
+                // TracerPath ctor -> reset(): sample count 0, seed 0 -> 1
+                T.n_samples = 0;
+                T.seed++;
+                *out = t.release();
+            });
+        }
+    int fh_tracer_direct_create(fh_device dev, uint32_t w, uint32_t h, fh_tracer* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(out);
+                auto t = std::make_unique<fh_tracer_s>();
+                TracerObj& T = t->obj;
+                T.dev = &dev->obj;
+                T.type = T_DIRECT;
+                T.aa_n = 8;
+                T.resize(w, h);
+                *out = t.release();
+            });
+        }
+    int fh_tracer_destroy(fh_tracer tr)
+        {
+        return guard(
+            [&]
+            {
+                if (!tr)
+                    return;
+                (void)hipStreamSynchronize(tr->obj.dev->stream);
+                delete tr;
+            });
+        }
+    int fh_tracer_resize(fh_tracer tr, uint32_t w, uint32_t h)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                TracerObj& T = tr->obj;
+                T.resize(w, h);
+                if (T.type == T_PATH)
+                    {
+                    // cpu/TracerPath.h:48-53
+                    T.n_samples = 0;
+                    T.seed++;
+                    }
+            });
+        }
+    int fh_tracer_get_size(fh_tracer tr, uint32_t* w, uint32_t* h)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                if (w)
+                    *w = tr->obj.W;
+                if (h)
+                    *h = tr->obj.H;
+            });
+        }
+    int fh_tracer_render(fh_tracer tr, fh_scene scene)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(scene);
+                if (tr->obj.type == T_DIRECT)
+                    render_preview(*tr->obj.dev, scene->obj, tr->obj);
+                else
+                    render_path(*tr->obj.dev, {&scene->obj}, {&tr->obj}, 1);
+            });
+        }
+    int fh_tracer_render_samples(fh_tracer tr, fh_scene scene, uint32_t samples)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(scene);
+                REQUIRE(tr->obj.type == T_PATH, "render_samples needs a Path tracer");
+                if (samples)
+                    render_path(*tr->obj.dev, {&scene->obj}, {&tr->obj}, samples);
+            });
+        }
+    int fh_tracer_map_output(fh_tracer tr, int which, void** data)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(data);
+                TracerObj& T = tr->obj;
+                const size_t n = (size_t)T.W * T.H;
+                if (which == FH_OUT_SRGB)
+                    {
+                    HIPCHECK(hipMemcpy(T.h_srgb.data(), T.srgb.p, n * 4, hipMemcpyDeviceToHost));
+                    *data = T.h_srgb.data();
+                    }
+                else if (which == FH_OUT_LINEAR)
+                    {
+                    HIPCHECK(hipMemcpy(T.h_linear.data(), T.linear.p, n * 16, hipMemcpyDeviceToHost));
+                    *data = T.h_linear.data();
+                    }
+                else
+                    throw Error(FH_ERR_INVALID, "unknown output buffer");
+            });
+        }
+    int fh_tracer_enable_highlight_warning(fh_tracer tr, fh_rgb color)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                tr->obj.highlight = true;
+                tr->obj.hw_color = color;
+            });
+        }
+    int fh_tracer_disable_highlight_warning(fh_tracer tr)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                tr->obj.highlight = false;
+            });
+        }
+    int fh_tracer_set_seed(fh_tracer tr, uint32_t seed)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                tr->obj.seed = seed;
+            });
+        }
+    int fh_tracer_get_seed(fh_tracer tr, uint32_t* seed)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(seed);
+                *seed = tr->obj.seed;
+            });
+        }
+    int fh_tracer_path_reset(fh_tracer tr)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                TracerObj& T = tr->obj;
+                T.n_samples = 0;
+                T.seed++;
+                T.clear();
+            });
+        }
+    int fh_tracer_path_set_light_samples(fh_tracer tr, uint32_t light_samples)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                REQUIRE(light_samples >= 1 && light_samples < 65536, "light_samples must be in [1, 65535]");
+                tr->obj.L = light_samples;
+            });
+        }
+    int fh_tracer_path_get_num_samples(fh_tracer tr, uint32_t* n)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(n);
+                *n = tr->obj.n_samples;
+            });
+        }
+    int fh_tracer_direct_set_aa_n(fh_tracer tr, uint32_t n)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                REQUIRE(n >= 1, "aa_n must be >= 1");
+                tr->obj.aa_n = n;
+            });
+        }
+    int fh_tracer_direct_get_aa_n(fh_tracer tr, uint32_t* n)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(n);
+                *n = tr->obj.aa_n;
+            });
+        }
+
+    // ---------------------------------------------------------------- batching
+    int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t samples)
+        {
+        return guard(
+            [&]
+            {
+                REQUIRE(n >= 1 && scenes && tracers, "empty batch");
+                std::vector<SceneObj*> sc;
+                std::vector<TracerObj*> tr;
+                for (uint32_t k = 0; k < n; k++)
+                    {
+                    NONNULL(scenes[k]);
+                    NONNULL(tracers[k]);
+                    sc.push_back(&scenes[k]->obj);
+                    tr.push_back(&tracers[k]->obj);
+                    for (uint32_t q = 0; q < k; q++)
+                        REQUIRE(tr[q] != tr.back(), "a tracer appears twice in one batch");
+                    }
+                if (samples)
+                    render_path(*tr[0]->dev, sc, tr, samples);
+            });
+        }
+
+    // ---------------------------------------------------------------- diagnostics
+    int fh_debug_philox(fh_device dev, const uint32_t* in, uint32_t n, uint32_t* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(in);
+                NONNULL(out);
+                DeviceObj& D = dev->obj;
+                DBuf<uint32_t> din, dout;
+                din.ensure(6 * (size_t)n);
+                dout.ensure(4 * (size_t)n);
+                HIPCHECK(hipMemcpy(din.p, in, 24 * (size_t)n, hipMemcpyHostToDevice));
+                k_debug_philox<<<blocks(n, 256), 256, 0, D.stream>>>(din.p, n, dout.p);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipStreamSynchronize(D.stream));
+                HIPCHECK(hipMemcpy(out, dout.p, 16 * (size_t)n, hipMemcpyDeviceToHost));
+            });
+        }
+    int fh_debug_primary(fh_tracer tr, fh_scene scene, uint32_t sample, float* rays, uint32_t* geom_id, uint32_t* prim,
+                         float* t)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(scene);
+                DeviceObj& D = *tr->obj.dev;
+                std::lock_guard<std::mutex> lock(D.mu);
+                sync_scene(scene->obj);
+                JobDev J = make_job(scene->obj, tr->obj, scene->obj.camera);
+                J.K = 1;
+                const uint32_t npix = J.npix;
+                DBuf<JobDev> dj;
+                dj.ensure(1);
+                DBuf<float> drays;
+                drays.ensure(6 * (size_t)npix);
+                DBuf<uint4> dhits;
+                dhits.ensure(npix);
+                HIPCHECK(hipMemcpy(dj.p, &J, sizeof(J), hipMemcpyHostToDevice));
+                k_debug_primary<<<blocks(npix, 64), 64, 0, D.stream>>>(dj.p, sample, drays.p, dhits.p);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipStreamSynchronize(D.stream));
+                std::vector<uint4> h(npix);
+                HIPCHECK(hipMemcpy(h.data(), dhits.p, npix * sizeof(uint4), hipMemcpyDeviceToHost));
+                if (rays)
+                    HIPCHECK(hipMemcpy(rays, drays.p, 24 * (size_t)npix, hipMemcpyDeviceToHost));
+                for (uint32_t p = 0; p < npix; p++)
+                    {
+                    if (h[p].w)
+                        throw Error(FH_ERR_STATE, "traversal stack overflow");
+                    if (geom_id)
+                        geom_id[p] = h[p].y;
+                    if (prim)
+                        prim[p] = h[p].z;
+                    if (t)
+                        std::memcpy(&t[p], &h[p].x, 4);
+                    }
+            });
+        }
+    int fh_debug_trace(fh_scene scene, const float* rays, uint32_t n, float tnear, uint32_t* geom_id, uint32_t* prim,
+                       float* t)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                NONNULL(rays);
+                DeviceObj& D = *scene->obj.dev;
+                std::lock_guard<std::mutex> lock(D.mu);
+                sync_scene(scene->obj);
+                DBuf<float> drays;
+                drays.ensure(6 * (size_t)n);
+                DBuf<uint4> dhits;
+                dhits.ensure(n);
+                HIPCHECK(hipMemcpy(drays.p, rays, 24 * (size_t)n, hipMemcpyHostToDevice));
+                k_debug_trace<<<blocks(n, 64), 64, 0, D.stream>>>(scene_dev(scene->obj), drays.p, n, tnear, dhits.p);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipStreamSynchronize(D.stream));
+                std::vector<uint4> h(n);
+                HIPCHECK(hipMemcpy(h.data(), dhits.p, n * sizeof(uint4), hipMemcpyDeviceToHost));
+                for (uint32_t p = 0; p < n; p++)
+                    {
+                    if (h[p].w)
+                        throw Error(FH_ERR_STATE, "traversal stack overflow");
+                    if (geom_id)
+                        geom_id[p] = h[p].y;
+                    if (prim)
+                        prim[p] = h[p].z;
+                    if (t)
+                        std::memcpy(&t[p], &h[p].x, 4);
+                    }
+            });
+        }
+    int fh_stats_enable(fh_device dev, int enable)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                dev->obj.stats_on = enable != 0;
+            });
+        }
+    int fh_stats_last(fh_device dev, char* names, size_t names_cap, double* ms, uint32_t* launches, uint32_t cap,
+                      uint32_t* n_kernels)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                std::string all;
+                uint32_t k = 0;
+                for (auto& s : dev->obj.last_stats)
+                    {
+                    if (k < cap)
+                        {
+                        if (ms)
+                            ms[k] = s.second.first;
+                        if (launches)
+                            launches[k] = s.second.second;
+                        }
+                    all += s.first + "\n";
+                    k++;
+                    }
+                copy_str(all, names, names_cap);
+                if (n_kernels)
+                    *n_kernels = k;
+            });
+        }
+    int fh_counters_enable(fh_device dev, int enable)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                dev->obj.count_on = enable != 0;
+            });
+        }
+    int fh_counters_last(fh_device dev, uint64_t out[8])
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(out);
+                for (int k = 0; k < 8; k++)
+                    out[k] = dev->obj.last_counters[k];
+            });
+        }
+    }

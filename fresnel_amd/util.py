@@ -64,6 +64,37 @@ class ImageArray(Array):
         return f.getvalue()
 
 
+def _face_corners(pts, n, tol):
+    """Corners of the convex polygon spanned by coplanar points (monotone chain).
+
+    Points inside the face or on its edges are not corners, as in a convex
+    hull's vertex list.
+    """
+    a = numpy.cross(n, [1.0, 0.0, 0.0])
+    if numpy.linalg.norm(a) < 0.5:
+        a = numpy.cross(n, [0.0, 1.0, 0.0])
+    a /= numpy.linalg.norm(a)
+    b = numpy.cross(n, a)
+    q = numpy.stack([pts @ a, pts @ b], axis=1)
+    order = numpy.lexsort((q[:, 1], q[:, 0]))
+
+    def cross(o, p, r):
+        return (p[0] - o[0]) * (r[1] - o[1]) - (p[1] - o[1]) * (r[0] - o[0])
+
+    def half(idx):
+        h = []
+        for i in idx:
+            while len(h) >= 2 and cross(q[h[-2]], q[h[-1]], q[i]) <= tol:
+                h.pop()
+            h.append(i)
+        return h
+
+    lower = half(order)
+    upper = half(order[::-1])
+    corners = lower[:-1] + upper[:-1]
+    return pts[corners]
+
+
 def convex_polyhedron_from_vertices(vertices):
     """Build the plane description of the convex hull of ``vertices``.
 
@@ -105,10 +136,10 @@ def convex_polyhedron_from_vertices(vertices):
     normals = []
     sides = []
     for n, off, on in faces:
-        pts = v[sorted(on)]
+        pts = _face_corners(v[sorted(on)], n, tol)
         origins.append(numpy.mean(pts, axis=0))
         normals.append(n)
-        sides.append(len(on))
+        sides.append(len(pts))
     radius = float(numpy.max(numpy.linalg.norm(v, axis=1)))
     return dict(face_origin=numpy.array(origins, dtype=numpy.float32),
                 face_normal=numpy.array(normals, dtype=numpy.float32),

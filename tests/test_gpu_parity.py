@@ -1,0 +1,188 @@
+"""GPU backend vs the CPU checker on the same inputs.
+
+Bars (stated per test):
+* Philox words, camera rays, primary hit ids and hit distances: bit-exact.
+  A primary hit id may differ only where the checker finds an exact tie in t,
+  and such pixels are counted.
+* Linear HDR image at equal seed and sample count: RMSE <= 1e-3 per channel.
+  The device evaluates cosf/sinf/powf with its own math library, so bounce
+  directions can differ from the host's in the last ulp; everything else is
+  the same IEEE arithmetic.
+* The GPU's own schedules (several samples per submission, several scenes in
+  one batch) are bit-identical to one-at-a-time rendering.
+"""
+
+import ctypes
+
+import numpy
+import pytest
+
+import fresnel_amd as fresnel
+import oracle
+from oracle import ptr
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+MISS = 0xffffffff
+
+
+def c1_scene(device, n=4096, w_seed=0):
+    """The small sphere scene: 4096 spheres in a cube of side 20, r=0.5, outline 0.1."""
+    rng = numpy.random.default_rng(w_seed)
+    pos = rng.uniform(-10, 10, (n, 3)).astype(numpy.float32)
+    s = fresnel.Scene(device, lights=fresnel.light.cloudy())
+    fresnel.geometry.Sphere(s, position=pos, radius=0.5, outline_width=0.1,
+                            material=fresnel.material.Material(color=fresnel.color.linear([0.1, 0.8, 0.1]),
+                                                               roughness=0.8, specular=0.2))
+    s.camera = fresnel.camera.Orthographic.fit(s)
+    return s
+
+
+PRIMARY_SCENES = {
+    'hex_sphere': scenes.hex_sphere,
+    'four_spheres': scenes.four_spheres,
+    'four_cylinders': scenes.four_cylinders,
+    'box': scenes.box,
+    'eight_polyhedra': scenes.eight_polyhedra,
+    'tetrahedra': scenes.tetrahedra,
+    'one_triangle': scenes.one_triangle,
+    'polygons': scenes.polygons,
+    'rounded_polygons': scenes.rounded_polygons,
+    'perspective_grid': scenes.perspective_grid,
+    'c1_spheres': c1_scene,
+}
+
+
+def test_philox_words(gpu_device):
+    """Device Philox4x32-10 words equal the checker's for random counters and keys."""
+    rng = numpy.random.default_rng(7)
+    inp = rng.integers(0, 2**32, size=(4096, 6), dtype=numpy.uint64).astype(numpy.uint32)
+    inp[:16] = 0
+    inp[16:32] = 0xffffffff
+    dev = gpu_device._device.debugPhilox(inp)
+    ref = numpy.zeros((4096, 4), dtype=numpy.uint32)
+    oracle.lib().oracle_philox(ptr(inp, ctypes.c_uint32), 4096, ptr(ref, ctypes.c_uint32))
+    numpy.testing.assert_array_equal(dev, ref)
+
+
+@pytest.mark.parametrize('name', sorted(PRIMARY_SCENES))
+def test_primary_rays_and_hits(name, gpu_device, cpu_device):
+    """Camera rays bit-exact; primary hit (geometry, primitive, t) bit-exact except counted exact ties."""
+    W, H = 96, 64
+    gs = PRIMARY_SCENES[name](gpu_device)
+    cs = PRIMARY_SCENES[name](cpu_device)
+    tr = fresnel.tracer.Path(gpu_device, w=W, h=H)
+    tr.seed = 12345
+    seed = tr.seed
+    for sample in (1, 7):
+        rays, gid, prim, t = tr._tracer.debugPrimary(gs._scene, sample)
+        h = cs._scene.handle()
+        L = oracle.lib()
+        ref_rays = numpy.zeros((W * H, 6), dtype=numpy.float32)
+        L.oracle_camera_rays(h, W, H, seed, sample, 1, ptr(ref_rays))
+        assert rays.view(numpy.uint32).tobytes() == ref_rays.view(numpy.uint32).tobytes(), \
+            f"camera rays differ in {numpy.sum(numpy.any(rays != ref_rays, axis=1))} pixels"
+        n = W * H
+        rg = numpy.zeros(n, numpy.uint32)
+        rp = numpy.zeros(n, numpy.uint32)
+        rt = numpy.zeros(n, numpy.float32)
+        L.oracle_trace(h, ptr(ref_rays), n, 1e-3, ptr(rg, ctypes.c_uint32), ptr(rp, ctypes.c_uint32), ptr(rt), None,
+                       0)
+        diff = (gid != rg) | (prim != rp)
+        if diff.any():
+            ties = numpy.zeros(n, numpy.uint32)
+            sel = numpy.ascontiguousarray(ref_rays[diff])
+            tsel = numpy.zeros(len(sel), numpy.uint32)
+            L.oracle_count_ties(h, ptr(sel), len(sel), 1e-3, ptr(tsel, ctypes.c_uint32))
+            assert numpy.all(tsel >= 2), f"{int(numpy.sum(tsel < 2))} primary hits differ without an exact tie"
+        same = ~diff
+        assert numpy.array_equal(t[same].view(numpy.uint32), rt[same].view(numpy.uint32)), "hit distances differ"
+        assert (gid != MISS).any(), "scene not visible"
+
+
+def _linear(device, builder, W, H, samples, L, seed=None):
+    s = builder(device)
+    t = fresnel.tracer.Path(device, w=W, h=H)
+    if seed is not None:
+        t.seed = seed
+    t.sample(s, samples=samples, light_samples=L, reset=seed is None)
+    return numpy.array(t.linear_output[:]), numpy.array(t.output[:])
+
+
+@pytest.mark.parametrize('name,samples,L', [('c1_spheres', 4, 2), ('hex_sphere', 8, 4), ('four_cylinders', 4, 2),
+                                            ('eight_polyhedra', 4, 2), ('tetrahedra', 4, 2),
+                                            ('perspective_grid', 4, 2)])
+def test_path_image_rmse(name, samples, L, gpu_device, cpu_device):
+    """Linear HDR image vs the checker at equal seed and samples: RMSE <= 1e-3."""
+    W, H = 96, 64
+    g, gs = _linear(gpu_device, PRIMARY_SCENES[name], W, H, samples, L)
+    c, cs = _linear(cpu_device, PRIMARY_SCENES[name], W, H, samples, L)
+    rmse = float(numpy.sqrt(numpy.mean((g - c)**2)))
+    exact = float(numpy.mean(numpy.all(g == c, axis=2)))
+    print(f"{name}: RMSE {rmse:.3e}, bit-identical pixels {exact:.4f}, "
+          f"max sRGB diff {int(numpy.max(numpy.abs(gs.astype(int) - cs.astype(int))))}")
+    assert numpy.all(numpy.isfinite(g))
+    assert rmse <= 1e-3
+
+
+def test_render_samples_equals_render_loop(gpu_device):
+    """S passes in one submission equal S render() calls, bit for bit."""
+    s = c1_scene(gpu_device)
+    a = fresnel.tracer.Path(gpu_device, w=80, h=60)
+    a.sample(s, samples=6, light_samples=3)
+    b = fresnel.tracer.Path(gpu_device, w=80, h=60)
+    b.reset()  # sample(reset=True) advanced a's seed the same way
+    b._tracer.setLightSamples(3)
+    for _ in range(6):
+        b._tracer.render(s._scene)
+    assert a._tracer.getNumSamples() == b._tracer.getNumSamples() == 6
+    assert numpy.array_equal(numpy.array(a.linear_output[:]), numpy.array(b.linear_output[:]))
+    assert numpy.array_equal(numpy.array(a.output[:]), numpy.array(b.output[:]))
+
+
+def test_batch_equals_plain(gpu_device):
+    """Several scenes of uneven size in one batch equal their plain renders, bit for bit."""
+    builders = [lambda d: c1_scene(d, n=1000, w_seed=1), scenes.four_cylinders, scenes.eight_polyhedra,
+                lambda d: c1_scene(d, n=5000, w_seed=2)]
+    sizes = [(64, 48), (80, 40), (50, 50), (96, 64)]
+    plain = []
+    for b, (w, h) in zip(builders, sizes):
+        plain.append(_linear(gpu_device, b, w, h, 5, 2)[0])
+    sc = [b(gpu_device) for b in builders]
+    trs = [fresnel.tracer.Path(gpu_device, w=w, h=h) for (w, h) in sizes]
+    for t in trs:
+        t.reset()
+        t._tracer.setLightSamples(2)
+    fresnel._gpu.batch_render([s._scene for s in sc], [t._tracer for t in trs], 5)
+    for k, t in enumerate(trs):
+        assert numpy.array_equal(numpy.array(t.linear_output[:]), plain[k]), f"scene {k} differs"
+
+
+def test_random_rays_vs_checker(gpu_device, cpu_device):
+    """Closest hits of random rays (many starting inside spheres) are bit-exact."""
+    rng = numpy.random.default_rng(3)
+
+    def build(dev):
+        s = fresnel.Scene(dev)
+        pos = rng_pos
+        fresnel.geometry.Sphere(s, position=pos, radius=rad)
+        return s
+
+    rng_pos = rng.uniform(-5, 5, (2000, 3)).astype(numpy.float32)
+    rad = rng.uniform(0.05, 0.6, 2000).astype(numpy.float32)
+    gs, cs = build(gpu_device), build(cpu_device)
+    n = 20000
+    o = rng.uniform(-6, 6, (n, 3)).astype(numpy.float32)
+    d = rng.normal(size=(n, 3))
+    d = (d / numpy.linalg.norm(d, axis=1, keepdims=True)).astype(numpy.float32)
+    rays = numpy.ascontiguousarray(numpy.concatenate([o, d], axis=1), dtype=numpy.float32)
+    gid, prim, t = gs._scene.debugTrace(rays, 1e-3)
+    rg = numpy.zeros(n, numpy.uint32)
+    rp = numpy.zeros(n, numpy.uint32)
+    rt = numpy.zeros(n, numpy.float32)
+    oracle.lib().oracle_trace(cs._scene.handle(), ptr(rays), n, 1e-3, ptr(rg, ctypes.c_uint32),
+                              ptr(rp, ctypes.c_uint32), ptr(rt), None, 0)
+    assert numpy.array_equal(gid, rg)
+    assert numpy.array_equal(prim, rp)
+    assert numpy.array_equal(t.view(numpy.uint32), rt.view(numpy.uint32))
