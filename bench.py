@@ -24,7 +24,6 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
 """
 
 import argparse
-import colorsys
 import json
 import math
 import os
@@ -169,6 +168,7 @@ def main():
     ap.add_argument('--workload', default='c2', choices=sorted(WORKLOADS))
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
+    ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -184,6 +184,13 @@ def main():
     tracer = fresnel.tracer.Path(dev, w=W, h=H)
     print(f"# scene built on host in {time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
     d = dev._device
+    tuned = None
+    if not args.no_tune:
+        from fresnel_amd import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.tune(dev, scene, W, H, samples=max(1, S // 16), light_samples=L, use_cache=False,
+                              log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        print(f"# autotune: {tuned} in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     d.statsEnable(True)
 
     for k in range(args.warmup):
@@ -251,6 +258,7 @@ def main():
                      "nodes_per_ray": round(nodes / max(rays, 1), 2),
                      "prims_per_ray": round(prims / max(rays, 1), 2)},
         "kernel_ms_last_step": kernel_ms,
+        "tuning": tuned,
         "capped_paths": int(ctr[4]),
     }
     if not args.no_cpu_baseline:

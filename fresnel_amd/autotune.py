@@ -1,0 +1,123 @@
+"""Launch-shape autotuner for the path tracer.
+
+The tunable shapes never change per-path arithmetic, so images are identical
+for every setting; only speed changes.  Knobs (see fh_device_tune):
+
+* ``stack_lds``   traversal stack entries kept in LDS per thread (8 or 16):
+                  fewer entries allow more wavefronts per CU, more entries
+                  spill less to the global overflow stack.
+* ``batch_paths`` light-sample paths per wavefront batch: larger batches
+                  amortise host synchronisation and the per-batch tail.
+* ``tail_rays``   live-ray count below which a batch finishes its remaining
+                  paths in one kernel.
+
+`tune` runs a coordinate descent over the candidates on a caller-provided
+scene, timing short renders, and stores the winner in a JSON cache keyed by
+GPU architecture and scene class (``$FRESNEL_AMD_TUNING`` or
+``~/.cache/fresnel_amd/tuning.json``).
+"""
+
+import json
+import os
+import time
+
+CANDIDATES = {
+    'stack_lds': [8, 16],
+    'batch_paths': [8 << 20, 32 << 20, 64 << 20],
+    'tail_rays': [8192, 32768, 131072],
+}
+
+
+def cache_path():
+    return os.environ.get('FRESNEL_AMD_TUNING',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'fresnel_amd', 'tuning.json'))
+
+
+def scene_class(scene, w, h):
+    """Key for the cache: geometry types, primitive count decade and image size."""
+    types = sorted({type(g).__name__ for g in scene.geometry})
+    n = 0
+    for g in scene.geometry:
+        try:
+            n += len(g.position[:]) if hasattr(g, 'position') else len(g.points[:])
+        except Exception:
+            pass
+    decade = len(str(max(n, 1))) - 1
+    return f"{'+'.join(types)}|1e{decade}|{w}x{h}"
+
+
+def arch(device):
+    """GPU name of the device's first entry in its description ("[0]: <name>, <n> CUs")."""
+    for line in device._device.describe().splitlines():
+        if line.startswith('[0]:'):
+            return line[4:].strip()
+    return 'unknown
+
+
+def apply(device, params):
+    """Set tuning knobs on a GPU device; returns the previous values."""
+    return {k: device._device.tune(k, int(v)) for k, v in params.items()}
+
+
+def current(device):
+    return {k: device._device.tune(k, 0) for k in CANDIDATES}
+
+
+def load(device, key):
+    try:
+        with open(cache_path()) as f:
+            return json.load(f).get(arch(device), {}).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def store(device, key, params):
+    path = cache_path()
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        data = {}
+    data.setdefault(arch(device), {})[key] = params
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, 'w') as f:
+        json.dump(data, f, indent=1)
+
+
+def tune(device, scene, w, h, samples=16, light_samples=1, repeats=2, use_cache=True, log=None):
+    """Pick the fastest knob values for rendering `scene` at w x h; applies and returns them."""
+    from . import tracer as _tracer
+    key = scene_class(scene, w, h)
+    if use_cache:
+        cached = load(device, key)
+        if cached:
+            apply(device, cached)
+            return dict(cached)
+    best = current(device)
+    t = _tracer.Path(device, w=w, h=h)
+
+    def timed(params):
+        apply(device, params)
+        t.sample(scene, samples=samples, light_samples=light_samples)  # warm queues at this size
+        dt = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            t.sample(scene, samples=samples, light_samples=light_samples)
+            dt.append(time.perf_counter() - t0)
+        return min(dt)
+
+    best_t = timed(best)
+    for k, values in CANDIDATES.items():
+        for v in values:
+            if v == best[k]:
+                continue
+            trial = dict(best)
+            trial[k] = v
+            dt = timed(trial)
+            if log:
+                log(f"autotune {key}: {trial} -> {dt * 1e3:.2f} ms")
+            if dt < best_t:
+                best, best_t = trial, dt
+    apply(device, best)
+    store(device, key, best)
+    return best

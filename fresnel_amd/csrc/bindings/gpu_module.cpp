@@ -10,6 +10,7 @@
 #include <pybind11/stl.h>
 
 #include <functional>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -370,6 +371,62 @@ struct TracerPath : Tracer
         }
     };
 
+// Render server: keeps the scene and tracer of each open request alive.
+struct Server
+    {
+    std::shared_ptr<Device> dev;
+    fh_server h = nullptr;
+    std::map<uint64_t, std::pair<std::shared_ptr<Scene>, std::shared_ptr<TracerPath>>> open;
+    Server(std::shared_ptr<Device> d, uint64_t budget, uint64_t target_paths) : dev(d)
+        {
+        check(fh_server_create(d->h, budget, target_paths, &h));
+        }
+    ~Server()
+        {
+        fh_server_destroy(h);
+        }
+    uint64_t submit(std::shared_ptr<Scene> s, std::shared_ptr<TracerPath> t, unsigned samples, unsigned L, double now)
+        {
+        if (s->dev.get() != dev.get() || t->dev.get() != dev.get())
+            throw std::runtime_error("Scene and Tracer devices do not match");
+        s->push();
+        uint64_t id;
+        check(fh_server_submit(h, s->h, t->h, samples, L, now, &id));
+        open[id] = {s, t};
+        return id;
+        }
+    py::tuple step(double now)
+        {
+        uint32_t a, q;
+        check(fh_server_step(h, now, &a, &q));
+        for (auto it = open.begin(); it != open.end();)
+            {
+            int st;
+            check(fh_server_poll(h, it->first, &st, nullptr, nullptr));
+            it = st >= 2 ? open.erase(it) : std::next(it);
+            }
+        return py::make_tuple(a, q);
+        }
+    py::tuple poll(uint64_t id)
+        {
+        int st;
+        uint32_t done;
+        double t[3];
+        check(fh_server_poll(h, id, &st, &done, t));
+        return py::make_tuple(st, done, py::make_tuple(t[0], t[1], t[2]));
+        }
+    void release(uint64_t id)
+        {
+        check(fh_server_release(h, id));
+        }
+    py::tuple memory()
+        {
+        uint64_t u, b;
+        check(fh_server_memory(h, &u, &b));
+        return py::make_tuple(u, b);
+        }
+    };
+
 PYBIND11_MODULE(_gpu, m)
     {
     py::module_::import("fresnel_amd._common");
@@ -413,6 +470,13 @@ PYBIND11_MODULE(_gpu, m)
                  uint64_t c[8];
                  check(fh_counters_last(d.h, c));
                  return std::vector<uint64_t>(c, c + 8);
+             })
+        .def("tune",
+             [](Device& d, const std::string& key, int64_t value)
+             {
+                 int64_t prev = 0;
+                 check(fh_device_tune(d.h, key.c_str(), value, &prev));
+                 return prev;
              })
         .def("debugPhilox",
              [](Device& d, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> in)
@@ -536,6 +600,14 @@ PYBIND11_MODULE(_gpu, m)
                  return py::make_tuple(rays, gid, prim, tt);
              });
 
+    py::class_<Server, std::shared_ptr<Server>>(m, "Server")
+        .def(py::init<std::shared_ptr<Device>, uint64_t, uint64_t>())
+        .def("submit", &Server::submit, py::arg("scene"), py::arg("tracer"), py::arg("samples"),
+             py::arg("light_samples") = 1, py::arg("now") = -1.0)
+        .def("step", &Server::step, py::arg("now") = -1.0)
+        .def("poll", &Server::poll)
+        .def("release", &Server::release)
+        .def("memory", &Server::memory);
     m.def("batch_render",
           [](std::vector<std::shared_ptr<Scene>> scenes, std::vector<std::shared_ptr<TracerPath>> tracers,
              unsigned samples)

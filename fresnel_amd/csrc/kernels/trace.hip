@@ -1,7 +1,8 @@
 // trace.hip -- wavefront path tracer kernels for gfx950.
 //
 // One camera-sample batch of a Path render runs as
-//   camgen -> traverse -> shade_primary -> { traverse -> shade }* -> finish -> accumulate
+//   traverse(primary) -> shade_primary -> { traverse -> shade }* -> finish -> accumulate
+// (primary rays are generated from their index where they are used)
 // over ping-pong SoA ray queues (float4 origin|tnear, float4 dir|job,
 // float4 attenuation, uint4 job|pixel|sample|depth).  shade appends the rays
 // that continue with one atomic per wavefront (ballot + popcount) and writes a
@@ -91,10 +92,30 @@ struct LocalStack
         }
     };
 
+constexpr uint32_t T_SPHERES = 1u << G_SPHERE;
+constexpr uint32_t T_ALL = 0x1fu;
+
+// Candidate distance of one primitive; TYPES lists the geometry types the
+// scene can hold, so a spheres-only traversal compiles without the other
+// intersectors (fewer registers, more wavefronts in flight).
+template<uint32_t TYPES>
+__device__ __forceinline__ bool leaf_test(const GeomDev& g, uint32_t prim, v3 o, v3 d, float tnear, float& t)
+    {
+    if constexpr (TYPES == T_SPHERES)
+        {
+        const float4 s = g.a[prim];
+        float de;
+        bool first;
+        return sphere_t(o, d, mk3(s.x, s.y, s.z), s.w, t, de, first) && (tnear < t);
+        }
+    else
+        return intersect_prim<false>(g, prim, o, d, tnear, t, nullptr);
+    }
+
 // Closest hit in the order (t, geometry, primitive) over the primitives whose
 // bounds pass the slab test with tmax >= tnear; a subtree is culled when its
 // entry distance exceeds the closest hit by the relative slack.
-template<class Stack, bool COUNT>
+template<class Stack, bool COUNT, uint32_t TYPES = T_ALL>
 __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear, HitT& best, Stack& st,
                                      uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
     {
@@ -126,7 +147,7 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
             float t;
             if (COUNT)
                 n_prims++;
-            if (intersect_prim<false>(sc.geoms[lk.z], prim, o, d, tnear, t, nullptr) && better(t, lk.z, prim, best))
+            if (leaf_test<TYPES>(sc.geoms[lk.z], prim, o, d, tnear, t) && better(t, lk.z, prim, best))
                 {
                 best.t = t;
                 best.geom = lk.z;
@@ -140,7 +161,7 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
             float t;
             if (COUNT)
                 n_prims++;
-            if (intersect_prim<false>(sc.geoms[lk.w], prim, o, d, tnear, t, nullptr) && better(t, lk.w, prim, best))
+            if (leaf_test<TYPES>(sc.geoms[lk.w], prim, o, d, tnear, t) && better(t, lk.w, prim, best))
                 {
                 best.t = t;
                 best.geom = lk.w;
@@ -239,36 +260,28 @@ struct Queue
     uint4* meta; // job, pixel, sample-in-batch | light sample << 16, depth
     };
 
-// ---------------------------------------------------------------- camgen
-// Primary ray of camera sample n_base + k + 1 for every pixel of every job
-// (fresnel/cpu/TracerPath.cc:100-136: tnear 1e-3).
-__global__ void k_camgen(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t total, Queue q)
-    {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total)
-        return;
-    const uint32_t j = find_job(jobs, n_jobs, i);
-    const JobDev& J = jobs[j];
-    const uint32_t local = i - J.ray_base;
-    const uint32_t k = local / J.npix;
-    const uint32_t pixel = local - k * J.npix;
-    const uint32_t py = pixel / J.W;
-    const uint32_t px = pixel - py * J.W;
-    v3 o, d;
-    camera_ray(J.cam, J.W, J.H, J.seed, px, py, J.n_base + k + 1, true, o, d);
-    q.o[i] = make_float4(o.x, o.y, o.z, 1e-3f);
-    q.d[i] = make_float4(d.x, d.y, d.z, __uint_as_float(j));
-    q.att[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    q.meta[i] = make_uint4(j, pixel, k, 0);
-    }
-
 // ---------------------------------------------------------------- traverse
 // Persistent threads: each wavefront takes 64 rays at a time from a counter
 // until the queue is exhausted.
-template<int BLOCK, int SS, bool COUNT>
-__global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ jobs, Queue q, uint32_t n, uint4* hits,
-                                                    uint32_t* work, uint2* gstack, uint32_t gstride,
-                                                    unsigned long long* counters)
+// Primary ray i of a batch: job, camera sample and pixel from the index
+// (fresnel/cpu/TracerPath.cc:100-136, tnear 1e-3).
+__device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
+                                            uint32_t& k, uint32_t& pixel, v3& o, v3& d)
+    {
+    job = find_job(jobs, n_jobs, i);
+    const JobDev& J = jobs[job];
+    const uint32_t local = i - J.ray_base;
+    k = local / J.npix;
+    pixel = local - k * J.npix;
+    const uint32_t py = pixel / J.W;
+    const uint32_t px = pixel - py * J.W;
+    camera_ray(J.cam, J.W, J.H, J.seed, px, py, J.n_base + k + 1, true, o, d);
+    }
+
+template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
+__global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
+                                                    uint32_t n, uint4* hits, uint32_t* work, uint2* gstack,
+                                                    uint32_t gstride, unsigned long long* counters)
     {
     __shared__ uint2 lds[SS * BLOCK];
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -291,12 +304,27 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
         const uint32_t i = base + lane;
         if (i < n)
             {
-            const float4 a = q.o[i];
-            const float4 b = q.d[i];
-            const SceneDev& sc = jobs[__float_as_uint(b.w)].scene;
+            v3 o, d;
+            float tnear;
+            uint32_t job;
+            if (PRIMARY)
+                {
+                uint32_t k, pixel;
+                primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+                tnear = 1e-3f;
+                }
+            else
+                {
+                const float4 a = q.o[i];
+                const float4 b = q.d[i];
+                o = mk3(a.x, a.y, a.z);
+                d = mk3(b.x, b.y, b.z);
+                tnear = a.w;
+                job = __float_as_uint(b.w);
+                }
+            const SceneDev& sc = jobs[job].scene;
             HitT h;
-            trace_closest<LdsStack<SS>, COUNT>(sc, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, h, st, n_nodes,
-                                               n_prims, err);
+            trace_closest<LdsStack<SS>, COUNT, TYPES>(sc, o, d, tnear, h, st, n_nodes, n_prims, err);
             hits[i] = make_uint4(__float_as_uint(h.t), h.geom, h.prim, 0);
             if (COUNT)
                 n_rays++;
@@ -324,8 +352,8 @@ __device__ __forceinline__ void hit_attributes(const SceneDev& sc, const HitT& h
 
 // Depth-0 shading: one primary hit spawns the L light-sample paths
 // (fresnel/cpu/TracerPath.cc:141-155 reuse the cached primary hit).
-__global__ void k_shade_primary(const JobDev* __restrict__ jobs, Queue in, const uint4* __restrict__ hits, uint32_t n,
-                                Queue out, uint32_t* out_count, uint4* records)
+__global__ void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs, const uint4* __restrict__ hits,
+                                uint32_t n, Queue out, uint32_t* out_count, uint4* records)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
@@ -334,22 +362,15 @@ __global__ void k_shade_primary(const JobDev* __restrict__ jobs, Queue in, const
     HitT h;
     h.geom = EMPTY;
     v3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
-    float tnear = 0;
+    const float tnear = 1e-3f;
     if (valid)
         {
-        const uint4 mt = in.meta[i];
-        job = mt.x;
-        pixel = mt.y;
-        k = mt.z;
+        // the primary ray is regenerated rather than stored (bit-identical)
+        primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
         const uint4 hh = hits[i];
         h.t = __uint_as_float(hh.x);
         h.geom = hh.y;
         h.prim = hh.z;
-        const float4 a = in.o[i];
-        const float4 b = in.d[i];
-        o = mk3(a.x, a.y, a.z);
-        d = mk3(b.x, b.y, b.z);
-        tnear = a.w;
         }
     const JobDev& J = jobs[job];
     if (valid)
@@ -464,7 +485,7 @@ __global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* 
     }
 
 // Tail: each thread finishes its path (trace + shade until termination).
-template<int BLOCK, int SS>
+template<int BLOCK, int SS, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ jobs, Queue in, uint32_t n,
                                                   uint4* records, uint32_t max_depth, uint2* gstack,
                                                   uint32_t gstride, unsigned long long* counters)
@@ -492,7 +513,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
     for (;;)
         {
         HitT h;
-        trace_closest<LdsStack<SS>, false>(J.scene, o, d, 1e-3f, h, st, dummy, dummy, err);
+        trace_closest<LdsStack<SS>, false, TYPES>(J.scene, o, d, 1e-3f, h, st, dummy, dummy, err);
         v3 no, nd;
         if (!shade_bounce(J, h, o, d, 1e-3f, mt.y, k, ls, depth, att, no, nd, rec))
             break;

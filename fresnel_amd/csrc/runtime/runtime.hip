@@ -6,7 +6,10 @@
 // and the sampling state (seed, sample count).  Every C entry point catches
 // exceptions and returns a status (include/fresnel_hip.h).
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <cmath>
+#include <deque>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -18,6 +21,7 @@
 #include "fresnel_hip.h"
 #include "kernels/bvh.hip"
 #include "kernels/trace.hip"
+#include "runtime/serve_policy.h"
 
 namespace fh
 {
@@ -119,7 +123,6 @@ static unsigned blocks(size_t n, unsigned b)
 // ====================================================================== device
 constexpr int TRAV_BLOCK = 128;
 constexpr int TRAV_SS = 16;
-constexpr uint32_t TAIL_RAYS = 32768;   // finish the remaining paths in one kernel below this
 constexpr uint32_t MAX_DEPTH = 4096;    // safety cap on path length (the reference has none)
 
 struct KernelTime
@@ -154,6 +157,21 @@ struct DeviceObj
     uint32_t* h_pinned = nullptr; // 16 uint32 of pinned host memory for counts
     bool stats_on = false;
     bool count_on = false;
+    uint64_t batch_paths = 32ull << 20; // light-sample paths per batch (sizes the queues)
+    uint32_t tail_rays = 32768;         // finish paths in one kernel below this many rays
+    int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
+    std::map<const void*, uint32_t> grids;
+    uint32_t grid_for(const void* kernel, int block)
+        {
+        auto it = grids.find(kernel);
+        if (it != grids.end())
+            return it->second;
+        int per_cu = 0;
+        HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+        const uint32_t g = (uint32_t)std::max(1, per_cu) * (uint32_t)n_cu;
+        grids[kernel] = g;
+        return g;
+        }
     std::vector<KernelTime> times;
     std::vector<hipEvent_t> event_pool;
     std::vector<std::pair<std::string, std::pair<double, uint32_t>>> last_stats;
@@ -229,6 +247,78 @@ struct DeviceObj
             last_counters[k] = c[k];
         }
     };
+
+
+// ====================================================================== traversal launch
+struct TravArgs
+    {
+    const JobDev* jobs;
+    uint32_t n_jobs;
+    Queue q;
+    uint32_t n;
+    uint4* hits;
+    uint32_t* work;
+    uint2* gstack;
+    uint32_t gstride;
+    unsigned long long* counters;
+    };
+
+template<int SS, bool C, bool P, uint32_t T>
+static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const TravArgs& a)
+    {
+    auto kern = &k_traverse<TRAV_BLOCK, SS, C, P, T>;
+    const uint32_t grid = D.grid_for(reinterpret_cast<const void*>(kern), TRAV_BLOCK);
+    D.launch(nm,
+             [&]
+             {
+                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.gstack, a.gstride,
+                                                 a.counters);
+             });
+    }
+
+template<int SS>
+static void dispatch_ss(DeviceObj& D, const char* nm, hipStream_t s, const TravArgs& a, bool primary, bool spheres)
+    {
+    const bool c = D.count_on;
+    if (primary)
+        {
+        if (spheres)
+            c ? launch_traverse<SS, true, true, T_SPHERES>(D, nm, s, a) : launch_traverse<SS, false, true, T_SPHERES>(D, nm, s, a);
+        else
+            c ? launch_traverse<SS, true, true, T_ALL>(D, nm, s, a) : launch_traverse<SS, false, true, T_ALL>(D, nm, s, a);
+        }
+    else
+        {
+        if (spheres)
+            c ? launch_traverse<SS, true, false, T_SPHERES>(D, nm, s, a) : launch_traverse<SS, false, false, T_SPHERES>(D, nm, s, a);
+        else
+            c ? launch_traverse<SS, true, false, T_ALL>(D, nm, s, a) : launch_traverse<SS, false, false, T_ALL>(D, nm, s, a);
+        }
+    }
+
+static void dispatch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const TravArgs& a, bool primary, bool spheres)
+    {
+    if (D.stack_lds == 8)
+        dispatch_ss<8>(D, nm, s, a, primary, spheres);
+    else
+        dispatch_ss<16>(D, nm, s, a, primary, spheres);
+    }
+
+// Largest resident grid of any traversal variant (sizes the overflow stacks).
+template<int SS> static uint32_t max_grid_ss(DeviceObj& D)
+    {
+    uint32_t g = 0;
+    for (const void* k : {reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, false, false, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, false, true, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, true, false, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, true, true, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, false, false, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, false, true, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, true, false, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, true, true, T_ALL>)})
+        g = std::max(g, D.grid_for(k, TRAV_BLOCK));
+    return g;
+    }
 
 // ====================================================================== geometry
 struct SceneObj;
@@ -715,8 +805,15 @@ static JobDev make_job(SceneObj& sc, TracerObj& tr, const fh_user_camera& cam_us
     }
 
 // ====================================================================== path rendering
-// Target number of light-sample paths per batch (sizes the queues).
-static uint64_t g_batch_paths = 8ull << 20;
+// Geometry types present (enabled) in a scene, as a bit mask.
+static uint32_t scene_types(const SceneObj& sc)
+    {
+    uint32_t m = 0;
+    for (auto& kv : sc.geoms)
+        if (kv.second->valid && kv.second->enabled && prim_count(*kv.second) > 0)
+            m |= 1u << kv.second->type;
+    return m;
+    }
 
 static void read_count(DeviceObj& D, const uint32_t* dptr, uint32_t& out)
     {
@@ -726,7 +823,7 @@ static void read_count(DeviceObj& D, const uint32_t* dptr, uint32_t& out)
     }
 
 // Run one batch: every job advances by jobs[j].K camera samples.
-static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs)
+static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
     {
     hipStream_t s = D.stream;
     RenderScratch& R = D.rs;
@@ -742,7 +839,8 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs)
         n_pix += J.npix;
         }
     REQUIRE(n_pri < (1ull << 31) && n_paths < (1ull << 31), "batch too large");
-    const uint64_t cap = std::max(n_pri, n_paths);
+    const uint64_t cap = n_paths; // primary rays are generated from their index, only hits are stored
+    R.hits.ensure(std::max(n_pri, n_paths));
     for (int q = 0; q < 2; q++)
         {
         R.qo[q].ensure(cap);
@@ -750,16 +848,15 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs)
         R.qatt[q].ensure(cap);
         R.qmeta[q].ensure(cap);
         }
-    R.hits.ensure(cap);
     R.records.ensure(n_paths);
     R.counters32.ensure(4);
     R.jobs.ensure(jobs.size());
     R.pix_base.ensure(jobs.size());
     HIPCHECK(hipMemcpyAsync(R.jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(R.pix_base.p, pix_base.data(), pix_base.size() * 4, hipMemcpyHostToDevice, s));
-    const uint32_t gstride = D.trav_grid * TRAV_BLOCK;
-    R.gstack.ensure((size_t)(STACK_MAX - TRAV_SS) * gstride);
-    R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * TAIL_RAYS);
+    const uint32_t gstride = D.trav_grid * TRAV_BLOCK; // max resident threads of any traversal variant
+    R.gstack.ensure((size_t)(STACK_MAX - 8) * gstride);
+    R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * D.tail_rays);
     Queue Q[2];
     for (int q = 0; q < 2; q++)
         Q[q] = Queue {R.qo[q].p, R.qd[q].p, R.qatt[q].p, R.qmeta[q].p};
@@ -768,32 +865,20 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs)
     uint32_t* qcount = R.counters32.p + 1;
     unsigned long long* ctr = D.counters.p;
 
-    auto traverse = [&](const Queue& q, uint32_t n)
+    const bool spheres = (types == T_SPHERES);
+    auto traverse = [&](const Queue& q, uint32_t n, bool primary)
     {
         HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
-        if (D.count_on)
-            D.launch("traverse",
-                     [&]
-                     {
-                         k_traverse<TRAV_BLOCK, TRAV_SS, true><<<D.trav_grid, TRAV_BLOCK, 0, s>>>(
-                             R.jobs.p, q, n, R.hits.p, work, R.gstack.p, gstride, ctr);
-                     });
-        else
-            D.launch("traverse",
-                     [&]
-                     {
-                         k_traverse<TRAV_BLOCK, TRAV_SS, false><<<D.trav_grid, TRAV_BLOCK, 0, s>>>(
-                             R.jobs.p, q, n, R.hits.p, work, R.gstack.p, gstride, ctr);
-                     });
+        TravArgs a {R.jobs.p, nj, q, n, R.hits.p, work, R.gstack.p, gstride, ctr};
+        dispatch_traverse(D, primary ? "traverse_primary" : "traverse", s, a, primary, spheres);
     };
 
-    D.launch("camgen", [&] { k_camgen<<<blocks(n_pri, 256), 256, 0, s>>>(R.jobs.p, nj, (uint32_t)n_pri, Q[0]); });
-    traverse(Q[0], (uint32_t)n_pri);
+    traverse(Q[0], (uint32_t)n_pri, true);
     HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
     D.launch("shade_primary",
              [&]
              {
-                 k_shade_primary<<<blocks(n_pri, 256), 256, 0, s>>>(R.jobs.p, Q[0], R.hits.p, (uint32_t)n_pri, Q[1],
+                 k_shade_primary<<<blocks(n_pri, 256), 256, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri, Q[1],
                                                                     qcount, R.records.p);
              });
     uint32_t n = 0;
@@ -801,17 +886,21 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs)
     int cur = 1;
     while (n > 0)
         {
-        if (n <= TAIL_RAYS)
+        if (n <= D.tail_rays)
             {
             D.launch("finish",
                      [&]
                      {
-                         k_finish<TRAV_BLOCK, TRAV_SS><<<blocks(n, TRAV_BLOCK), TRAV_BLOCK, 0, s>>>(
-                             R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p, TAIL_RAYS, ctr);
+                         if (spheres)
+                             k_finish<TRAV_BLOCK, TRAV_SS, T_SPHERES><<<blocks(n, TRAV_BLOCK), TRAV_BLOCK, 0, s>>>(
+                                 R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p, n, ctr);
+                         else
+                             k_finish<TRAV_BLOCK, TRAV_SS, T_ALL><<<blocks(n, TRAV_BLOCK), TRAV_BLOCK, 0, s>>>(
+                                 R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p, n, ctr);
                      });
             break;
             }
-        traverse(Q[cur], n);
+        traverse(Q[cur], n, false);
         HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
         D.launch("shade",
                  [&]
@@ -852,7 +941,10 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
     uint64_t per_sample = 0;
     for (auto* tr : tracers)
         per_sample += (uint64_t)tr->W * tr->H * tr->L;
-    uint32_t K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(samples, g_batch_paths / std::max<uint64_t>(per_sample, 1)));
+    uint32_t K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(samples, D.batch_paths / std::max<uint64_t>(per_sample, 1)));
+    uint32_t types = 0;
+    for (auto* sc : uniq)
+        types |= scene_types(*sc);
     uint32_t done = 0;
     while (done < samples)
         {
@@ -864,7 +956,7 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
             J.K = k;
             jobs.push_back(J);
             }
-        run_batch(D, jobs);
+        run_batch(D, jobs, types);
         for (auto* tr : tracers)
             tr->n_samples += k;
         done += k;
@@ -891,6 +983,125 @@ static void render_preview(DeviceObj& D, SceneObj& sc, TracerObj& tr)
     D.launch("preview", [&] { k_preview<<<blocks(J.npix, 64), 64, 0, D.stream>>>(R.jobs.p, tr.aa_n); });
     D.end_stats();
     }
+
+// ====================================================================== serving
+static uint32_t prim_bytes_of(uint32_t type)
+    {
+    return type == G_SPHERE ? 16u : 32u;
+    }
+static uint64_t scene_prims(const SceneObj& sc, uint32_t& max_prim_bytes)
+    {
+    uint64_t n = 0;
+    max_prim_bytes = 16;
+    for (auto& kv : sc.geoms)
+        if (kv.second->valid && kv.second->enabled)
+            {
+            n += prim_count(*kv.second);
+            max_prim_bytes = std::max(max_prim_bytes, prim_bytes_of(kv.second->type));
+            }
+    return n;
+    }
+
+struct RequestObj
+    {
+    uint64_t id = 0;
+    SceneObj* scene = nullptr;
+    TracerObj* tracer = nullptr;
+    uint32_t samples = 0, done = 0, L = 1;
+    double t_arrival = 0, t_admit = -1, t_done = -1;
+    uint64_t footprint = 0;
+    int state = 0; // 0 queued, 1 active, 2 done, 3 rejected
+    };
+
+struct ServerObj
+    {
+    DeviceObj* dev = nullptr;
+    uint64_t budget = 0, in_use = 0, target_paths = 8ull << 20;
+    std::deque<PolicyItem> queue;
+    std::vector<uint64_t> active; // admission order
+    std::map<uint64_t, RequestObj> reqs;
+    uint64_t next_id = 1;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    uint64_t iterations = 0;
+    double clock(double now) const
+        {
+        if (now >= 0)
+            return now;
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+    };
+
+// One server iteration: admit, advance every active request, retire.
+static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& n_queued)
+    {
+    DeviceObj& D = *S.dev;
+    std::vector<PolicyItem> admitted, rejected;
+    policy_admit(S.queue, S.budget, S.in_use, admitted, rejected);
+    for (auto& it : rejected)
+        {
+        RequestObj& r = S.reqs[it.id];
+        r.state = 3;
+        r.t_done = S.clock(now);
+        }
+    {
+    std::lock_guard<std::mutex> lock(D.mu);
+    for (auto& it : admitted)
+        {
+        RequestObj& r = S.reqs[it.id];
+        r.state = 1;
+        r.t_admit = S.clock(now);
+        sync_scene(*r.scene); // BVH build and uploads happen at admission
+        S.active.push_back(it.id);
+        }
+    }
+    if (!S.active.empty())
+        {
+        std::vector<PolicyItem> items;
+        for (uint64_t id : S.active)
+            {
+            const RequestObj& r = S.reqs[id];
+            items.push_back(PolicyItem {id, r.footprint, (uint64_t)r.tracer->W * r.tracer->H * r.L, r.samples - r.done});
+            }
+        std::vector<uint32_t> k = policy_allocate(items, S.target_paths);
+        std::lock_guard<std::mutex> lock(D.mu);
+        D.begin_stats();
+        std::vector<JobDev> jobs;
+        for (size_t i = 0; i < S.active.size(); i++)
+            {
+            RequestObj& r = S.reqs[S.active[i]];
+            JobDev J = make_job(*r.scene, *r.tracer, r.scene->camera);
+            J.K = k[i];
+            jobs.push_back(J);
+            }
+        uint32_t types = 0;
+        for (uint64_t id : S.active)
+            types |= scene_types(*S.reqs[id].scene);
+        run_batch(D, jobs, types);
+        D.end_stats();
+        if (D.last_counters[7])
+            throw Error(FH_ERR_STATE, "traversal stack overflow");
+        S.iterations++;
+        std::vector<uint64_t> still;
+        for (size_t i = 0; i < S.active.size(); i++)
+            {
+            RequestObj& r = S.reqs[S.active[i]];
+            r.tracer->n_samples += k[i];
+            r.done += k[i];
+            if (r.done >= r.samples)
+                {
+                r.state = 2;
+                r.t_done = S.clock(now);
+                S.in_use -= r.footprint;
+                r.tracer->L = 1; // as Path.sample leaves it (fresnel/tracer.py:259-261)
+                }
+            else
+                still.push_back(S.active[i]);
+            }
+        S.active.swap(still);
+        }
+    n_active = (uint32_t)S.active.size();
+    n_queued = (uint32_t)S.queue.size();
+    }
     } // namespace fh
 
 // ====================================================================== C ABI
@@ -911,6 +1122,10 @@ struct fh_geometry_s
 struct fh_tracer_s
     {
     TracerObj obj;
+    };
+struct fh_server_s
+    {
+    ServerObj obj;
     };
 
 #define NONNULL(p)                                                                                   \
@@ -986,10 +1201,12 @@ extern "C"
                 D.n_cu = p.multiProcessorCount;
                 D.name = std::string(p.name) + " (" + p.gcnArchName + ")";
                 HIPCHECK(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
-                int per_cu = 0;
-                HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per_cu, reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, TRAV_SS, false>), TRAV_BLOCK, 0));
-                D.trav_grid = (uint32_t)std::max(1, per_cu) * (uint32_t)D.n_cu;
+                // overflow stacks are sized for the largest resident traversal grid
+                D.trav_grid = std::max(max_grid_ss<8>(D), max_grid_ss<16>(D));
+                if (const char* e = getenv("FRESNEL_AMD_BATCH_PATHS"))
+                    D.batch_paths = std::max<uint64_t>(1, strtoull(e, nullptr, 10));
+                if (const char* e = getenv("FRESNEL_AMD_TAIL_RAYS"))
+                    D.tail_rays = (uint32_t)std::max<uint64_t>(1, strtoull(e, nullptr, 10));
                 void* hp = nullptr;
                 HIPCHECK(hipHostMalloc(&hp, 64, hipHostMallocDefault));
                 D.h_pinned = (uint32_t*)hp;
@@ -1729,6 +1946,242 @@ extern "C"
                     }
                 if (samples)
                     render_path(*tr[0]->dev, sc, tr, samples);
+            });
+        }
+
+
+    // ---------------------------------------------------------------- serving
+    int fh_server_create(fh_device dev, uint64_t hbm_budget, uint64_t target_paths, fh_server* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(out);
+                auto s = std::make_unique<fh_server_s>();
+                s->obj.dev = &dev->obj;
+                s->obj.budget = hbm_budget;
+                if (target_paths)
+                    s->obj.target_paths = target_paths;
+                *out = s.release();
+            });
+        }
+    int fh_server_destroy(fh_server srv)
+        {
+        return guard([&] { delete srv; });
+        }
+    int fh_server_submit(fh_server srv, fh_scene scene, fh_tracer tr, uint32_t samples, uint32_t light_samples,
+                         double now, uint64_t* id)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(srv);
+                NONNULL(scene);
+                NONNULL(tr);
+                NONNULL(id);
+                ServerObj& S = srv->obj;
+                REQUIRE(scene->obj.dev == S.dev && tr->obj.dev == S.dev, "Scene and Tracer devices do not match");
+                REQUIRE(tr->obj.type == T_PATH, "requests need a Path tracer");
+                REQUIRE(light_samples >= 1 && light_samples < 65536, "light_samples must be in [1, 65535]");
+                for (auto& kv : S.reqs)
+                    REQUIRE(kv.second.tracer != &tr->obj || kv.second.state >= 2, "tracer already has an open request");
+                RequestObj r;
+                r.id = S.next_id++;
+                r.scene = &scene->obj;
+                r.tracer = &tr->obj;
+                r.samples = samples;
+                r.L = light_samples;
+                r.t_arrival = S.clock(now);
+                uint32_t pb;
+                const uint64_t np = scene_prims(scene->obj, pb);
+                r.footprint = request_footprint(np, pb, tr->obj.W, tr->obj.H, light_samples);
+                // Path.sample(reset=True) semantics (fresnel/tracer.py:249-252)
+                tr->obj.n_samples = 0;
+                tr->obj.seed++;
+                tr->obj.L = light_samples;
+                tr->obj.clear();
+                S.reqs[r.id] = r;
+                if (samples == 0)
+                    {
+                    S.reqs[r.id].state = 2;
+                    S.reqs[r.id].t_done = r.t_arrival;
+                    }
+                else
+                    S.queue.push_back(PolicyItem {r.id, r.footprint, (uint64_t)tr->obj.W * tr->obj.H * light_samples, samples});
+                *id = r.id;
+            });
+        }
+    int fh_server_step(fh_server srv, double now, uint32_t* n_active, uint32_t* n_queued)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(srv);
+                uint32_t a = 0, q = 0;
+                server_step(srv->obj, now, a, q);
+                if (n_active)
+                    *n_active = a;
+                if (n_queued)
+                    *n_queued = q;
+            });
+        }
+    int fh_server_poll(fh_server srv, uint64_t id, int* state, uint32_t* samples_done, double times[3])
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(srv);
+                auto it = srv->obj.reqs.find(id);
+                REQUIRE(it != srv->obj.reqs.end(), "unknown request id");
+                const RequestObj& r = it->second;
+                if (state)
+                    *state = r.state;
+                if (samples_done)
+                    *samples_done = r.done;
+                if (times)
+                    {
+                    times[0] = r.t_arrival;
+                    times[1] = r.t_admit;
+                    times[2] = r.t_done;
+                    }
+            });
+        }
+    int fh_server_release(fh_server srv, uint64_t id)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(srv);
+                auto it = srv->obj.reqs.find(id);
+                REQUIRE(it != srv->obj.reqs.end(), "unknown request id");
+                REQUIRE(it->second.state >= 2, "request is still queued or active");
+                srv->obj.reqs.erase(it);
+            });
+        }
+    int fh_server_memory(fh_server srv, uint64_t* in_use, uint64_t* budget)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(srv);
+                if (in_use)
+                    *in_use = srv->obj.in_use;
+                if (budget)
+                    *budget = srv->obj.budget;
+            });
+        }
+
+    // ---------------------------------------------------------------- policy (no device needed)
+    int fh_policy_footprint(uint64_t n_prims, uint64_t prim_bytes, uint64_t W, uint64_t H, uint64_t L, uint64_t* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                *out = request_footprint(n_prims, prim_bytes, W, H, L);
+            });
+        }
+    // Run the policy on synthetic requests.  Request i arrives before iteration
+    // arrive[i]; one iteration advances every active request.  Outputs the
+    // iteration of admission and of completion (-1: never, -2: rejected).
+    int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* paths_per_sample,
+                           const uint32_t* samples, const uint32_t* arrive, uint64_t budget, uint64_t target_paths,
+                           uint32_t max_iterations, int32_t* admit_iter, int32_t* done_iter, uint64_t* peak_in_use)
+        {
+        return guard(
+            [&]
+            {
+                REQUIRE(footprint && paths_per_sample && samples && arrive && admit_iter && done_iter,
+                        "null pointer");
+                std::deque<PolicyItem> queue;
+                std::vector<PolicyItem> active;
+                uint64_t in_use = 0, peak = 0;
+                for (uint32_t i = 0; i < n; i++)
+                    admit_iter[i] = done_iter[i] = -1;
+                std::vector<uint32_t> order(n);
+                for (uint32_t i = 0; i < n; i++)
+                    order[i] = i;
+                std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return arrive[a] < arrive[b]; });
+                size_t next = 0;
+                for (uint32_t it = 0; it < max_iterations; it++)
+                    {
+                    while (next < n && arrive[order[next]] <= it)
+                        {
+                        const uint32_t i = order[next++];
+                        queue.push_back(PolicyItem {i, footprint[i], paths_per_sample[i], samples[i]});
+                        }
+                    std::vector<PolicyItem> adm, rej;
+                    policy_admit(queue, budget, in_use, adm, rej);
+                    for (auto& r : rej)
+                        admit_iter[r.id] = done_iter[r.id] = -2;
+                    for (auto& a : adm)
+                        {
+                        admit_iter[a.id] = (int32_t)it;
+                        active.push_back(a);
+                        }
+                    peak = std::max(peak, in_use);
+                    if (active.empty())
+                        {
+                        if (next >= n && queue.empty())
+                            break;
+                        continue;
+                        }
+                    std::vector<uint32_t> k = policy_allocate(active, target_paths);
+                    std::vector<PolicyItem> still;
+                    for (size_t j = 0; j < active.size(); j++)
+                        {
+                        active[j].remaining -= k[j];
+                        if (active[j].remaining == 0)
+                            {
+                            done_iter[active[j].id] = (int32_t)it;
+                            in_use -= active[j].footprint;
+                            }
+                        else
+                            still.push_back(active[j]);
+                        }
+                    active.swap(still);
+                    }
+                if (peak_in_use)
+                    *peak_in_use = peak;
+            });
+        }
+    // ---------------------------------------------------------------- tuning
+    int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(key);
+                DeviceObj& D = dev->obj;
+                std::string k(key);
+                int64_t prev;
+                if (k == "batch_paths")
+                    {
+                    prev = (int64_t)D.batch_paths;
+                    if (value > 0)
+                        D.batch_paths = (uint64_t)value;
+                    }
+                else if (k == "stack_lds")
+                    {
+                    prev = D.stack_lds;
+                    if (value > 0)
+                        {
+                        REQUIRE(value == 8 || value == 16, "stack_lds must be 8 or 16");
+                        D.stack_lds = (int)value;
+                        }
+                    }
+                else if (k == "tail_rays")
+                    {
+                    prev = (int64_t)D.tail_rays;
+                    if (value > 0)
+                        D.tail_rays = (uint32_t)value;
+                    }
+                else
+                    throw Error(FH_ERR_INVALID, "unknown tuning key: " + k);
+                if (previous)
+                    *previous = prev;
             });
         }
 

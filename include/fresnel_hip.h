@@ -200,6 +200,49 @@ int fh_tracer_direct_get_aa_n(fh_tracer tr, uint32_t* n);
  * on that pair alone. */
 int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t samples);
 
+/* ------------------------------------------------------ serving (no reference
+ * counterpart).  A server keeps admitted requests' scenes (BVH, primitives)
+ * and accumulation buffers resident and advances all active requests together:
+ * each fh_server_step admits queued requests first-come first-served while
+ * their HBM footprints fit the budget, then renders one batch in which every
+ * active request takes some camera samples, then retires finished requests.
+ * A request's image is bit-identical to Path.sample(scene, samples,
+ * light_samples) on its own: submission resets the tracer like sample()'s
+ * reset, and scheduling never changes per-path arithmetic.
+ * The scene and tracer of a request must stay alive until it is done; the
+ * tracer's light-sample count is set back to 1 at completion, as
+ * Path.sample leaves it.
+ * Times are seconds; pass now < 0 to use the server's monotonic clock. */
+typedef struct fh_server_s* fh_server;
+int fh_server_create(fh_device dev, uint64_t hbm_budget_bytes, uint64_t target_paths_per_step,
+                     fh_server* out);
+int fh_server_destroy(fh_server srv);
+int fh_server_submit(fh_server srv, fh_scene scene, fh_tracer tracer, uint32_t samples,
+                     uint32_t light_samples, double now, uint64_t* id);
+int fh_server_step(fh_server srv, double now, uint32_t* n_active, uint32_t* n_queued);
+/* state: 0 queued, 1 active, 2 done, 3 rejected (footprint above the budget);
+ * times[0..2] = arrival, admission, completion (-1 if not yet). */
+int fh_server_poll(fh_server srv, uint64_t id, int* state, uint32_t* samples_done, double times[3]);
+/* Forget a done or rejected request (its id becomes unknown). */
+int fh_server_release(fh_server srv, uint64_t id);
+int fh_server_memory(fh_server srv, uint64_t* in_use, uint64_t* budget);
+/* The server's policy without a device: footprint formula and a simulation of
+ * admission and completion iterations for synthetic requests. */
+int fh_policy_footprint(uint64_t n_prims, uint64_t prim_bytes, uint64_t W, uint64_t H, uint64_t L,
+                        uint64_t* out);
+int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* paths_per_sample,
+                       const uint32_t* samples, const uint32_t* arrive, uint64_t budget,
+                       uint64_t target_paths, uint32_t max_iterations, int32_t* admit_iter,
+                       int32_t* done_iter, uint64_t* peak_in_use);
+
+/* ------------------------------------------------------ tuning (no reference
+ * counterpart).  key "batch_paths": light-sample paths per wavefront batch
+ * (default 32 Mi; sizes the ray queues); "tail_rays": below this many live
+ * rays a batch finishes its paths in one kernel (default 32768); "stack_lds":
+ * traversal stack entries kept in LDS per thread, 8 or 16 (default 8).  value <= 0
+ * only reads the current value into *previous. */
+int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
+
 /* ------------------------------------------------------ diagnostics (no
  * reference counterpart; used by the parity tests and the benchmark) */
 /* Philox4x32-10 on the device: in[6k..6k+5] = ctr0..3, key0, key1 -> out[4k..] */

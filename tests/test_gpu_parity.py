@@ -186,3 +186,76 @@ def test_random_rays_vs_checker(gpu_device, cpu_device):
     assert numpy.array_equal(gid, rg)
     assert numpy.array_equal(prim, rp)
     assert numpy.array_equal(t.view(numpy.uint32), rt.view(numpy.uint32))
+
+
+@pytest.mark.parametrize('builder', [lambda d: c1_scene(d, n=20000, w_seed=4), scenes.eight_polyhedra],
+                         ids=['spheres', 'polyhedra'])
+def test_tuning_knobs_do_not_change_image(builder, gpu_device):
+    """Launch-shape knobs (LDS stack depth, batch size, tail threshold) leave the image bit-identical.
+
+    A 4096-path batch forces many batches per sample and an 8-entry LDS stack
+    forces spills to the global overflow stack on the 20,000-sphere BVH.
+    """
+    d = gpu_device._device
+    keys = ('stack_lds', 'batch_paths', 'tail_rays')
+    saved = {k: d.tune(k, 0) for k in keys}
+    settings = [dict(stack_lds=16, batch_paths=1 << 25, tail_rays=32768),
+                dict(stack_lds=8, batch_paths=4096, tail_rays=1),
+                dict(stack_lds=8, batch_paths=1 << 20, tail_rays=1 << 20),
+                dict(stack_lds=16, batch_paths=10000, tail_rays=100)]
+    images = []
+    try:
+        for st in settings:
+            for k, v in st.items():
+                d.tune(k, v)
+            images.append(_linear(gpu_device, builder, 72, 40, 3, 2)[0])
+    finally:
+        for k, v in saved.items():
+            d.tune(k, v)
+    for k, img in enumerate(images[1:], 1):
+        assert numpy.array_equal(img, images[0]), f"setting {settings[k]} changed the image"
+
+
+def test_tune_rejects_bad_keys(gpu_device):
+    d = gpu_device._device
+    with pytest.raises(Exception, match='stack_lds'):
+        d.tune('stack_lds', 12)
+    with pytest.raises(Exception, match='unknown tuning key'):
+        d.tune('warp_size', 32)
+
+
+def test_served_requests_equal_plain_sample(gpu_device):
+    """Requests advanced together by the render server equal Path.sample on each alone, bit for bit."""
+    from fresnel_amd import serve
+    builders = [lambda d: c1_scene(d, n=3000, w_seed=5), scenes.four_cylinders, scenes.eight_polyhedra,
+                lambda d: c1_scene(d, n=800, w_seed=6)]
+    specs = [((64, 48), 7, 1), ((80, 40), 3, 2), ((50, 50), 12, 1), ((96, 64), 1, 3)]
+    plain = [_linear(gpu_device, b, w, h, S, L)[0] for b, ((w, h), S, L) in zip(builders, specs)]
+    sc = [b(gpu_device) for b in builders]
+    trs = [fresnel.tracer.Path(gpu_device, w=w, h=h) for ((w, h), _, _) in specs]
+    # a small per-step target splits requests over several steps with changing mixes
+    srv = serve.Server(gpu_device, hbm_budget=1 << 34, target_paths=9000)
+    reqs = [srv.submit(s, t, S, L) for s, t, (_, S, L) in zip(sc, trs, specs)]
+    steps = 0
+    while not all(r.done for r in reqs):
+        srv.step()
+        steps += 1
+        assert steps < 100
+    assert steps > 1
+    for k, (r, t) in enumerate(zip(reqs, trs)):
+        st, n, _ = r.poll()
+        assert st == serve.DONE and n == specs[k][1]
+        assert t._tracer.getNumSamples() == specs[k][1]
+        assert numpy.array_equal(numpy.array(t.linear_output[:]), plain[k]), f"request {k} differs"
+    assert srv.memory()[0] == 0
+
+
+def test_server_rejects_over_budget(gpu_device):
+    from fresnel_amd import serve
+    s = c1_scene(gpu_device, n=100)
+    t = fresnel.tracer.Path(gpu_device, w=32, h=32)
+    srv = serve.Server(gpu_device, hbm_budget=1024, target_paths=1 << 20)
+    r = srv.submit(s, t, 4)
+    srv.step()
+    assert r.state == serve.REJECTED
+    srv.release(r)
