@@ -174,6 +174,18 @@ class Scene(object):
         """Device: the device of this scene."""
         return self._device
 
+    def set_bvh_refit(self, enable=True):
+        """Refit the BVH instead of rebuilding it when primitives only move.
+
+        Not in the reference, which rebuilds on every change.  Images are
+        bit-identical either way (the closest hit does not depend on the
+        hierarchy); a refit is faster to produce and, for small moves between
+        frames of a trajectory, about as fast to traverse.
+        """
+        if not hasattr(self._scene, 'setRefit'):
+            raise RuntimeError("BVH refit needs the GPU backend")
+        self._scene.setRefit(bool(enable))
+
     @property
     def camera(self):
         """camera.Camera: the view (a view onto the scene's own camera record)."""

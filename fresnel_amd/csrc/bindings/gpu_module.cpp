@@ -467,9 +467,9 @@ PYBIND11_MODULE(_gpu, m)
         .def("countersLast",
              [](Device& d)
              {
-                 uint64_t c[8];
+                 uint64_t c[16];
                  check(fh_counters_last(d.h, c));
-                 return std::vector<uint64_t>(c, c + 8);
+                 return std::vector<uint64_t>(c, c + 16);
              })
         .def("tune",
              [](Device& d, const std::string& key, int64_t value)
@@ -500,6 +500,25 @@ PYBIND11_MODULE(_gpu, m)
         .def("setBackgroundAlpha", [](Scene& s, float a) { s.bg_alpha = a; })
         .def("getLights", [](Scene& s) -> fh_lights& { return s.lights; }, py::return_value_policy::reference_internal)
         .def("setLights", [](Scene& s, const fh_lights& l) { s.lights = l; })
+        .def("setRefit", [](Scene& s, bool on) { check(fh_scene_set_refit(s.h, on ? 1 : 0)); })
+        .def("bvhInfo",
+             [](Scene& s)
+             {
+                 uint64_t b, r;
+                 uint32_t n;
+                 check(fh_scene_bvh_info(s.h, &b, &r, &n));
+                 return py::make_tuple(b, r, n);
+             })
+        .def("debugBVH",
+             [](Scene& s)
+             {
+                 s.push();
+                 uint64_t n = 0;
+                 check(fh_debug_bvh(s.h, nullptr, 0, &n));
+                 py::array_t<uint32_t> out({(py::ssize_t)n, (py::ssize_t)16});
+                 check(fh_debug_bvh(s.h, out.request().ptr, n, &n));
+                 return out;
+             })
         .def("debugTrace",
              [](Scene& s, py::array_t<float, py::array::c_style | py::array::forcecast> rays, float tnear)
              {

@@ -384,12 +384,18 @@ __global__ void k_refit(const uint32_t* __restrict__ vals, const float4* __restr
     for (;;)
         {
         const uint32_t node = par >> 1;
+        // Boxes and flags are device-scope atomics, which are coherent across
+        // the XCDs' L2 caches without flushing them.  s_waitcnt 0 (every
+        // counter of this wave drained) makes the box stores complete at the
+        // device coherence point before the flag increment is issued; the
+        // sibling's box loads are issued only after its increment returned 1.
+        // A device-scope fence here would write back and invalidate the whole
+        // L2 once per thread and level.
         store_box(nodes + node, par & 1u, box);
-        __threadfence();
-        const uint32_t prev = atomicAdd(flags + node, 1u);
+        __builtin_amdgcn_s_waitcnt(0);
+        const uint32_t prev = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (prev == 0)
             return; // the sibling finishes this node
-        __threadfence();
         float b0[6], b1[6];
         load_box(nodes + node, 0, b0);
         load_box(nodes + node, 1, b1);

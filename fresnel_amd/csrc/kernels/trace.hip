@@ -332,12 +332,13 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
         }
     if (COUNT)
         {
-        atomicAdd(counters + 0, (unsigned long long)n_rays);
-        atomicAdd(counters + 1, (unsigned long long)n_nodes);
-        atomicAdd(counters + 2, (unsigned long long)n_prims);
+        const int c = PRIMARY ? CTR_PRIMARY : CTR_SECONDARY;
+        atomicAdd(counters + c + 0, (unsigned long long)n_rays);
+        atomicAdd(counters + c + 1, (unsigned long long)n_nodes);
+        atomicAdd(counters + c + 2, (unsigned long long)n_prims);
         }
     if (err)
-        atomicOr((unsigned int*)(counters + 7), 1u);
+        atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
     }
 
 // ---------------------------------------------------------------- shading
@@ -469,7 +470,7 @@ __global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* 
             // depth cap (the reference has none): stop without contribution
             cont = false;
             rec = make_rec(mk3(0, 0, 0), REC_NONE);
-            atomicAdd(counters + 4, 1ull);
+            atomicAdd(counters + CTR_CAPPED, 1ull);
             }
         if (!cont)
             records[rec_index(J, k, mt.y, ls)] = rec;
@@ -485,7 +486,7 @@ __global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* 
     }
 
 // Tail: each thread finishes its path (trace + shade until termination).
-template<int BLOCK, int SS, uint32_t TYPES>
+template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ jobs, Queue in, uint32_t n,
                                                   uint4* records, uint32_t max_depth, uint2* gstack,
                                                   uint32_t gstride, unsigned long long* counters)
@@ -508,19 +509,21 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
     v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
     const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
     uint32_t depth = mt.w;
-    uint32_t dummy = 0, err = 0;
+    uint32_t n_nodes = 0, n_prims = 0, n_rays = 0, err = 0;
     uint4 rec;
     for (;;)
         {
         HitT h;
-        trace_closest<LdsStack<SS>, false, TYPES>(J.scene, o, d, 1e-3f, h, st, dummy, dummy, err);
+        trace_closest<LdsStack<SS>, COUNT, TYPES>(J.scene, o, d, 1e-3f, h, st, n_nodes, n_prims, err);
+        if (COUNT)
+            n_rays++;
         v3 no, nd;
         if (!shade_bounce(J, h, o, d, 1e-3f, mt.y, k, ls, depth, att, no, nd, rec))
             break;
         if (depth + 1 >= max_depth)
             {
             rec = make_rec(mk3(0, 0, 0), REC_NONE);
-            atomicAdd(counters + 4, 1ull);
+            atomicAdd(counters + CTR_CAPPED, 1ull);
             break;
             }
         o = no;
@@ -528,8 +531,14 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
         depth++;
         }
     records[rec_index(J, k, mt.y, ls)] = rec;
+    if (COUNT)
+        {
+        atomicAdd(counters + CTR_TAIL + 0, (unsigned long long)n_rays);
+        atomicAdd(counters + CTR_TAIL + 1, (unsigned long long)n_nodes);
+        atomicAdd(counters + CTR_TAIL + 2, (unsigned long long)n_prims);
+        }
     if (err)
-        atomicOr((unsigned int*)(counters + 7), 1u);
+        atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
     }
 
 // ---------------------------------------------------------------- accumulate

@@ -60,6 +60,11 @@ struct GeomDev
 // BVH2 node with both children's boxes (64 bytes).  A child is an inner node
 // index, a leaf (LEAF bit | local primitive index, geometry slot in g), or
 // EMPTY with a NaN box that every slab test rejects.
+// Device work-counter slots (64-bit each): rays traced, BVH nodes visited
+// and primitive tests of the primary, secondary and tail traversals; paths
+// stopped at the depth cap; traversal-stack overflow flag.
+constexpr int CTR_SECONDARY = 0, CTR_PRIMARY = 3, CTR_TAIL = 6, CTR_CAPPED = 9, CTR_OVERFLOW = 15, N_CTR = 16;
+
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t EMPTY = 0xffffffffu;
 struct __attribute__((aligned(64))) Node

@@ -175,7 +175,7 @@ struct DeviceObj
     std::vector<KernelTime> times;
     std::vector<hipEvent_t> event_pool;
     std::vector<std::pair<std::string, std::pair<double, uint32_t>>> last_stats;
-    uint64_t last_counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t last_counters[N_CTR] = {};
     std::mutex mu;
 
     hipEvent_t event()
@@ -216,11 +216,8 @@ struct DeviceObj
             event_pool.push_back(t.b);
             }
         times.clear();
-        if (count_on || true)
-            {
-            counters.ensure(8);
-            HIPCHECK(hipMemsetAsync(counters.p, 0, 8 * sizeof(unsigned long long), stream));
-            }
+        counters.ensure(N_CTR);
+        HIPCHECK(hipMemsetAsync(counters.p, 0, N_CTR * sizeof(unsigned long long), stream));
         }
     void end_stats()
         {
@@ -241,9 +238,9 @@ struct DeviceObj
             if (!found)
                 last_stats.push_back({t.name, {ms, 1u}});
             }
-        unsigned long long c[8];
+        unsigned long long c[N_CTR];
         HIPCHECK(hipMemcpy(c, counters.p, sizeof(c), hipMemcpyDeviceToHost));
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < N_CTR; k++)
             last_counters[k] = c[k];
         }
     };
@@ -891,12 +888,13 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
             D.launch("finish",
                      [&]
                      {
-                         if (spheres)
-                             k_finish<TRAV_BLOCK, TRAV_SS, T_SPHERES><<<blocks(n, TRAV_BLOCK), TRAV_BLOCK, 0, s>>>(
-                                 R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p, n, ctr);
-                         else
-                             k_finish<TRAV_BLOCK, TRAV_SS, T_ALL><<<blocks(n, TRAV_BLOCK), TRAV_BLOCK, 0, s>>>(
-                                 R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p, n, ctr);
+                         const dim3 g(blocks(n, TRAV_BLOCK));
+                         auto* f = spheres ? (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_SPHERES>
+                                                         : k_finish<TRAV_BLOCK, TRAV_SS, false, T_SPHERES>)
+                                           : (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_ALL>
+                                                         : k_finish<TRAV_BLOCK, TRAV_SS, false, T_ALL>);
+                         f<<<g, TRAV_BLOCK, 0, s>>>(R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p,
+                                                    n, ctr);
                      });
             break;
             }
@@ -962,7 +960,7 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
         done += k;
         }
     D.end_stats();
-    if (D.last_counters[7])
+    if (D.last_counters[CTR_OVERFLOW])
         throw Error(FH_ERR_STATE, "traversal stack overflow");
     }
 
@@ -1078,7 +1076,7 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
             types |= scene_types(*S.reqs[id].scene);
         run_batch(D, jobs, types);
         D.end_stats();
-        if (D.last_counters[7])
+        if (D.last_counters[CTR_OVERFLOW])
             throw Error(FH_ERR_STATE, "traversal stack overflow");
         S.iterations++;
         std::vector<uint64_t> still;
@@ -1210,8 +1208,8 @@ extern "C"
                 void* hp = nullptr;
                 HIPCHECK(hipHostMalloc(&hp, 64, hipHostMallocDefault));
                 D.h_pinned = (uint32_t*)hp;
-                D.counters.ensure(8);
-                HIPCHECK(hipMemset(D.counters.p, 0, 8 * sizeof(unsigned long long)));
+                D.counters.ensure(N_CTR);
+                HIPCHECK(hipMemset(D.counters.p, 0, N_CTR * sizeof(unsigned long long)));
                 *out = d.release();
             });
         }
@@ -1338,6 +1336,29 @@ extern "C"
                     *color = scene->obj.bg;
                 if (alpha)
                     *alpha = scene->obj.bg_alpha;
+            });
+        }
+    int fh_scene_set_refit(fh_scene scene, int enable)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                scene->obj.allow_refit = enable != 0;
+            });
+        }
+    int fh_scene_bvh_info(fh_scene scene, uint64_t* builds, uint64_t* refits, uint32_t* n_prims)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                if (builds)
+                    *builds = scene->obj.builds;
+                if (refits)
+                    *refits = scene->obj.refits;
+                if (n_prims)
+                    *n_prims = scene->obj.n_prims;
             });
         }
     int fh_scene_set_lights(fh_scene scene, const fh_lights* lights)
@@ -2246,6 +2267,23 @@ extern "C"
                     }
             });
         }
+    int fh_debug_bvh(fh_scene scene, void* nodes, uint64_t cap, uint64_t* n_nodes)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                NONNULL(n_nodes);
+                SceneObj& sc = scene->obj;
+                DeviceObj& D = *sc.dev;
+                std::lock_guard<std::mutex> lock(D.mu);
+                sync_scene(sc);
+                const uint64_t n = sc.n_prims == 0 ? 0 : std::max<uint32_t>(sc.n_prims - 1, 1);
+                *n_nodes = n;
+                if (nodes && n)
+                    HIPCHECK(hipMemcpy(nodes, sc.nodes.p, std::min(n, cap) * sizeof(Node), hipMemcpyDeviceToHost));
+            });
+        }
     int fh_debug_trace(fh_scene scene, const float* rays, uint32_t n, float tnear, uint32_t* geom_id, uint32_t* prim,
                        float* t)
         {
@@ -2324,14 +2362,14 @@ extern "C"
                 dev->obj.count_on = enable != 0;
             });
         }
-    int fh_counters_last(fh_device dev, uint64_t out[8])
+    int fh_counters_last(fh_device dev, uint64_t out[16])
         {
         return guard(
             [&]
             {
                 NONNULL(dev);
                 NONNULL(out);
-                for (int k = 0; k < 8; k++)
+                for (int k = 0; k < N_CTR; k++)
                     out[k] = dev->obj.last_counters[k];
             });
         }

@@ -93,6 +93,14 @@ int fh_scene_set_background(fh_scene scene, fh_rgb color, float alpha);
 int fh_scene_get_background(fh_scene scene, fh_rgb* color, float* alpha);
 int fh_scene_set_lights(fh_scene scene, const fh_lights* lights);
 int fh_scene_get_lights(fh_scene scene, fh_lights* lights);
+/* BVH maintenance (no reference counterpart: the reference rebuilds on every
+ * commit, fresnel/gpu/Scene.cc).  With refit enabled, a change that keeps the
+ * set of primitives (positions, radii, orientations moved) only recomputes the
+ * boxes of the existing hierarchy.  Closest hits are defined independently of
+ * the hierarchy, so images are bit-identical either way; only traversal speed
+ * differs. */
+int fh_scene_set_refit(fh_scene scene, int enable);
+int fh_scene_bvh_info(fh_scene scene, uint64_t* builds, uint64_t* refits, uint32_t* n_prims);
 
 /* ---------------------------------------------------------------- geometry */
 #define FH_GEOM_SPHERE 0
@@ -255,6 +263,12 @@ int fh_debug_primary(fh_tracer tr, fh_scene scene, uint32_t sample, float* rays,
 /* Closest hits for arbitrary rays (6 floats each) with tnear. */
 int fh_debug_trace(fh_scene scene, const float* rays, uint32_t n, float tnear,
                    uint32_t* geom_id, uint32_t* prim, float* t);
+/* The scene's BVH after bringing it up to date: *n_nodes 64-byte nodes
+ * (child 0 box lo.xyz hi.xyz, child 1 box lo.xyz hi.xyz, then links c0, c1,
+ * g0, g1; a link with the top bit set is a leaf holding primitive c & 0x7fffffff
+ * of geometry slot g; 0xffffffff is an empty child).  Copies at most `cap`
+ * nodes into `nodes` (NULL to query the count). */
+int fh_debug_bvh(fh_scene scene, void* nodes, uint64_t cap, uint64_t* n_nodes);
 /* Per-kernel device time of the most recent fh_tracer_render_samples /
  * fh_batch_render call, from HIP events recorded on the render stream.
  * names: '\n'-separated kernel names; ms[k]: summed time; launches[k]: count. */
@@ -262,12 +276,15 @@ int fh_stats_last(fh_device dev, char* names, size_t names_cap, double* ms,
                   uint32_t* launches, uint32_t cap, uint32_t* n_kernels);
 /* Enable per-kernel event timing (adds one event pair per launch). */
 int fh_stats_enable(fh_device dev, int enable);
-/* Deterministic work counters of the most recent render: [0] rays traced,
- * [1] BVH nodes visited, [2] primitive tests, [3] paths finished, [4] paths
- * stopped at the depth cap.  Counting is enabled with fh_counters_enable and
- * uses an instrumented traversal kernel. */
+/* Deterministic work counters of the most recent render: rays traced, BVH
+ * nodes visited and primitive tests of the secondary-ray traversal kernel
+ * ([0], [1], [2]), the primary-ray traversal kernel ([3], [4], [5]) and the
+ * tail kernel that finishes the last paths of a batch ([6], [7], [8]); [9]
+ * paths stopped at the depth cap; [15] non-zero if a traversal stack
+ * overflowed.  Counting is enabled with fh_counters_enable and uses
+ * instrumented kernels (slower; the counts do not depend on timing). */
 int fh_counters_enable(fh_device dev, int enable);
-int fh_counters_last(fh_device dev, uint64_t out[8]);
+int fh_counters_last(fh_device dev, uint64_t out[16]);
 
 #ifdef __cplusplus
 }
