@@ -348,6 +348,10 @@ def run_plain(args):
         "path_roofline": {"alg_bytes_per_path": round(path_bytes / paths, 1),
                           "achieved_GBs": round(path_bytes / mean_s / 1e9, 1),
                           "frac": round(path_bytes / mean_s / 1e9 / HBM_PEAK_GBS, 4)},
+        "primary_traversal": {"rays": int(ctr[3]), "nodes_per_ray": round(ctr[4] / max(ctr[3], 1), 2),
+                              "prims_per_ray": round(ctr[5] / max(ctr[3], 1), 2),
+                              "ms": kernel_ms.get('traverse_primary')},
+        "tail": {"rays": int(ctr[6]), "nodes_per_ray": round(ctr[7] / max(ctr[6], 1), 2)},
         "kernel_ms_last_step": kernel_ms,
         "kernel_launches_last_step": kernel_launches,
         "tuning": tuned,

@@ -174,17 +174,20 @@ class Scene(object):
         """Device: the device of this scene."""
         return self._device
 
-    def set_bvh_refit(self, enable=True):
+    def set_bvh_refit(self, enable=True, limit=1.5):
         """Refit the BVH instead of rebuilding it when primitives only move.
 
         Not in the reference, which rebuilds on every change.  Images are
         bit-identical either way (the closest hit does not depend on the
         hierarchy); a refit is faster to produce and, for small moves between
-        frames of a trajectory, about as fast to traverse.
+        frames of a trajectory, about as fast to traverse.  When a refit raises
+        the hierarchy's surface-area cost above ``limit`` times its value after
+        the last build, the scene is rebuilt instead.
         """
         if not hasattr(self._scene, 'setRefit'):
             raise RuntimeError("BVH refit needs the GPU backend")
         self._scene.setRefit(bool(enable))
+        self._scene.setRefitLimit(float(limit))
 
     @property
     def camera(self):

@@ -10,6 +10,7 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
                   amortise host synchronisation and the per-batch tail.
 * ``tail_rays``   live-ray count below which a batch finishes its remaining
                   paths in one kernel.
+* ``trav_chunk``  rays a traversal wavefront claims per work-counter atomic.
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -25,6 +26,7 @@ CANDIDATES = {
     'stack_lds': [8, 16],
     'batch_paths': [8 << 20, 32 << 20, 64 << 20],
     'tail_rays': [8192, 32768, 131072],
+    'trav_chunk': [64, 256, 1024],
 }
 
 
@@ -51,7 +53,7 @@ def arch(device):
     for line in device._device.describe().splitlines():
         if line.startswith('[0]:'):
             return line[4:].strip()
-    return 'unknown
+    return 'unknown'
 
 
 def apply(device, params):

@@ -501,6 +501,14 @@ PYBIND11_MODULE(_gpu, m)
         .def("getLights", [](Scene& s) -> fh_lights& { return s.lights; }, py::return_value_policy::reference_internal)
         .def("setLights", [](Scene& s, const fh_lights& l) { s.lights = l; })
         .def("setRefit", [](Scene& s, bool on) { check(fh_scene_set_refit(s.h, on ? 1 : 0)); })
+        .def("setRefitLimit", [](Scene& s, float limit) { check(fh_scene_set_refit_limit(s.h, limit)); })
+        .def("bvhCost",
+             [](Scene& s)
+             {
+                 float b, c;
+                 check(fh_scene_bvh_cost(s.h, &b, &c));
+                 return py::make_tuple(b, c);
+             })
         .def("bvhInfo",
              [](Scene& s)
              {

@@ -413,4 +413,54 @@ __global__ void k_refit(const uint32_t* __restrict__ vals, const float4* __restr
         par = parent[node];
         }
     }
+// ---------------------------------------------------------------- quality
+// Surface-area cost of a hierarchy: the sum over internal nodes of their two
+// child boxes' half surface areas.  A random ray visits a node with
+// probability proportional to its box's area, so the sum tracks traversal
+// cost; the refit policy compares it with the value right after a build.
+__device__ __forceinline__ float half_area(float x0, float y0, float z0, float x1, float y1, float z1)
+    {
+    const float dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
+    return dx * dy + dy * dz + dz * dx;
+    }
+
+__device__ __forceinline__ float block_sum(float v, float* sh)
+    {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1)
+        {
+        if (threadIdx.x < s)
+            sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+        }
+    return sh[0];
+    }
+
+__global__ void k_sah_partial(const Node* __restrict__ nodes, uint32_t n_nodes, float* partial)
+    {
+    __shared__ float sh[256];
+    float v = 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_nodes; i += gridDim.x * blockDim.x)
+        {
+        const Node n = nodes[i];
+        v += half_area(n.a_lo.x, n.a_lo.y, n.a_lo.z, n.a_lo.w, n.a_hi.x, n.a_hi.y);
+        if (n.link.y != EMPTY)
+            v += half_area(n.a_hi.z, n.a_hi.w, n.b_box.x, n.b_box.y, n.b_box.z, n.b_box.w);
+        }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = v;
+    }
+
+__global__ void k_sah_final(const float* __restrict__ partial, uint32_t n_part, float* out)
+    {
+    __shared__ float sh[256];
+    float v = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n_part; i += blockDim.x)
+        v += partial[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0)
+        out[0] = v;
+    }
     } // namespace fh

@@ -15,9 +15,7 @@
 
 namespace fh
 {
-// relative slack of the cull by the current closest hit (shared with the CPU checker)
-constexpr float CULL_SLACK = 1.0000152587890625f; // 1 + 2^-16
-constexpr uint32_t STACK_MAX = 64;               // >= depth of a Karras tree over 64-bit keys
+constexpr uint32_t STACK_MAX = 64; // >= depth of a Karras tree over 64-bit keys
 
 struct HitT
     {
@@ -34,6 +32,16 @@ __device__ __forceinline__ void slab(v3 o, v3 inv, float lx, float ly, float lz,
     const float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
     tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    }
+
+// Lower bound on the computed distance of any primitive inside a box whose
+// slab entry is tmin (see CULL_K).  A box is culled when this bound exceeds
+// the closest hit, so every primitive that could still win is tested and the
+// result does not depend on the hierarchy or the visiting order.
+__device__ __forceinline__ float cull_bound(float tmin, float lx, float ly, float lz, float hx, float hy, float hz)
+    {
+    const float size = ((hx - lx) + (hy - ly)) + (hz - lz);
+    return tmin - CULL_K * (fmaxf(tmin, 0.0f) + size);
     }
 
 __device__ __forceinline__ bool better(float t, uint32_t g, uint32_t p, const HitT& b)
@@ -113,8 +121,8 @@ __device__ __forceinline__ bool leaf_test(const GeomDev& g, uint32_t prim, v3 o,
     }
 
 // Closest hit in the order (t, geometry, primitive) over the primitives whose
-// bounds pass the slab test with tmax >= tnear; a subtree is culled when its
-// entry distance exceeds the closest hit by the relative slack.
+// bounds pass the slab test with tmax >= tnear; a subtree is culled when the
+// lower bound of its primitives' distances exceeds the closest hit.
 template<class Stack, bool COUNT, uint32_t TYPES = T_ALL>
 __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear, HitT& best, Stack& st,
                                      uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
@@ -139,8 +147,10 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
         float t0min, t0max, t1min, t1max;
         slab(o, inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
         slab(o, inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
-        bool h0 = (t0min <= t0max) && (t0max >= tnear) && !(t0min > best.t * CULL_SLACK);
-        bool h1 = (t1min <= t1max) && (t1max >= tnear) && !(t1min > best.t * CULL_SLACK);
+        const float c0 = cull_bound(t0min, A.x, A.y, A.z, A.w, B.x, B.y);
+        const float c1 = cull_bound(t1min, B.z, B.w, C.x, C.y, C.z, C.w);
+        bool h0 = (t0min <= t0max) && (t0max >= tnear) && !(c0 > best.t);
+        bool h1 = (t1min <= t1max) && (t1max >= tnear) && !(c1 > best.t);
         if (h0 && (lk.x & LEAF))
             {
             const uint32_t prim = lk.x & ~LEAF;
@@ -169,12 +179,12 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
                 }
             h1 = false;
             }
-        h0 = h0 && !(t0min > best.t * CULL_SLACK);
-        h1 = h1 && !(t1min > best.t * CULL_SLACK);
+        h0 = h0 && !(c0 > best.t);
+        h1 = h1 && !(c1 > best.t);
         if (h0 && h1)
             {
             const bool swap = t1min < t0min;
-            st.push(swap ? lk.x : lk.y, swap ? t0min : t1min, err);
+            st.push(swap ? lk.x : lk.y, swap ? c0 : c1, err);
             node = swap ? lk.y : lk.x;
             }
         else if (h0)
@@ -189,7 +199,7 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
                 uint32_t n;
                 float tm;
                 st.pop(n, tm);
-                if (!(tm > best.t * CULL_SLACK))
+                if (!(tm > best.t))
                     {
                     node = n;
                     found = true;
@@ -265,23 +275,61 @@ struct Queue
 // until the queue is exhausted.
 // Primary ray i of a batch: job, camera sample and pixel from the index
 // (fresnel/cpu/TracerPath.cc:100-136, tnear 1e-3).
+// Order of a job's primary rays: the image is cut into chunks of 64 pixels
+// (8x8 tiles over the largest multiple-of-8 region, then the right and bottom
+// strips row by row), and the K samples of a chunk are consecutive.  A
+// wavefront therefore traces one sample of one compact tile, and the rays in
+// flight at any time cover a K times smaller part of the image than in
+// sample-major order.  local in [0, npix K) -> (sample k, pixel px, py).
+__device__ __forceinline__ void primary_order(const JobDev& J, uint32_t local, uint32_t& k, uint32_t& px,
+                                              uint32_t& py)
+    {
+    const uint32_t chunk = local / (64u * J.K);
+    const uint32_t rem = local - chunk * 64u * J.K;
+    const uint32_t size = min(64u, J.npix - chunk * 64u);
+    k = rem / size;
+    const uint32_t j = chunk * 64u + (rem - k * size);
+    const uint32_t W8 = J.W & ~7u, H8 = J.H & ~7u;
+    if (j < W8 * H8)
+        {
+        const uint32_t tile = j >> 6, within = j & 63u, tiles_x = W8 >> 3;
+        const uint32_t ty = tile / tiles_x;
+        px = (tile - ty * tiles_x) * 8u + (within & 7u);
+        py = ty * 8u + (within >> 3);
+        return;
+        }
+    uint32_t r = j - W8 * H8;
+    const uint32_t rw = J.W - W8;
+    if (r < rw * H8)
+        {
+        py = r / rw;
+        px = W8 + (r - py * rw);
+        return;
+        }
+    r -= rw * H8;
+    const uint32_t by = r / J.W;
+    py = H8 + by;
+    px = r - by * J.W;
+    }
+
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
                                             uint32_t& k, uint32_t& pixel, v3& o, v3& d)
     {
     job = find_job(jobs, n_jobs, i);
     const JobDev& J = jobs[job];
-    const uint32_t local = i - J.ray_base;
-    k = local / J.npix;
-    pixel = local - k * J.npix;
-    const uint32_t py = pixel / J.W;
-    const uint32_t px = pixel - py * J.W;
+    uint32_t px, py;
+    primary_order(J, i - J.ray_base, k, px, py);
+    pixel = py * J.W + px;
     camera_ray(J.cam, J.W, J.H, J.seed, px, py, J.n_base + k + 1, true, o, d);
     }
 
+// Persistent traversal: each wavefront claims `chunk` rays (a multiple of 64)
+// per atomic on the work counter and claims its next chunk before tracing the
+// current one, so the atomic's round trip overlaps the traversal.
 template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
-                                                    uint32_t n, uint4* hits, uint32_t* work, uint2* gstack,
-                                                    uint32_t gstride, unsigned long long* counters)
+                                                    uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
+                                                    uint2* gstack, uint32_t gstride, unsigned long long* counters)
     {
     __shared__ uint2 lds[SS * BLOCK];
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -293,16 +341,20 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
     st.sp = 0;
     const uint32_t lane = lane_id();
     uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
-    for (;;)
+    uint32_t claim = 0;
+    if (lane == 0)
+        claim = atomicAdd(work, chunk);
+    uint32_t base = __shfl(claim, 0, 64);
+    while (base < n)
         {
-        uint32_t base = 0;
+        uint32_t next = 0;
         if (lane == 0)
-            base = atomicAdd(work, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= n)
-            break;
-        const uint32_t i = base + lane;
-        if (i < n)
+            next = atomicAdd(work, chunk);
+        const uint32_t end = min(n, base + chunk);
+        for (uint32_t sub = base; sub < end; sub += 64)
+        {
+        const uint32_t i = sub + lane;
+        if (i < end)
             {
             v3 o, d;
             float tnear;
@@ -329,6 +381,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
             if (COUNT)
                 n_rays++;
             }
+        }
+        base = __shfl(next, 0, 64);
         }
     if (COUNT)
         {

@@ -95,6 +95,16 @@ __device__ inline void prim_bounds(const GeomDev& g, uint32_t prim, v3& lo, v3& 
 // ---------------------------------------------------------------- sphere
 // fresnel/common/IntersectSphere.h:36-85 (first root past 1e-4, else the second)
 // `first` tells which root was taken (the normal flips for the second).
+// Cull margin of the closest-hit traversal.  The reference's sphere solve
+// (b*b - v.v + r*r) cancels for distant grazing hits: in float arithmetic
+// t = b - sqrt(det) can fall below the true entry distance by up to about
+// sqrt(14 u)|v| ~ 9.2e-4 |v| (u = 2^-24, |v| = distance from the ray origin to
+// the centre), and the spherocylinder caps use the same solve.  A box whose
+// slab entry is tmin holds no primitive with a computed distance below
+// tmin - CULL_K (max(tmin, 0) + L1 size of the box); CULL_K = 2^-9 keeps a
+// factor two over that bound.
+constexpr float CULL_K = 0.001953125f; // 2^-9
+
 __device__ __forceinline__ bool sphere_t(v3 o, v3 d, v3 p, float r, float& t, float& d_edge, bool& first)
     {
     const v3 v = p - o;

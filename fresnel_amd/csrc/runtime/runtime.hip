@@ -159,6 +159,7 @@ struct DeviceObj
     bool count_on = false;
     uint64_t batch_paths = 32ull << 20; // light-sample paths per batch (sizes the queues)
     uint32_t tail_rays = 32768;         // finish paths in one kernel below this many rays
+    uint32_t trav_chunk = 256;          // rays a traversal wavefront claims per work-counter atomic
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
     std::map<const void*, uint32_t> grids;
     uint32_t grid_for(const void* kernel, int block)
@@ -255,6 +256,7 @@ struct TravArgs
     uint32_t n;
     uint4* hits;
     uint32_t* work;
+    uint32_t chunk;
     uint2* gstack;
     uint32_t gstride;
     unsigned long long* counters;
@@ -268,8 +270,8 @@ static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const T
     D.launch(nm,
              [&]
              {
-                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.gstack, a.gstride,
-                                                 a.counters);
+                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.chunk, a.gstack,
+                                                 a.gstride, a.counters);
              });
     }
 
@@ -395,6 +397,9 @@ struct SceneObj
     std::vector<uint32_t> topo; // (slot, prim count) of the enabled geometries in the last build
     uint64_t builds = 0, refits = 0;
     bool allow_refit = false;   // refit instead of rebuilding when only positions moved
+    float refit_limit = 1.5f;   // rebuild when a refit raises the surface-area cost by this factor
+    float sah_built = 0.0f;     // surface-area cost right after the last build
+    float sah_last = 0.0f;      // surface-area cost of the current hierarchy
 
     uint32_t alloc_slot()
         {
@@ -615,6 +620,46 @@ static void build_bvh(SceneObj& sc, bool refit_only)
     sc.bvh_valid = true;
     }
 
+// Surface-area cost of the scene's current hierarchy (synchronizes the stream).
+static float bvh_sah(SceneObj& sc)
+    {
+    if (sc.n_prims < 2)
+        return 0.0f;
+    DeviceObj& D = *sc.dev;
+    const uint32_t n_nodes = sc.n_prims - 1;
+    const uint32_t n_part = std::min<uint32_t>(blocks(n_nodes, 256), 1024);
+    sc.partial.ensure(std::max<uint32_t>(6 * n_part, n_part + 1));
+    D.launch("lbvh_sah",
+             [&]
+             {
+                 k_sah_partial<<<n_part, 256, 0, D.stream>>>(sc.nodes.p, n_nodes, sc.partial.p + 1);
+                 k_sah_final<<<1, 256, 0, D.stream>>>(sc.partial.p + 1, n_part, sc.partial.p);
+             });
+    float v = 0.0f;
+    HIPCHECK(hipMemcpyAsync(&v, sc.partial.p, sizeof(float), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    return v;
+    }
+
+// Refit-or-rebuild policy for a scene whose primitives moved: refit, and
+// rebuild if the refit hierarchy's surface-area cost exceeds refit_limit times
+// the cost right after the last build.
+static void update_bvh(SceneObj& sc)
+    {
+    if (!sc.allow_refit)
+        {
+        build_bvh(sc, false);
+        return;
+        }
+    build_bvh(sc, true);
+    sc.sah_last = bvh_sah(sc);
+    if (sc.sah_last > sc.refit_limit * sc.sah_built)
+        {
+        build_bvh(sc, false);
+        sc.sah_built = sc.sah_last = bvh_sah(sc);
+        }
+    }
+
 // Bring the device copy of the scene up to date: uploads, geometry table, BVH.
 static void sync_scene(SceneObj& sc)
     {
@@ -668,9 +713,13 @@ static void sync_scene(SceneObj& sc)
     sc.n_slots = n_slots;
     const bool topo_same = sc.bvh_valid && topo == sc.topo;
     if (!sc.bvh_valid || !topo_same)
+        {
         build_bvh(sc, false);
+        if (sc.allow_refit)
+            sc.sah_built = sc.sah_last = bvh_sah(sc);
+        }
     else if (any_dirty)
-        build_bvh(sc, sc.allow_refit);
+        update_bvh(sc);
     sc.topo = topo;
     // the table upload above must complete before the host vector goes away
     HIPCHECK(hipStreamSynchronize(D.stream));
@@ -866,7 +915,7 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
     auto traverse = [&](const Queue& q, uint32_t n, bool primary)
     {
         HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
-        TravArgs a {R.jobs.p, nj, q, n, R.hits.p, work, R.gstack.p, gstride, ctr};
+        TravArgs a {R.jobs.p, nj, q, n, R.hits.p, work, D.trav_chunk, R.gstack.p, gstride, ctr};
         dispatch_traverse(D, primary ? "traverse_primary" : "traverse", s, a, primary, spheres);
     };
 
@@ -1345,6 +1394,28 @@ extern "C"
             {
                 NONNULL(scene);
                 scene->obj.allow_refit = enable != 0;
+            });
+        }
+    int fh_scene_set_refit_limit(fh_scene scene, float limit)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                REQUIRE(limit >= 1.0f, "the refit limit must be >= 1");
+                scene->obj.refit_limit = limit;
+            });
+        }
+    int fh_scene_bvh_cost(fh_scene scene, float* built, float* current)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(scene);
+                if (built)
+                    *built = scene->obj.sah_built;
+                if (current)
+                    *current = scene->obj.sah_last;
             });
         }
     int fh_scene_bvh_info(fh_scene scene, uint64_t* builds, uint64_t* refits, uint32_t* n_prims)
@@ -2191,6 +2262,15 @@ extern "C"
                         {
                         REQUIRE(value == 8 || value == 16, "stack_lds must be 8 or 16");
                         D.stack_lds = (int)value;
+                        }
+                    }
+                else if (k == "trav_chunk")
+                    {
+                    prev = D.trav_chunk;
+                    if (value > 0)
+                        {
+                        REQUIRE(value % 64 == 0 && value <= 65536, "trav_chunk must be a multiple of 64, at most 65536");
+                        D.trav_chunk = (uint32_t)value;
                         }
                     }
                 else if (k == "tail_rays")

@@ -100,6 +100,11 @@ int fh_scene_get_lights(fh_scene scene, fh_lights* lights);
  * the hierarchy, so images are bit-identical either way; only traversal speed
  * differs. */
 int fh_scene_set_refit(fh_scene scene, int enable);
+/* A refit is kept while the hierarchy's surface-area cost (sum of child-box
+ * half areas) stays within `limit` times its value right after the last
+ * build; beyond that the scene is rebuilt (default 1.5). */
+int fh_scene_set_refit_limit(fh_scene scene, float limit);
+int fh_scene_bvh_cost(fh_scene scene, float* built, float* current);
 int fh_scene_bvh_info(fh_scene scene, uint64_t* builds, uint64_t* refits, uint32_t* n_prims);
 
 /* ---------------------------------------------------------------- geometry */
@@ -247,8 +252,10 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * counterpart).  key "batch_paths": light-sample paths per wavefront batch
  * (default 32 Mi; sizes the ray queues); "tail_rays": below this many live
  * rays a batch finishes its paths in one kernel (default 32768); "stack_lds":
- * traversal stack entries kept in LDS per thread, 8 or 16 (default 8).  value <= 0
- * only reads the current value into *previous. */
+ * traversal stack entries kept in LDS per thread, 8 or 16 (default 8);
+ * "trav_chunk": rays a traversal wavefront claims per work-counter atomic, a
+ * multiple of 64 up to 65536 (default 256).  value <= 0 only reads the
+ * current value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
 /* ------------------------------------------------------ diagnostics (no

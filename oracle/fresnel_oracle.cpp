@@ -605,8 +605,19 @@ static inline void slab(v3 o, v3 inv, const Box& b, float& tmin, float& tmax)
     tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
     }
-// relative slack applied when culling by the current closest hit
-static const float CULL_SLACK = 1.0f + 0x1p-16f;
+// Cull rule by the current closest hit.  The reference's sphere solve
+// (IntersectSphere.h: det = b*b - v.v + r*r) cancels for distant grazing hits,
+// so the computed t can lie below the box's slab entry by up to about
+// 9.2e-4 |v| (|v| = distance from the ray origin to the centre).  A box whose
+// slab entry is tmin holds no primitive with a computed distance below
+// cull_bound(); boxes are culled only when that bound exceeds the closest hit,
+// which makes the result independent of the hierarchy and of visiting order.
+static const float CULL_K = 0x1p-9f;
+static inline float cull_bound(float tmin, const Box& b)
+    {
+    const float size = ((b.hi.x - b.lo.x) + (b.hi.y - b.lo.y)) + (b.hi.z - b.lo.z);
+    return tmin - CULL_K * (fmaxf(tmin, 0.0f) + size);
+    }
 
 // ---------------------------------------------------------------- intersectors
 struct Hit
@@ -1203,7 +1214,7 @@ static inline void test_ref(const Scene& s, unsigned r, v3 o, v3 d, v3 inv, floa
     slab(o, inv, s.ref_box[r], tmin, tmax);
     if (!(tmin <= tmax && tmax >= tnear))
         return;
-    if (cull && best.hit && tmin > best.t * CULL_SLACK)
+    if (cull && best.hit && cull_bound(tmin, s.ref_box[r]) > best.t)
         return;
     const PrimRef& pr = s.refs[r];
     const Geom& g = s.geoms[pr.geom];
@@ -1243,7 +1254,7 @@ static HitRec closest_hit(const Scene& s, v3 o, v3 d, float tnear)
         slab(o, inv, n.box, tmin, tmax);
         if (!(tmin <= tmax && tmax >= tnear))
             continue;
-        if (best.hit && tmin > best.t * CULL_SLACK)
+        if (best.hit && cull_bound(tmin, n.box) > best.t)
             continue;
         if (n.left < 0)
             {

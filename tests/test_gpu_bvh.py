@@ -107,7 +107,7 @@ def test_refit_matches_rebuild(gpu_device):
     pos = rng.uniform(-15, 15, (n, 3)).astype(numpy.float32)
     rad = rng.uniform(0.2, 0.6, n).astype(numpy.float32)
     s, g = sphere_scene(gpu_device, pos, rad)
-    s._scene.setRefit(True)
+    s.set_bvh_refit(True, limit=100.0)
     t = fresnel.tracer.Path(gpu_device, w=96, h=64)
     t.sample(s, samples=2, light_samples=2)
     b0, r0, _ = s._scene.bvhInfo()
@@ -126,3 +126,30 @@ def test_refit_matches_rebuild(gpu_device):
         assert numpy.array_equal(refit_img, numpy.array(t2.linear_output[:])), f"frame {frame}"
     b1, r1, _ = s._scene.bvhInfo()
     assert b1 == b0 and r1 == r0 + 3
+
+
+def test_refit_policy_rebuilds_degraded_hierarchy(gpu_device):
+    """A refit that inflates the surface-area cost beyond the limit is replaced by a rebuild."""
+    n = 20000
+    rng = numpy.random.default_rng(12)
+    pos = rng.uniform(-15, 15, (n, 3)).astype(numpy.float32)
+    rad = numpy.full(n, 0.3, numpy.float32)
+    s, g = sphere_scene(gpu_device, pos, rad)
+    s.set_bvh_refit(True, limit=1.5)
+    s._scene.debugBVH()  # build
+    b0, r0, _ = s._scene.bvhInfo()
+    built, cur = s._scene.bvhCost()
+    assert built > 0 and cur == built
+    # small move: refit kept
+    g.position[:] = (pos + rng.normal(0, 0.01, pos.shape)).astype(numpy.float32)
+    s._scene.debugBVH()
+    b1, r1, _ = s._scene.bvhInfo()
+    assert (b1, r1) == (b0, r0 + 1)
+    assert s._scene.bvhCost()[1] < 1.5 * built
+    # scramble: the refit tree's cost explodes, so the scene is rebuilt
+    g.position[:] = rng.permutation(pos)
+    nodes = s._scene.debugBVH()
+    b2, r2, _ = s._scene.bvhInfo()
+    assert b2 == b1 + 1 and r2 == r1 + 1
+    new_pos = numpy.array(g.position[:])
+    check_bvh(nodes, n, (new_pos - rad[:, None], new_pos + rad[:, None]))

@@ -197,12 +197,12 @@ def test_tuning_knobs_do_not_change_image(builder, gpu_device):
     forces spills to the global overflow stack on the 20,000-sphere BVH.
     """
     d = gpu_device._device
-    keys = ('stack_lds', 'batch_paths', 'tail_rays')
+    keys = ('stack_lds', 'batch_paths', 'tail_rays', 'trav_chunk')
     saved = {k: d.tune(k, 0) for k in keys}
-    settings = [dict(stack_lds=16, batch_paths=1 << 25, tail_rays=32768),
-                dict(stack_lds=8, batch_paths=4096, tail_rays=1),
-                dict(stack_lds=8, batch_paths=1 << 20, tail_rays=1 << 20),
-                dict(stack_lds=16, batch_paths=10000, tail_rays=100)]
+    settings = [dict(stack_lds=16, batch_paths=1 << 25, tail_rays=32768, trav_chunk=256),
+                dict(stack_lds=8, batch_paths=4096, tail_rays=1, trav_chunk=64),
+                dict(stack_lds=8, batch_paths=1 << 20, tail_rays=1 << 20, trav_chunk=4096),
+                dict(stack_lds=16, batch_paths=10000, tail_rays=100, trav_chunk=192)]
     images = []
     try:
         for st in settings:
@@ -220,6 +220,8 @@ def test_tune_rejects_bad_keys(gpu_device):
     d = gpu_device._device
     with pytest.raises(Exception, match='stack_lds'):
         d.tune('stack_lds', 12)
+    with pytest.raises(Exception, match='trav_chunk'):
+        d.tune('trav_chunk', 100)
     with pytest.raises(Exception, match='unknown tuning key'):
         d.tune('warp_size', 32)
 
@@ -259,3 +261,18 @@ def test_server_rejects_over_budget(gpu_device):
     srv.step()
     assert r.state == serve.REJECTED
     srv.release(r)
+
+
+def test_grazing_rays_vs_brute_force(gpu_device):
+    """Distant grazing hits whose computed t lies below their box entry: the GPU's culled
+    traversal returns the brute-force closest hit, bit for bit."""
+    from test_oracle import grazing_case, _sphere_scene, _trace
+    pos, rad, rays, _ = grazing_case()
+    s = fresnel.Scene(gpu_device)
+    fresnel.geometry.Sphere(s, position=pos, radius=rad)
+    gid, prim, t = s._scene.debugTrace(rays, 1e-3)
+    h = _sphere_scene(pos, rad, brute=True)
+    _, pf, tf, _ = _trace(h, rays)
+    oracle.lib().oracle_scene_free(h)
+    assert numpy.array_equal(prim, pf)
+    assert numpy.array_equal(t.view(numpy.uint32), tf.view(numpy.uint32))

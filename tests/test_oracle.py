@@ -194,3 +194,103 @@ def test_perspective_rays_unit_and_through_focal_plane(cpu_device):
     s = -10 / out[:, 5]
     numpy.testing.assert_allclose(out[:, 3] * s, x, atol=1e-5)
     numpy.testing.assert_allclose(out[:, 4] * s, y, atol=1e-5)
+
+
+def _f32_sphere_t(o, d, p, r):
+    """The reference's sphere solve (IntersectSphere.h) in float32, first root."""
+    f = numpy.float32
+    v = (p - o).astype(f)
+    b = f(f(f(v[0] * d[0]) + f(v[1] * d[1])) + f(v[2] * d[2]))
+    vv = f(f(f(v[0] * v[0]) + f(v[1] * v[1])) + f(v[2] * v[2]))
+    det = f(f(f(b * b) - vv) + f(r * r))
+    return f(b - f(numpy.sqrt(det)))
+
+
+def _f32_slab_tmin(o, d, lo, hi):
+    f = numpy.float32
+    inv = [f(f(1) / d[k]) for k in range(3)]
+    t0 = [f(f(lo[k] - o[k]) * inv[k]) for k in range(3)]
+    t1 = [f(f(hi[k] - o[k]) * inv[k]) for k in range(3)]
+    return max(min(t0[0], t1[0]), min(t0[1], t1[1]), min(t0[2], t1[2]))
+
+
+def test_cull_keeps_grazing_hits_below_their_box_entry():
+    """A distant grazing hit whose computed t lies 5e-4 (relative) below its box's slab entry.
+
+    Sphere B (primitive 0) is tested first and hit at t_B, between sphere A's
+    computed distance and A's box entry.  A cull by entry distance with a small
+    relative slack would drop A; the error-bounded cull keeps it, so the
+    hierarchy's answer equals brute force: A.
+    """
+    f = numpy.float32
+    o = numpy.zeros(3, f)
+    d = numpy.array([0.9991304874420166, -0.03203902393579483, 0.026680011302232742], f)
+    pA = numpy.array([74.91876220703125, -2.902667999267578, 2.0010008811950684], f)
+    r = f(0.5)
+    tA = _f32_sphere_t(o, d, pA, r)
+    entryA = _f32_slab_tmin(o, d, pA - r, pA + r)
+    assert (entryA - tA) / tA > 5e-4
+    pB = (o + d * f(74.97 + 0.5)).astype(f)
+    tB = _f32_sphere_t(o, d, pB, r)
+    assert tA < tB and entryA > tB * (1 + 2 ** -16)  # a relative-slack cull would drop A
+    rays = numpy.array([numpy.concatenate([o, d])], numpy.float32)
+    pos = numpy.array([pB, pA], numpy.float32)
+    rad = numpy.array([r, r], numpy.float32)
+    for brute in (False, True):
+        h = _sphere_scene(pos, rad, brute=brute)
+        g, p, t, _ = _trace(h, rays)
+        oracle.lib().oracle_scene_free(h)
+        assert p[0] == 1 and t[0] == tA, f"brute={brute}"
+
+
+def grazing_case(m=30000, seed=21):
+    """Adversarial scene for the cull rule: pairs of a distant grazing hit A and a small
+    sphere B hit between A's computed distance and A's box entry.
+
+    Returns (positions, radii, rays (k, 6), index of A for each ray).
+    """
+    f = numpy.float32
+    rng = numpy.random.default_rng(seed)
+    d = rng.normal(size=(m, 3)) * [1, 0.2, 0.2] + [5, 0, 0]
+    d = (d / numpy.linalg.norm(d, axis=1, keepdims=True)).astype(f)
+    dist = rng.uniform(40, 120, m)
+    r = rng.uniform(0.3, 0.6, m).astype(f)
+    perp = numpy.cross(d, rng.normal(size=(m, 3)))
+    perp /= numpy.linalg.norm(perp, axis=1, keepdims=True)
+    pA = (d * dist[:, None] + perp * (r[:, None] * (1 - rng.uniform(0, 4e-4, (m, 1))))).astype(f)
+    o = numpy.zeros(3, f)
+    pos, rad, rays, want = [], [], [], []
+    for i in range(m):
+        v = (pA[i] - o).astype(f)
+        b = f(f(f(v[0] * d[i, 0]) + f(v[1] * d[i, 1])) + f(v[2] * d[i, 2]))
+        vv = f(f(f(v[0] * v[0]) + f(v[1] * v[1])) + f(v[2] * v[2]))
+        det = f(f(f(b * b) - vv) + f(r[i] * r[i]))
+        if det < 0:
+            continue
+        tA = f(b - f(numpy.sqrt(det)))
+        entry = _f32_slab_tmin(o, d[i], pA[i] - r[i], pA[i] + r[i])
+        if not entry > tA * (1 + 2 ** -14):
+            continue
+        tB = 0.5 * (float(tA) + float(entry))
+        pos.append(d[i] * f(tB + 0.05))
+        rad.append(0.05)
+        pos.append(pA[i])
+        rad.append(r[i])
+        rays.append(numpy.concatenate([o, d[i]]))
+        want.append(len(pos) - 1)
+    return numpy.array(pos, f), numpy.array(rad, f), numpy.array(rays, f), numpy.array(want)
+
+
+def test_bvh_equals_brute_force_on_distant_grazing_rays():
+    """Hierarchy with culling == brute force on the adversarial grazing pairs, hit for hit."""
+    pos, rad, rays, want = grazing_case()
+    assert len(rays) > 100
+    hb = _sphere_scene(pos, rad, brute=False)
+    hf = _sphere_scene(pos, rad, brute=True)
+    gb, pb, tb, _ = _trace(hb, rays)
+    gf, pf, tf, _ = _trace(hf, rays)
+    oracle.lib().oracle_scene_free(hb)
+    oracle.lib().oracle_scene_free(hf)
+    assert numpy.sum(pf == want) > 50  # the adversarial pairs survive in the shared scene
+    numpy.testing.assert_array_equal(pb, pf)
+    numpy.testing.assert_array_equal(tb.view(numpy.uint32), tf.view(numpy.uint32))
