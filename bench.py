@@ -272,11 +272,15 @@ def run_plain(args):
     tracer = fresnel.tracer.Path(dev, w=W, h=H)
     print(f"# scene built on host in {time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
     d = dev._device
+    if args.copy_calibration:
+        ms = d.debugCopy(args.copy_calibration << 20)
+        log(f"copy calibration: {args.copy_calibration} MiB read + written in {ms:.3f} ms "
+            f"({2 * (args.copy_calibration << 20) / ms / 1e6:.0f} GB/s)")
     tuned = None
     if not args.no_tune:
         from fresnel_amd import autotune
         t0 = time.perf_counter()
-        tuned = autotune.tune(dev, scene, W, H, samples=max(1, S // 16), light_samples=L, use_cache=False,
+        tuned = autotune.tune(dev, scene, W, H, samples=max(1, S // 4), light_samples=L, use_cache=False,
                               log=lambda m: print("# " + m, file=sys.stderr, flush=True))
         print(f"# autotune: {tuned} in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     d.statsEnable(True)
@@ -605,6 +609,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='first run a plain device copy of MIB MiB (the reference kernel for HBM byte counters)')
     ap.add_argument('--frames', type=int, default=64, help='c4: trajectory frames')
     ap.add_argument('--requests', type=int, default=24, help='c5: requests per stream')
     ap.add_argument('--rates', default='5,10,20,50,100,200', help='c5: arrival rates (req/s)')

@@ -11,6 +11,8 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
 * ``tail_rays``   live-ray count below which a batch finishes its remaining
                   paths in one kernel.
 * ``trav_chunk``  rays a traversal wavefront claims per work-counter atomic.
+* ``trav_refill`` idle lanes of a traversal wavefront that trigger taking new
+                  rays (65: trace 64 rays at a time).
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -27,6 +29,7 @@ CANDIDATES = {
     'batch_paths': [8 << 20, 32 << 20, 64 << 20],
     'tail_rays': [8192, 32768, 131072],
     'trav_chunk': [64, 256, 1024],
+    'trav_refill': [16, 32, 48, 65],
 }
 
 

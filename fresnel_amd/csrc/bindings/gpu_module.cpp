@@ -471,6 +471,13 @@ PYBIND11_MODULE(_gpu, m)
                  check(fh_counters_last(d.h, c));
                  return std::vector<uint64_t>(c, c + 16);
              })
+        .def("debugCopy",
+             [](Device& d, uint64_t bytes)
+             {
+                 float ms = 0;
+                 check(fh_debug_copy(d.h, bytes, &ms));
+                 return ms;
+             })
         .def("tune",
              [](Device& d, const std::string& key, int64_t value)
              {

@@ -100,8 +100,6 @@ struct LocalStack
         }
     };
 
-constexpr uint32_t T_SPHERES = 1u << G_SPHERE;
-constexpr uint32_t T_ALL = 0x1fu;
 
 // Candidate distance of one primitive; TYPES lists the geometry types the
 // scene can hold, so a spheres-only traversal compiles without the other
@@ -120,96 +118,128 @@ __device__ __forceinline__ bool leaf_test(const GeomDev& g, uint32_t prim, v3 o,
         return intersect_prim<false>(g, prim, o, d, tnear, t, nullptr);
     }
 
-// Closest hit in the order (t, geometry, primitive) over the primitives whose
-// bounds pass the slab test with tmax >= tnear; a subtree is culled when the
-// lower bound of its primitives' distances exceeds the closest hit.
+// Closest-hit traversal state of one ray.
+struct TraceState
+    {
+    v3 o, d, inv;
+    float tnear;
+    HitT best;
+    uint32_t node;
+    };
+
+template<class Stack>
+__device__ __forceinline__ void trace_begin(const SceneDev& sc, v3 o, v3 d, float tnear, TraceState& r, Stack& st)
+    {
+    r.o = o;
+    r.d = d;
+    r.inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    r.tnear = tnear;
+    r.best.t = __builtin_huge_valf();
+    r.best.geom = EMPTY;
+    r.best.prim = EMPTY;
+    r.node = sc.root;
+    st.sp = 0;
+    }
+
+// One node visit of the closest-hit search: the primitives of leaf children
+// are tested, the nearer hit inner child is visited next and the farther one
+// pushed.  Returns false when the search is complete.  The closest hit is
+// the smallest (t, geometry, primitive) over the primitives whose bounds pass
+// the slab test with tmax >= tnear; a subtree is culled when the lower bound
+// of its primitives' distances (cull_bound) exceeds the closest hit.
+template<class Stack, bool COUNT, uint32_t TYPES>
+__device__ __forceinline__ bool trace_visit(const Node* __restrict__ nodes, const GeomDev* __restrict__ geoms,
+                                            TraceState& r, Stack& st, uint32_t& n_nodes, uint32_t& n_prims,
+                                            uint32_t& err)
+    {
+    if (r.node == EMPTY)
+        return false;
+    const Node* np = nodes + r.node;
+    const float4 A = np->a_lo;
+    const float4 B = np->a_hi;
+    const float4 C = np->b_box;
+    const uint4 lk = np->link;
+    if (COUNT)
+        n_nodes++;
+    float t0min, t0max, t1min, t1max;
+    slab(r.o, r.inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
+    slab(r.o, r.inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
+    const float c0 = cull_bound(t0min, A.x, A.y, A.z, A.w, B.x, B.y);
+    const float c1 = cull_bound(t1min, B.z, B.w, C.x, C.y, C.z, C.w);
+    bool h0 = (t0min <= t0max) && (t0max >= r.tnear) && !(c0 > r.best.t);
+    bool h1 = (t1min <= t1max) && (t1max >= r.tnear) && !(c1 > r.best.t);
+    if (h0 && (lk.x & LEAF))
+        {
+        const uint32_t prim = lk.x & ~LEAF;
+        float t;
+        if (COUNT)
+            n_prims++;
+        if (leaf_test<TYPES>(geoms[lk.z], prim, r.o, r.d, r.tnear, t) && better(t, lk.z, prim, r.best))
+            {
+            r.best.t = t;
+            r.best.geom = lk.z;
+            r.best.prim = prim;
+            }
+        h0 = false;
+        }
+    if (h1 && (lk.y & LEAF))
+        {
+        const uint32_t prim = lk.y & ~LEAF;
+        float t;
+        if (COUNT)
+            n_prims++;
+        if (leaf_test<TYPES>(geoms[lk.w], prim, r.o, r.d, r.tnear, t) && better(t, lk.w, prim, r.best))
+            {
+            r.best.t = t;
+            r.best.geom = lk.w;
+            r.best.prim = prim;
+            }
+        h1 = false;
+        }
+    h0 = h0 && !(c0 > r.best.t);
+    h1 = h1 && !(c1 > r.best.t);
+    if (h0 && h1)
+        {
+        const bool swap = t1min < t0min;
+        st.push(swap ? lk.x : lk.y, swap ? c0 : c1, err);
+        r.node = swap ? lk.y : lk.x;
+        return true;
+        }
+    if (h0)
+        {
+        r.node = lk.x;
+        return true;
+        }
+    if (h1)
+        {
+        r.node = lk.y;
+        return true;
+        }
+    while (st.sp > 0)
+        {
+        uint32_t n;
+        float tm;
+        st.pop(n, tm);
+        if (!(tm > r.best.t))
+            {
+            r.node = n;
+            return true;
+            }
+        }
+    r.node = EMPTY;
+    return false;
+    }
+
 template<class Stack, bool COUNT, uint32_t TYPES = T_ALL>
 __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear, HitT& best, Stack& st,
                                      uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
     {
-    best.t = __builtin_huge_valf();
-    best.geom = EMPTY;
-    best.prim = EMPTY;
-    if (sc.root == EMPTY)
-        return;
-    const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    st.sp = 0;
-    uint32_t node = sc.root;
-    for (;;)
+    TraceState r;
+    trace_begin(sc, o, d, tnear, r, st);
+    while (trace_visit<Stack, COUNT, TYPES>(sc.nodes, sc.geoms, r, st, n_nodes, n_prims, err))
         {
-        const Node* np = sc.nodes + node;
-        const float4 A = np->a_lo;
-        const float4 B = np->a_hi;
-        const float4 C = np->b_box;
-        const uint4 lk = np->link;
-        if (COUNT)
-            n_nodes++;
-        float t0min, t0max, t1min, t1max;
-        slab(o, inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
-        slab(o, inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
-        const float c0 = cull_bound(t0min, A.x, A.y, A.z, A.w, B.x, B.y);
-        const float c1 = cull_bound(t1min, B.z, B.w, C.x, C.y, C.z, C.w);
-        bool h0 = (t0min <= t0max) && (t0max >= tnear) && !(c0 > best.t);
-        bool h1 = (t1min <= t1max) && (t1max >= tnear) && !(c1 > best.t);
-        if (h0 && (lk.x & LEAF))
-            {
-            const uint32_t prim = lk.x & ~LEAF;
-            float t;
-            if (COUNT)
-                n_prims++;
-            if (leaf_test<TYPES>(sc.geoms[lk.z], prim, o, d, tnear, t) && better(t, lk.z, prim, best))
-                {
-                best.t = t;
-                best.geom = lk.z;
-                best.prim = prim;
-                }
-            h0 = false;
-            }
-        if (h1 && (lk.y & LEAF))
-            {
-            const uint32_t prim = lk.y & ~LEAF;
-            float t;
-            if (COUNT)
-                n_prims++;
-            if (leaf_test<TYPES>(sc.geoms[lk.w], prim, o, d, tnear, t) && better(t, lk.w, prim, best))
-                {
-                best.t = t;
-                best.geom = lk.w;
-                best.prim = prim;
-                }
-            h1 = false;
-            }
-        h0 = h0 && !(c0 > best.t);
-        h1 = h1 && !(c1 > best.t);
-        if (h0 && h1)
-            {
-            const bool swap = t1min < t0min;
-            st.push(swap ? lk.x : lk.y, swap ? c0 : c1, err);
-            node = swap ? lk.y : lk.x;
-            }
-        else if (h0)
-            node = lk.x;
-        else if (h1)
-            node = lk.y;
-        else
-            {
-            bool found = false;
-            while (st.sp > 0)
-                {
-                uint32_t n;
-                float tm;
-                st.pop(n, tm);
-                if (!(tm > best.t))
-                    {
-                    node = n;
-                    found = true;
-                    break;
-                    }
-                }
-            if (!found)
-                break;
-            }
         }
+    best = r.best;
     }
 
 __device__ __forceinline__ uint32_t lane_id()
@@ -228,6 +258,50 @@ __device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
         base = atomicAdd(counter, (uint32_t)__popcll(m));
     base = __shfl(base, 0, 64);
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    }
+
+// Append slot for lanes with pred set: one atomic per workgroup of BS
+// threads (wavefront counts combined in LDS).  Every thread of the
+// workgroup must call it; sh holds BS / 64 + 1 words.
+template<int BS>
+__device__ __forceinline__ uint32_t block_append(bool pred, uint32_t* counter, uint32_t* sh)
+    {
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane == 0)
+        sh[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        {
+        uint32_t total = 0;
+        for (int w = 0; w < BS / 64; w++)
+            {
+            const uint32_t c = sh[w];
+            sh[w] = total;
+            total += c;
+            }
+        sh[BS / 64] = total ? atomicAdd(counter, total) : 0u;
+        }
+    __syncthreads();
+    const uint32_t slot = sh[BS / 64] + sh[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads(); // sh is reused by the next call
+    return slot;
+    }
+
+template<int BS>
+__device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* sh)
+    {
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+    if (lane_id() == 0)
+        sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = 0;
+    for (int w = 0; w < BS / 64; w++)
+        r = max(r, sh[w]);
+    __syncthreads();
+    return r;
     }
 
 __device__ __forceinline__ uint32_t wave_max(uint32_t v)
@@ -397,20 +471,24 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
 
 // ---------------------------------------------------------------- shading
 // Shading attributes of the closest hit: re-evaluate the winning primitive.
+template<uint32_t TYPES = T_ALL>
 __device__ __forceinline__ void hit_attributes(const SceneDev& sc, const HitT& h, v3 o, v3 d, float tnear, Attr& at,
                                                const GeomDev*& g)
     {
     g = sc.geoms + h.geom;
     float t2;
-    intersect_prim<true>(*g, h.prim, o, d, tnear, t2, &at);
+    intersect_prim<true, TYPES>(*g, h.prim, o, d, tnear, t2, &at);
     }
 
 // Depth-0 shading: one primary hit spawns the L light-sample paths
 // (fresnel/cpu/TracerPath.cc:141-155 reuse the cached primary hit).
-__global__ void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs, const uint4* __restrict__ hits,
-                                uint32_t n, Queue out, uint32_t* out_count, uint4* records)
+template<int BS, uint32_t TYPES>
+__global__ __launch_bounds__(BS) void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs,
+                                                      const uint4* __restrict__ hits, uint32_t n, Queue out,
+                                                      uint32_t* out_count, uint4* records)
     {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t sh[BS / 64 + 1];
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
     const bool valid = i < n;
     uint32_t L = 0;
     uint32_t job = 0, pixel = 0, k = 0;
@@ -443,10 +521,10 @@ __global__ void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs
     const Mat* m = nullptr;
     if (L)
         {
-        hit_attributes(J.scene, h, o, d, tnear, at, g);
+        hit_attributes<TYPES>(J.scene, h, o, d, tnear, at, g);
         m = (at.d >= g->outline_width) ? &g->mat : &g->outline;
         }
-    const uint32_t Lmax = wave_max(L);
+    const uint32_t Lmax = block_max<BS>(L, sh);
     const uint32_t n_sample = valid ? J.n_base + k + 1 : 0;
     for (uint32_t ls = 0; ls < Lmax; ls++)
         {
@@ -459,7 +537,7 @@ __global__ void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs
             if (!cont)
                 records[rec_index(J, k, pixel, ls)] = make_rec(addv ? add : mk3(0, 0, 0), addv ? REC_ADD : REC_NONE);
             }
-        const uint32_t slot = wave_append(cont, out_count);
+        const uint32_t slot = block_append<BS>(cont, out_count, sh);
         if (cont)
             {
             out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
@@ -472,6 +550,7 @@ __global__ void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs
 
 // Shade one hit or miss of a path at depth >= 1.  Returns true when the path
 // continues with (no, nd, att); otherwise rec holds its termination record.
+template<uint32_t TYPES>
 __device__ inline bool shade_bounce(const JobDev& J, const HitT& h, v3 o, v3 d, float tnear, uint32_t pixel,
                                     uint32_t k, uint32_t ls, uint32_t depth, v3& att, v3& no, v3& nd, uint4& rec)
     {
@@ -483,7 +562,7 @@ __device__ inline bool shade_bounce(const JobDev& J, const HitT& h, v3 o, v3 d, 
         }
     Attr at;
     const GeomDev* g;
-    hit_attributes(J.scene, h, o, d, tnear, at, g);
+    hit_attributes<TYPES>(J.scene, h, o, d, tnear, at, g);
     const Mat& m = (at.d >= g->outline_width) ? g->mat : g->outline;
     const uint32_t n_sample = J.n_base + k + 1;
     v3 add;
@@ -495,11 +574,14 @@ __device__ inline bool shade_bounce(const JobDev& J, const HitT& h, v3 o, v3 d, 
     return cont;
     }
 
-__global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* __restrict__ hits, uint32_t n,
-                        Queue out, uint32_t* out_count, uint4* records, uint32_t max_depth,
-                        unsigned long long* counters)
+template<int BS, uint32_t TYPES>
+__global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, Queue in,
+                                              const uint4* __restrict__ hits, uint32_t n, Queue out,
+                                              uint32_t* out_count, uint4* records, uint32_t max_depth,
+                                              unsigned long long* counters)
     {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t sh[BS / 64 + 1];
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
     bool cont = false;
     v3 att, no, nd;
     uint4 mt = make_uint4(0, 0, 0, 0);
@@ -518,7 +600,8 @@ __global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* 
         att = mk3(at4.x, at4.y, at4.z);
         const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
         uint4 rec;
-        cont = shade_bounce(J, h, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, mt.y, k, ls, mt.w, att, no, nd, rec);
+        cont = shade_bounce<TYPES>(J, h, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, mt.y, k, ls, mt.w, att, no, nd,
+                                   rec);
         if (cont && mt.w + 1 >= max_depth)
             {
             // depth cap (the reference has none): stop without contribution
@@ -529,7 +612,7 @@ __global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* 
         if (!cont)
             records[rec_index(J, k, mt.y, ls)] = rec;
         }
-    const uint32_t slot = wave_append(cont, out_count);
+    const uint32_t slot = block_append<BS>(cont, out_count, sh);
     if (cont)
         {
         out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
@@ -537,6 +620,106 @@ __global__ void k_shade(const JobDev* __restrict__ jobs, Queue in, const uint4* 
         out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
         out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
         }
+    }
+
+// Persistent traversal with per-lane refill: a lane whose ray is finished
+// takes the next unclaimed ray of its wavefront's chunk as soon as at least
+// `refill` lanes of the wavefront are idle (or all are), so lanes do not wait
+// for the slowest ray of a 64-ray batch.  Chunks of `chunk` rays are claimed
+// per atomic, one chunk ahead.  Same per-ray arithmetic as k_traverse.
+template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
+__global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
+                                                        uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
+                                                        uint32_t refill, uint2* gstack, uint32_t gstride,
+                                                        unsigned long long* counters)
+    {
+    __shared__ uint2 lds[SS * BLOCK];
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    LdsStack<SS> st;
+    st.lds = lds + threadIdx.x;
+    st.lstride = BLOCK;
+    st.glob = gstack + gtid;
+    st.gstride = gstride;
+    st.sp = 0;
+    const uint32_t lane = lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    uint32_t claim = 0, ahead = 0;
+    if (lane == 0)
+        {
+        claim = atomicAdd(work, chunk);
+        ahead = atomicAdd(work, chunk);
+        }
+    uint32_t pool = __shfl(claim, 0, 64);
+    uint32_t pool_end = min(n, pool + chunk);
+    bool active = false;
+    uint32_t i = 0;
+    TraceState r;
+    const Node* nodes = nullptr;
+    const GeomDev* geoms = nullptr;
+    for (;;)
+        {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (pool < pool_end && (n_idle >= refill || n_idle == 64))
+            {
+            const uint32_t avail = pool_end - pool;
+            const uint32_t rank = (uint32_t)__popcll(idle & below);
+            if (!active && rank < avail)
+                {
+                i = pool + rank;
+                v3 o, d;
+                float tnear;
+                uint32_t job;
+                if (PRIMARY)
+                    {
+                    uint32_t k, pixel;
+                    primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+                    tnear = 1e-3f;
+                    }
+                else
+                    {
+                    const float4 a = q.o[i];
+                    const float4 b = q.d[i];
+                    o = mk3(a.x, a.y, a.z);
+                    d = mk3(b.x, b.y, b.z);
+                    tnear = a.w;
+                    job = __float_as_uint(b.w);
+                    }
+                const SceneDev& sc = jobs[job].scene;
+                nodes = sc.nodes;
+                geoms = sc.geoms;
+                trace_begin(sc, o, d, tnear, r, st);
+                active = true;
+                }
+            pool += min(n_idle, avail);
+            if (pool >= pool_end)
+                {
+                pool = __shfl(ahead, 0, 64);
+                pool_end = min(n, pool + chunk);
+                if (lane == 0 && pool < n)
+                    ahead = atomicAdd(work, chunk);
+                }
+            }
+        if (__ballot(active) == 0)
+            break;
+        if (active && !trace_visit<LdsStack<SS>, COUNT, TYPES>(nodes, geoms, r, st, n_nodes, n_prims, err))
+            {
+            hits[i] = make_uint4(__float_as_uint(r.best.t), r.best.geom, r.best.prim, 0);
+            active = false;
+            if (COUNT)
+                n_rays++;
+            }
+        }
+    if (COUNT)
+        {
+        const int c = PRIMARY ? CTR_PRIMARY : CTR_SECONDARY;
+        atomicAdd(counters + c + 0, (unsigned long long)n_rays);
+        atomicAdd(counters + c + 1, (unsigned long long)n_nodes);
+        atomicAdd(counters + c + 2, (unsigned long long)n_prims);
+        }
+    if (err)
+        atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
     }
 
 // Tail: each thread finishes its path (trace + shade until termination).
@@ -572,7 +755,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
         if (COUNT)
             n_rays++;
         v3 no, nd;
-        if (!shade_bounce(J, h, o, d, 1e-3f, mt.y, k, ls, depth, att, no, nd, rec))
+        if (!shade_bounce<TYPES>(J, h, o, d, 1e-3f, mt.y, k, ls, depth, att, no, nd, rec))
             break;
         if (depth + 1 >= max_depth)
             {
@@ -733,6 +916,13 @@ __global__ void k_preview(const JobDev* __restrict__ jobp, uint32_t aa_n)
     }
 
 // ---------------------------------------------------------------- diagnostics
+// Plain streaming copy, the reference point that scales the HBM byte counters.
+__global__ void k_debug_copy(const float4* __restrict__ src, float4* __restrict__ dst, uint64_t n)
+    {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+    }
+
 __global__ void k_debug_philox(const uint32_t* in, uint32_t n, uint32_t* out)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

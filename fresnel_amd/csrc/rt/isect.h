@@ -304,10 +304,32 @@ __device__ inline bool triangle(float& u, float& v, float& w, float& t, float& d
 
 // ---------------------------------------------------------------- intersect one primitive
 // The callbacks of fresnel/cpu/Geometry*.cc with `t < tfar` left to the caller.
-template<bool ATTR>
+// Geometry-type sets a kernel is compiled for: a spheres-only scene uses
+// kernels without the other intersectors (fewer registers, more wavefronts).
+constexpr uint32_t T_SPHERES = 1u << G_SPHERE;
+constexpr uint32_t T_ALL = 0x1fu;
+
+template<bool ATTR, uint32_t TYPES = T_ALL>
 __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 d, float tnear, float& t_out,
                                       Attr* attr)
     {
+    if constexpr (TYPES == T_SPHERES)
+        {
+        const float4 s = g.a[prim];
+        const v3 p = xyz(s);
+        float t, de;
+        bool first;
+        if (!sphere_t(o, d, p, s.w, t, de, first) || !(tnear < t))
+            return false;
+        t_out = t;
+        if (ATTR)
+            {
+            attr->Ng = mk3(o.x + t * d.x - p.x, o.y + t * d.y - p.y, o.z + t * d.z - p.z);
+            attr->d = de;
+            attr->color = ld3(g.color, prim);
+            }
+        return true;
+        }
     switch (g.type)
         {
     case G_SPHERE: // fresnel/cpu/GeometrySphere.cc:81-117

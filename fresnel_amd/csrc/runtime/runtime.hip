@@ -123,6 +123,7 @@ static unsigned blocks(size_t n, unsigned b)
 // ====================================================================== device
 constexpr int TRAV_BLOCK = 128;
 constexpr int TRAV_SS = 16;
+constexpr int SHADE_BLOCK = 512; // shading workgroup: one queue-append atomic per 8 wavefronts
 constexpr uint32_t MAX_DEPTH = 4096;    // safety cap on path length (the reference has none)
 
 struct KernelTime
@@ -157,9 +158,11 @@ struct DeviceObj
     uint32_t* h_pinned = nullptr; // 16 uint32 of pinned host memory for counts
     bool stats_on = false;
     bool count_on = false;
-    uint64_t batch_paths = 32ull << 20; // light-sample paths per batch (sizes the queues)
-    uint32_t tail_rays = 32768;         // finish paths in one kernel below this many rays
+    uint64_t batch_paths = 64ull << 20; // light-sample paths per batch (sizes the queues)
+    uint32_t tail_rays = 131072;        // finish paths in one kernel below this many rays
     uint32_t trav_chunk = 256;          // rays a traversal wavefront claims per work-counter atomic
+    uint32_t trav_refill = 65;          // lanes take new rays once this many of a wavefront are idle
+                                        // (65: batch-synchronous kernel, 64 rays at a time)
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
     std::map<const void*, uint32_t> grids;
     uint32_t grid_for(const void* kernel, int block)
@@ -265,8 +268,22 @@ struct TravArgs
 template<int SS, bool C, bool P, uint32_t T>
 static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const TravArgs& a)
     {
+    if (D.trav_refill <= 64)
+        {
+        auto kern = &k_traverse_dyn<TRAV_BLOCK, SS, C, P, T>;
+        const uint32_t grid = D.grid_for(reinterpret_cast<const void*>(kern), TRAV_BLOCK);
+        REQUIRE((uint64_t)grid * TRAV_BLOCK <= a.gstride, "traversal grid exceeds the overflow stacks");
+        D.launch(nm,
+                 [&]
+                 {
+                     kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.chunk,
+                                                     D.trav_refill, a.gstack, a.gstride, a.counters);
+                 });
+        return;
+        }
     auto kern = &k_traverse<TRAV_BLOCK, SS, C, P, T>;
     const uint32_t grid = D.grid_for(reinterpret_cast<const void*>(kern), TRAV_BLOCK);
+    REQUIRE((uint64_t)grid * TRAV_BLOCK <= a.gstride, "traversal grid exceeds the overflow stacks");
     D.launch(nm,
              [&]
              {
@@ -314,7 +331,15 @@ template<int SS> static uint32_t max_grid_ss(DeviceObj& D)
                           reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, false, false, T_ALL>),
                           reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, false, true, T_ALL>),
                           reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, true, false, T_ALL>),
-                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, true, true, T_ALL>)})
+                          reinterpret_cast<const void*>(&k_traverse<TRAV_BLOCK, SS, true, true, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, false, false, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, false, true, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, true, false, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, true, true, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, false, false, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, false, true, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, true, false, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, true, true, T_ALL>)})
         g = std::max(g, D.grid_for(k, TRAV_BLOCK));
     return g;
     }
@@ -924,8 +949,9 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
     D.launch("shade_primary",
              [&]
              {
-                 k_shade_primary<<<blocks(n_pri, 256), 256, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri, Q[1],
-                                                                    qcount, R.records.p);
+                 auto* f = spheres ? k_shade_primary<SHADE_BLOCK, T_SPHERES> : k_shade_primary<SHADE_BLOCK, T_ALL>;
+                 f<<<blocks(n_pri, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri, Q[1],
+                                                                      qcount, R.records.p);
              });
     uint32_t n = 0;
     read_count(D, qcount, n);
@@ -952,8 +978,9 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
         D.launch("shade",
                  [&]
                  {
-                     k_shade<<<blocks(n, 256), 256, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, n, Q[1 - cur], qcount,
-                                                            R.records.p, MAX_DEPTH, ctr);
+                     auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
+                     f<<<blocks(n, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, n, Q[1 - cur],
+                                                                      qcount, R.records.p, MAX_DEPTH, ctr);
                  });
         read_count(D, qcount, n);
         cur = 1 - cur;
@@ -2273,6 +2300,15 @@ extern "C"
                         D.trav_chunk = (uint32_t)value;
                         }
                     }
+                else if (k == "trav_refill")
+                    {
+                    prev = D.trav_refill;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 65, "trav_refill must be in [1, 65]");
+                        D.trav_refill = (uint32_t)value;
+                        }
+                    }
                 else if (k == "tail_rays")
                     {
                     prev = (int64_t)D.tail_rays;
@@ -2287,6 +2323,36 @@ extern "C"
         }
 
     // ---------------------------------------------------------------- diagnostics
+    int fh_debug_copy(fh_device dev, uint64_t bytes, float* ms)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                REQUIRE(bytes >= 16 && bytes % 16 == 0, "bytes must be a positive multiple of 16");
+                DeviceObj& D = dev->obj;
+                const uint64_t n = bytes / 16;
+                DBuf<float4> a, b;
+                a.ensure(n);
+                b.ensure(n);
+                HIPCHECK(hipMemsetAsync(a.p, 0, bytes, D.stream));
+                const uint32_t grid = std::max<uint32_t>(1, D.n_cu) * 16;
+                hipEvent_t e0, e1;
+                HIPCHECK(hipEventCreate(&e0));
+                HIPCHECK(hipEventCreate(&e1));
+                HIPCHECK(hipEventRecord(e0, D.stream));
+                k_debug_copy<<<grid, 256, 0, D.stream>>>(a.p, b.p, n);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipEventRecord(e1, D.stream));
+                HIPCHECK(hipEventSynchronize(e1));
+                float t = 0;
+                HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+                HIPCHECK(hipEventDestroy(e0));
+                HIPCHECK(hipEventDestroy(e1));
+                if (ms)
+                    *ms = t;
+            });
+        }
     int fh_debug_philox(fh_device dev, const uint32_t* in, uint32_t n, uint32_t* out)
         {
         return guard(

@@ -250,16 +250,24 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
 
 /* ------------------------------------------------------ tuning (no reference
  * counterpart).  key "batch_paths": light-sample paths per wavefront batch
- * (default 32 Mi; sizes the ray queues); "tail_rays": below this many live
- * rays a batch finishes its paths in one kernel (default 32768); "stack_lds":
+ * (default 64 Mi; sizes the ray queues); "tail_rays": below this many live
+ * rays a batch finishes its paths in one kernel (default 131072); "stack_lds":
  * traversal stack entries kept in LDS per thread, 8 or 16 (default 8);
  * "trav_chunk": rays a traversal wavefront claims per work-counter atomic, a
- * multiple of 64 up to 65536 (default 256).  value <= 0 only reads the
- * current value into *previous. */
+ * multiple of 64 up to 65536 (default 256); "trav_refill": a traversal lane
+ * whose ray is done takes a new one once this many lanes of its wavefront are
+ * idle, 1..64, or 65 for the batch-synchronous kernel that traces 64 rays at
+ * a time (default 65).  value <= 0 only reads the current value into
+ * *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
 /* ------------------------------------------------------ diagnostics (no
  * reference counterpart; used by the parity tests and the benchmark) */
+/* Copy `bytes` (a multiple of 16) between two fresh device buffers with a
+ * plain streaming kernel (k_debug_copy); *ms = its duration.  It reads and
+ * writes exactly `bytes`, so HBM byte counters of other kernels can be scaled
+ * by what the same counters read for it. */
+int fh_debug_copy(fh_device dev, uint64_t bytes, float* ms);
 /* Philox4x32-10 on the device: in[6k..6k+5] = ctr0..3, key0, key1 -> out[4k..] */
 int fh_debug_philox(fh_device dev, const uint32_t* in, uint32_t n, uint32_t* out);
 /* Primary rays and closest hits of pass `sample` for every pixel of a Path
