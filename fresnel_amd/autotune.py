@@ -13,6 +13,7 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
 * ``trav_chunk``  rays a traversal wavefront claims per work-counter atomic.
 * ``trav_refill`` idle lanes of a traversal wavefront that trigger taking new
                   rays (65: trace 64 rays at a time).
+* ``trav_packet`` primary rays traced as wavefront packets (2) or per ray (1).
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -30,6 +31,7 @@ CANDIDATES = {
     'tail_rays': [8192, 32768, 131072],
     'trav_chunk': [64, 256, 1024],
     'trav_refill': [16, 32, 48, 65],
+    'trav_packet': [1, 2],
 }
 
 

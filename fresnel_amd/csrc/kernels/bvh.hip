@@ -437,6 +437,46 @@ __device__ __forceinline__ float block_sum(float v, float* sh)
     return sh[0];
     }
 
+// Largest primitive half-extent of the scene (sets the traversal's cull margin).
+__device__ __forceinline__ float block_maxf(float v, float* sh)
+    {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1)
+        {
+        if (threadIdx.x < s)
+            sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+        }
+    return sh[0];
+    }
+
+__global__ void k_extent_partial(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t N,
+                                 float* partial)
+    {
+    __shared__ float sh[256];
+    float v = 0.0f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        {
+        const float4 a = lo[i], b = hi[i];
+        v = fmaxf(v, fmaxf(fmaxf(b.x - a.x, b.y - a.y), b.z - a.z) * 0.5f);
+        }
+    v = block_maxf(v, sh);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = v;
+    }
+
+__global__ void k_extent_final(const float* __restrict__ partial, uint32_t n_part, float* out)
+    {
+    __shared__ float sh[256];
+    float v = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n_part; i += blockDim.x)
+        v = fmaxf(v, partial[i]);
+    v = block_maxf(v, sh);
+    if (threadIdx.x == 0)
+        out[0] = v;
+    }
+
 __global__ void k_sah_partial(const Node* __restrict__ nodes, uint32_t n_nodes, float* partial)
     {
     __shared__ float sh[256];

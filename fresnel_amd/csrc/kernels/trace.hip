@@ -35,13 +35,13 @@ __device__ __forceinline__ void slab(v3 o, v3 inv, float lx, float ly, float lz,
     }
 
 // Lower bound on the computed distance of any primitive inside a box whose
-// slab entry is tmin (see CULL_K).  A box is culled when this bound exceeds
-// the closest hit, so every primitive that could still win is tested and the
-// result does not depend on the hierarchy or the visiting order.
-__device__ __forceinline__ float cull_bound(float tmin, float lx, float ly, float lz, float hx, float hy, float hz)
+// slab interval is [tmin, tmax] (see CULL_K).  A box is culled when this
+// bound exceeds the closest hit, so every primitive that could still win is
+// tested and the result does not depend on the hierarchy or the visiting
+// order.  margin = CULL_K * 3 * the scene's largest primitive half-extent.
+__device__ __forceinline__ float cull_bound(float tmin, float tmax, float margin)
     {
-    const float size = ((hx - lx) + (hy - ly)) + (hz - lz);
-    return tmin - CULL_K * (fmaxf(tmin, 0.0f) + size);
+    return tmin - (CULL_K * fmaxf(tmax, 0.0f) + margin);
     }
 
 __device__ __forceinline__ bool better(float t, uint32_t g, uint32_t p, const HitT& b)
@@ -118,11 +118,14 @@ __device__ __forceinline__ bool leaf_test(const GeomDev& g, uint32_t prim, v3 o,
         return intersect_prim<false>(g, prim, o, d, tnear, t, nullptr);
     }
 
+typedef const Node __attribute__((address_space(1))) GlobalNode;
+
 // Closest-hit traversal state of one ray.
 struct TraceState
     {
     v3 o, d, inv;
     float tnear;
+    float margin;
     HitT best;
     uint32_t node;
     };
@@ -134,6 +137,7 @@ __device__ __forceinline__ void trace_begin(const SceneDev& sc, v3 o, v3 d, floa
     r.d = d;
     r.inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     r.tnear = tnear;
+    r.margin = sc.cull_margin;
     r.best.t = __builtin_huge_valf();
     r.best.geom = EMPTY;
     r.best.prim = EMPTY;
@@ -154,18 +158,19 @@ __device__ __forceinline__ bool trace_visit(const Node* __restrict__ nodes, cons
     {
     if (r.node == EMPTY)
         return false;
-    const Node* np = nodes + r.node;
-    const float4 A = np->a_lo;
-    const float4 B = np->a_hi;
-    const float4 C = np->b_box;
-    const uint4 lk = np->link;
+    // global address space: direct global loads instead of flat ones
+    const GlobalNode* np = (const GlobalNode*)(nodes + r.node);
+    const float4 A = make_float4(np->a_lo.x, np->a_lo.y, np->a_lo.z, np->a_lo.w);
+    const float4 B = make_float4(np->a_hi.x, np->a_hi.y, np->a_hi.z, np->a_hi.w);
+    const float4 C = make_float4(np->b_box.x, np->b_box.y, np->b_box.z, np->b_box.w);
+    const uint4 lk = make_uint4(np->link.x, np->link.y, np->link.z, np->link.w);
     if (COUNT)
         n_nodes++;
     float t0min, t0max, t1min, t1max;
     slab(r.o, r.inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
     slab(r.o, r.inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
-    const float c0 = cull_bound(t0min, A.x, A.y, A.z, A.w, B.x, B.y);
-    const float c1 = cull_bound(t1min, B.z, B.w, C.x, C.y, C.z, C.w);
+    const float c0 = cull_bound(t0min, t0max, r.margin);
+    const float c1 = cull_bound(t1min, t1max, r.margin);
     bool h0 = (t0min <= t0max) && (t0max >= r.tnear) && !(c0 > r.best.t);
     bool h1 = (t1min <= t1max) && (t1max >= r.tnear) && !(c1 > r.best.t);
     if (h0 && (lk.x & LEAF))
@@ -620,6 +625,214 @@ __global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, Q
         out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
         out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
         }
+    }
+
+// Primary rays as wavefront packets.  The 64 rays of a wavefront come from
+// one 8x8 pixel tile of one camera sample, so they need nearly the same
+// nodes: the wavefront walks one node sequence (uniform node index, one
+// fetch per visit) and every lane tests the visited node's children against
+// its own ray, with its own closest hit, cull and leaf tests.  A lane acts
+// only on nodes it wants (its own slab and cull tests passed at the parent),
+// so each ray's result is the one the per-ray search returns; the wavefront
+// descends into a child when any lane wants it, nearer child first by lane
+// majority, and pushes the other with each lane's own cull bound.  Groups of
+// 64 rays that straddle two jobs fall back to the per-ray search.
+constexpr uint32_t UNWANTED = 0x80000000u; // stack flag: this lane does not need the node
+typedef const float __attribute__((address_space(4))) ConstF;
+typedef const uint32_t __attribute__((address_space(4))) ConstU;
+typedef const GeomDev __attribute__((address_space(4))) ConstGeom;
+
+// Leaf test of a primitive every lane of the wavefront shares: the geometry
+// record and, for spheres, the sphere arrive through scalar loads.
+template<uint32_t TYPES>
+__device__ __forceinline__ bool leaf_test_uniform(const GeomDev* geoms, uint32_t slot, uint32_t prim, v3 o, v3 d,
+                                                  float tnear, float& t)
+    {
+    if constexpr (TYPES == T_SPHERES)
+        {
+        const ConstGeom* g = (const ConstGeom*)geoms + slot;
+        const ConstF* sp = (const ConstF*)g->a + 4 * (size_t)prim;
+        float de;
+        bool first;
+        return sphere_t(o, d, mk3(sp[0], sp[1], sp[2]), sp[3], t, de, first) && (tnear < t);
+        }
+    else
+        return leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
+    }
+
+template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
+__global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restrict__ jobs, uint32_t n_jobs,
+                                                           uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
+                                                           uint2* gstack, uint32_t gstride,
+                                                           unsigned long long* counters)
+    {
+    __shared__ uint2 lds[SS * BLOCK];
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    LdsStack<SS> st;
+    st.lds = lds + threadIdx.x;
+    st.lstride = BLOCK;
+    st.glob = gstack + gtid;
+    st.gstride = gstride;
+    st.sp = 0;
+    const uint32_t lane = lane_id();
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    uint32_t claim = 0;
+    if (lane == 0)
+        claim = atomicAdd(work, chunk);
+    uint32_t base = __shfl(claim, 0, 64);
+    while (base < n)
+        {
+        uint32_t next = 0;
+        if (lane == 0)
+            next = atomicAdd(work, chunk);
+        const uint32_t end = min(n, base + chunk);
+        for (uint32_t sub = base; sub < end; sub += 64)
+            {
+            const uint32_t i = sub + lane;
+            const bool valid = i < end;
+            v3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
+            uint32_t job = 0;
+            if (valid)
+                {
+                uint32_t k, pixel;
+                primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+                }
+            const uint32_t job0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);
+            HitT best;
+            if (__ballot(valid && job != job0) != 0)
+                {
+                if (valid)
+                    trace_closest<LdsStack<SS>, COUNT, TYPES>(jobs[job].scene, o, d, 1e-3f, best, st, n_nodes,
+                                                              n_prims, err);
+                }
+            else
+                {
+                const SceneDev& sc = jobs[job0].scene;
+                const Node* __restrict__ nodes = sc.nodes;
+                const GeomDev* __restrict__ geoms = sc.geoms;
+                const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                const float tnear = 1e-3f;
+                const float margin = sc.cull_margin;
+                best.t = __builtin_huge_valf();
+                best.geom = EMPTY;
+                best.prim = EMPTY;
+                st.sp = 0;
+                uint32_t node = sc.root;
+                bool want = valid;
+                while (node != EMPTY)
+                    {
+                    // the node index is uniform: fetch through the constant address
+                    // space so the node arrives in scalar registers
+                    node = (uint32_t)__builtin_amdgcn_readfirstlane((int)node);
+                    const ConstF* nf = (const ConstF*)(nodes + node);
+                    const ConstU* nu = (const ConstU*)(nodes + node);
+                    const float4 A = make_float4(nf[0], nf[1], nf[2], nf[3]);
+                    const float4 B = make_float4(nf[4], nf[5], nf[6], nf[7]);
+                    const float4 C = make_float4(nf[8], nf[9], nf[10], nf[11]);
+                    const uint4 lk = make_uint4(nu[12], nu[13], nu[14], nu[15]);
+                    if (COUNT && want)
+                        n_nodes++;
+                    float t0min, t0max, t1min, t1max;
+                    slab(o, inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
+                    slab(o, inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
+                    const float c0 = cull_bound(t0min, t0max, margin);
+                    const float c1 = cull_bound(t1min, t1max, margin);
+                    bool w0 = want && (t0min <= t0max) && (t0max >= tnear) && !(c0 > best.t);
+                    bool w1 = want && (t1min <= t1max) && (t1max >= tnear) && !(c1 > best.t);
+                    if (lk.x & LEAF)
+                        {
+                        if (w0)
+                            {
+                            const uint32_t prim = lk.x & ~LEAF;
+                            float t;
+                            if (COUNT)
+                                n_prims++;
+                            if (leaf_test_uniform<TYPES>(geoms, lk.z, prim, o, d, tnear, t) && better(t, lk.z, prim, best))
+                                {
+                                best.t = t;
+                                best.geom = lk.z;
+                                best.prim = prim;
+                                }
+                            }
+                        w0 = false;
+                        }
+                    if (lk.y & LEAF)
+                        {
+                        if (w1)
+                            {
+                            const uint32_t prim = lk.y & ~LEAF;
+                            float t;
+                            if (COUNT)
+                                n_prims++;
+                            if (leaf_test_uniform<TYPES>(geoms, lk.w, prim, o, d, tnear, t) && better(t, lk.w, prim, best))
+                                {
+                                best.t = t;
+                                best.geom = lk.w;
+                                best.prim = prim;
+                                }
+                            }
+                        w1 = false;
+                        }
+                    w0 = w0 && !(c0 > best.t);
+                    w1 = w1 && !(c1 > best.t);
+                    const uint64_t b0 = __ballot(w0), b1 = __ballot(w1);
+                    if (b0 && b1)
+                        {
+                        const uint64_t both = b0 & b1;
+                        const uint64_t pref1 = __ballot(w0 && w1 && t1min < t0min);
+                        const bool near1 = 2 * __popcll(pref1) > __popcll(both);
+                        const uint32_t far_node = near1 ? lk.x : lk.y;
+                        const bool far_want = near1 ? w0 : w1;
+                        st.push(far_want ? far_node : (far_node | UNWANTED), near1 ? c0 : c1, err);
+                        node = near1 ? lk.y : lk.x;
+                        want = near1 ? w1 : w0;
+                        continue;
+                        }
+                    if (b0)
+                        {
+                        node = lk.x;
+                        want = w0;
+                        continue;
+                        }
+                    if (b1)
+                        {
+                        node = lk.y;
+                        want = w1;
+                        continue;
+                        }
+                    node = EMPTY;
+                    while (st.sp > 0)
+                        {
+                        uint32_t e;
+                        float cv;
+                        st.pop(e, cv);
+                        const bool w = !(e & UNWANTED) && !(cv > best.t);
+                        if (__ballot(w) != 0)
+                            {
+                            node = (uint32_t)__builtin_amdgcn_readfirstlane((int)e) & ~UNWANTED;
+                            want = w;
+                            break;
+                            }
+                        }
+                    }
+                }
+            if (valid)
+                {
+                hits[i] = make_uint4(__float_as_uint(best.t), best.geom, best.prim, 0);
+                if (COUNT)
+                    n_rays++;
+                }
+            }
+        base = __shfl(next, 0, 64);
+        }
+    if (COUNT)
+        {
+        atomicAdd(counters + CTR_PRIMARY + 0, (unsigned long long)n_rays);
+        atomicAdd(counters + CTR_PRIMARY + 1, (unsigned long long)n_nodes);
+        atomicAdd(counters + CTR_PRIMARY + 2, (unsigned long long)n_prims);
+        }
+    if (err)
+        atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
     }
 
 // Persistent traversal with per-lane refill: a lane whose ray is finished

@@ -99,10 +99,12 @@ __device__ inline void prim_bounds(const GeomDev& g, uint32_t prim, v3& lo, v3& 
 // (b*b - v.v + r*r) cancels for distant grazing hits: in float arithmetic
 // t = b - sqrt(det) can fall below the true entry distance by up to about
 // sqrt(14 u)|v| ~ 9.2e-4 |v| (u = 2^-24, |v| = distance from the ray origin to
-// the centre), and the spherocylinder caps use the same solve.  A box whose
-// slab entry is tmin holds no primitive with a computed distance below
-// tmin - CULL_K (max(tmin, 0) + L1 size of the box); CULL_K = 2^-9 keeps a
-// factor two over that bound.
+// the centre), and the spherocylinder caps use the same solve.  For a
+// primitive inside a box with slab interval [tmin, tmax], tmin >= 0, a computed
+// hit needs the ray to pass within about its radius of the centre, so
+// |v| <= tmax + 3 R with R the scene's largest primitive half-extent; no
+// primitive of the box has a computed distance below
+// tmin - (CULL_K max(tmax, 0) + CULL_K 3 R).  CULL_K = 2^-9 keeps a factor two.
 constexpr float CULL_K = 0.001953125f; // 2^-9
 
 __device__ __forceinline__ bool sphere_t(v3 o, v3 d, v3 p, float r, float& t, float& d_edge, bool& first)

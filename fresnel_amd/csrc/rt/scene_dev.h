@@ -82,7 +82,7 @@ struct SceneDev
     uint32_t root;      // EMPTY when the scene has no primitive
     uint32_t n_geoms;
     uint32_t n_prims;
-    uint32_t pad;
+    float cull_margin;  // CULL_K * 3 * largest primitive half-extent
     };
 
 // Camera constants derived on the host from the UserCamera

@@ -5,6 +5,40 @@
 
 namespace fh
 {
+// Shading arithmetic (BRDF, lobe sampling and pdfs, MIS weights, Russian
+// roulette, transmission) runs on the hardware reciprocal, reciprocal square
+// root, square root and sine/cosine (v_rcp/v_rsq/v_sqrt/v_sin/v_cos_f32, about
+// one ulp; sine and cosine take revolutions).  These values never feed a
+// bit-exact comparison: camera rays and intersections keep IEEE division and
+// square root, and images are compared with the CPU checker within an RMSE
+// bound, as the device's cosf/sinf/powf already differ from the host's.
+__device__ __forceinline__ float sh_rcp(float x)
+    {
+    return __builtin_amdgcn_rcpf(x);
+    }
+__device__ __forceinline__ float sh_div(float a, float b)
+    {
+    return a * __builtin_amdgcn_rcpf(b);
+    }
+__device__ __forceinline__ float sh_sqrt(float x)
+    {
+    return __builtin_amdgcn_sqrtf(x);
+    }
+__device__ __forceinline__ v3 sh_normalize(v3 a)
+    {
+    return a * __builtin_amdgcn_rsqf(dot(a, a));
+    }
+// cos(2 pi x), sin(2 pi x)
+__device__ __forceinline__ float sh_cos_rev(float x)
+    {
+    return __builtin_amdgcn_cosf(x);
+    }
+__device__ __forceinline__ float sh_sin_rev(float x)
+    {
+    return __builtin_amdgcn_sinf(x);
+    }
+constexpr float INV_PI_F = 0.318309886f;
+
 __device__ __forceinline__ v3 mat_color(const Mat& m)
     {
     return mk3(m.cr, m.cg, m.cb);
@@ -28,7 +62,7 @@ __device__ __forceinline__ v3 disney_diffuse(const Mat& m, v3 base_color, float 
     const float FL = schlick(ndotl);
     const float FV = schlick(ndotv);
     const float RR = 2 * m.roughness * ldoth * ldoth;
-    return div_rcp(base_color, PI_F)
+    return base_color * INV_PI_F
            * ((1.0f - 0.5f * FL) * (1.0f - 0.5f * FV) + RR * (FL + FV + FL * FV * (RR - 1.0f)));
     }
 
@@ -39,7 +73,7 @@ __device__ inline v3 brdf(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
     if (ndotv <= 0)
         return mk3(0, 0, 0);
     v3 h = l + v;
-    h = div_true(h, sqrtf(dot(h, h)));
+    h = sh_normalize(h);
     const float ndotl = dot(n, l);
     const float ldoth = dot(l, h);
     const float ndoth = dot(n, h);
@@ -48,14 +82,14 @@ __device__ inline v3 brdf(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
     const float alpha = m.roughness * m.roughness;
     const float alpha_sq = alpha * alpha;
     const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
-    const float D = alpha_sq / (PI_F * denom_rt * denom_rt);
+    const float D = sh_div(alpha_sq, PI_F * denom_rt * denom_rt);
     const float f0 = 0.08f * m.specular;
     const v3 F0 = lerp(m.metal, mk3(f0, f0, f0), base_color);
     const v3 F = F0 + (mk3(1.0f, 1.0f, 1.0f) - F0) * schlick(ldoth);
     const float ndotv_sq = ndotv * ndotv;
     const float ndotl_sq = ndotl * ndotl;
-    const float V1 = 1.0f / (ndotv + sqrtf(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
-    const float V2 = 1.0f / (ndotl + sqrtf(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
+    const float V1 = sh_rcp(ndotv + sh_sqrt(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
+    const float V2 = sh_rcp(ndotl + sh_sqrt(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
     const float V = V1 * V2;
     const v3 f_s = D * F * V;
     return f_d * (1.0f - m.metal) + f_s;
@@ -68,7 +102,7 @@ __device__ inline v3 brdf_diffuse(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
     if (ndotv <= 0 || m.metal == 1.0f)
         return mk3(0, 0, 0);
     v3 h = l + v;
-    h = div_true(h, sqrtf(dot(h, h)));
+    h = sh_normalize(h);
     const float ndotl = dot(n, l);
     const float ldoth = dot(l, h);
     return disney_diffuse(m, get_color(m, sc), ndotl, ndotv, ldoth) * (1.0f - m.metal);
@@ -83,7 +117,7 @@ __device__ inline v3 brdf_specular(const Mat& m, v3 l, v3 v, v3 n, v3 sc, float 
     if (ndotv <= 0)
         return mk3(0, 0, 0);
     v3 h = l + v;
-    h = div_true(h, sqrtf(dot(h, h)));
+    h = sh_normalize(h);
     const float ndotl = dot(n, l);
     const float ldoth = dot(l, h);
     const float ndoth = dot(n, h);
@@ -91,7 +125,7 @@ __device__ inline v3 brdf_specular(const Mat& m, v3 l, v3 v, v3 n, v3 sc, float 
     const float alpha = m.roughness * m.roughness;
     const float alpha_sq = alpha * alpha;
     const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
-    float D = alpha_sq / (PI_F * denom_rt * denom_rt);
+    float D = sh_div(alpha_sq, PI_F * denom_rt * denom_rt);
     if (light_half_angle > 0.99f * PI_F / 2.0f)
         light_half_angle = 0.99f * PI_F / 2.0f;
     if (light_half_angle > 0.0f)
@@ -99,15 +133,15 @@ __device__ inline v3 brdf_specular(const Mat& m, v3 l, v3 v, v3 n, v3 sc, float 
         float alpha_prime = alpha + 0.5f * tan_clamped;
         if (alpha_prime > 1.0f)
             alpha_prime = 1.0f;
-        D = D * alpha_sq / (alpha_prime * alpha_prime);
+        D = sh_div(D * alpha_sq, alpha_prime * alpha_prime);
         }
     const float f0 = 0.08f * m.specular;
     const v3 F0 = lerp(m.metal, mk3(f0, f0, f0), base_color);
     const v3 F = F0 + (mk3(1.0f, 1.0f, 1.0f) - F0) * schlick(ldoth);
     const float ndotv_sq = ndotv * ndotv;
     const float ndotl_sq = ndotl * ndotl;
-    const float V1 = 1.0f / (ndotv + sqrtf(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
-    const float V2 = 1.0f / (ndotl + sqrtf(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
+    const float V1 = sh_rcp(ndotv + sh_sqrt(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
+    const float V2 = sh_rcp(ndotl + sh_sqrt(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
     return D * F * (V1 * V2);
     }
 
@@ -119,7 +153,7 @@ __device__ __forceinline__ void tangent_frame(v3 n, v3& t_x, v3& t_y)
     if (fabsf(n.z) >= 0.999f)
         up = mk3(1.0f, 0, 0);
     t_x = cross(up, n);
-    t_x = div_rcp(t_x, sqrtf(dot(t_x, t_x)));
+    t_x = sh_normalize(t_x);
     t_y = cross(n, t_x);
     }
 
@@ -127,10 +161,10 @@ __device__ __forceinline__ void tangent_frame(v3 n, v3& t_x, v3& t_y)
 __device__ inline v3 sample_ggx(const Mat& m, float xi_x, float xi_y, v3 v, v3 n)
     {
     const float alpha = m.roughness * m.roughness;
-    const float phi = 2.0f * PI_F * xi_x;
-    const float cos_theta = sqrtf((1.0f - xi_y) / (1.0f + (alpha * alpha - 1.0f) * xi_y));
-    const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
-    const v3 h_t = mk3(sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta);
+    // phi = 2 pi xi_x
+    const float cos_theta = sh_sqrt(sh_div(1.0f - xi_y, 1.0f + (alpha * alpha - 1.0f) * xi_y));
+    const float sin_theta = sh_sqrt(1.0f - cos_theta * cos_theta);
+    const v3 h_t = mk3(sin_theta * sh_cos_rev(xi_x), sin_theta * sh_sin_rev(xi_x), cos_theta);
     v3 t_x, t_y;
     tangent_frame(n, t_x, t_y);
     const v3 h = t_x * h_t.x + t_y * h_t.y + n * h_t.z;
@@ -140,22 +174,22 @@ __device__ inline v3 sample_ggx(const Mat& m, float xi_x, float xi_y, v3 v, v3 n
 __device__ inline float pdf_ggx(const Mat& m, v3 l, v3 v, v3 n)
     {
     v3 h = l + v;
-    h = div_true(h, sqrtf(dot(h, h)));
+    h = sh_normalize(h);
     const float ldoth = dot(l, h);
     const float ndoth = dot(n, h);
     const float alpha = m.roughness * m.roughness;
     const float alpha_sq = alpha * alpha;
     const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
-    const float D = alpha_sq / (PI_F * denom_rt * denom_rt);
-    const float pdf_l = D * ndoth / (4.0f * ldoth);
+    const float D = sh_div(alpha_sq, PI_F * denom_rt * denom_rt);
+    const float pdf_l = sh_div(D * ndoth, 4.0f * ldoth);
     return (pdf_l > 0.0f) ? pdf_l : 0.0f;
     }
 // Material::importanceSampleDiffuse, Material.h:282-304
 __device__ inline v3 sample_diffuse(float xi_x, float xi_y, v3 n)
     {
-    const float r = sqrtf(xi_x);
-    const float theta = 2 * PI_F * xi_y;
-    const v3 v_t = mk3(r * cosf(theta), r * sinf(theta), sqrtf(1.0f - xi_x));
+    const float r = sh_sqrt(xi_x);
+    // theta = 2 pi xi_y
+    const v3 v_t = mk3(r * sh_cos_rev(xi_y), r * sh_sin_rev(xi_y), sh_sqrt(1.0f - xi_x));
     v3 t_x, t_y;
     tangent_frame(n, t_x, t_y);
     return t_x * v_t.x + t_y * v_t.y + n * v_t.z;
@@ -164,7 +198,7 @@ __device__ inline v3 sample_diffuse(float xi_x, float xi_y, v3 n)
 __device__ __forceinline__ float pdf_diffuse(v3 l, v3 n)
     {
     const float ndotl = dot(n, l);
-    return (ndotl > 0.0f) ? ndotl / PI_F : 0.0f;
+    return (ndotl > 0.0f) ? ndotl * INV_PI_F : 0.0f;
     }
 
 // ---------------------------------------------------------------- camera
@@ -220,7 +254,7 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
     {
     add_valid = false;
     const v3 Ng = at.Ng;
-    const v3 n = Ng * (1.0f / sqrtf(dot(Ng, Ng)));
+    const v3 n = sh_normalize(Ng);
     const v3 v = -rd;
     if (m.solid > 0.5f)
         {
@@ -244,21 +278,21 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
         l = sample_diffuse(xi_x, xi_y, n);
         const float pd = pdf_diffuse(l, n);
         const float pg = pdf_ggx(m, l, v, n);
-        const float w_d = pd / (pd + pg);
-        factor = w_d / (0.5f * pd);
+        const float w_d = sh_div(pd, pd + pg);
+        factor = sh_div(w_d, 0.5f * pd);
         }
     else
         {
         l = sample_ggx(m, xi_x, xi_y, v, n);
         const float pd = pdf_diffuse(l, n);
         const float pg = pdf_ggx(m, l, v, n);
-        const float w_g = pg / (pd + pg);
-        factor = w_g / (0.5f * pg);
+        const float w_g = sh_div(pg, pd + pg);
+        factor = sh_div(w_g, 0.5f * pg);
         }
     if (transmit)
         {
         const v3 tc = get_color(m, at.color);
-        att = att * mk3(sqrtf(tc.x), sqrtf(tc.y), sqrtf(tc.z));
+        att = att * mk3(sh_sqrt(tc.x), sh_sqrt(tc.y), sh_sqrt(tc.z));
         }
     else
         {
@@ -279,7 +313,7 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
     if (u01(rr.v[0]) > p_continue)
         done = true;
     else
-        att = div_true(att, p_continue);
+        att = att * sh_rcp(p_continue);
     return !done;
     }
 

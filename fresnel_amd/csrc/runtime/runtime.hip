@@ -161,6 +161,7 @@ struct DeviceObj
     uint64_t batch_paths = 64ull << 20; // light-sample paths per batch (sizes the queues)
     uint32_t tail_rays = 131072;        // finish paths in one kernel below this many rays
     uint32_t trav_chunk = 256;          // rays a traversal wavefront claims per work-counter atomic
+    uint32_t trav_packet = 2;           // primary rays: 2 wavefront packets, 1 per ray
     uint32_t trav_refill = 65;          // lanes take new rays once this many of a wavefront are idle
                                         // (65: batch-synchronous kernel, 64 rays at a time)
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
@@ -268,6 +269,19 @@ struct TravArgs
 template<int SS, bool C, bool P, uint32_t T>
 static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const TravArgs& a)
     {
+    if (P && D.trav_packet == 2)
+        {
+        auto kern = &k_traverse_packet<TRAV_BLOCK, SS, C, T>;
+        const uint32_t grid = D.grid_for(reinterpret_cast<const void*>(kern), TRAV_BLOCK);
+        REQUIRE((uint64_t)grid * TRAV_BLOCK <= a.gstride, "traversal grid exceeds the overflow stacks");
+        D.launch(nm,
+                 [&]
+                 {
+                     kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.n, a.hits, a.work, a.chunk, a.gstack,
+                                                     a.gstride, a.counters);
+                 });
+        return;
+        }
     if (D.trav_refill <= 64)
         {
         auto kern = &k_traverse_dyn<TRAV_BLOCK, SS, C, P, T>;
@@ -339,7 +353,11 @@ template<int SS> static uint32_t max_grid_ss(DeviceObj& D)
                           reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, false, false, T_ALL>),
                           reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, false, true, T_ALL>),
                           reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, true, false, T_ALL>),
-                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, true, true, T_ALL>)})
+                          reinterpret_cast<const void*>(&k_traverse_dyn<TRAV_BLOCK, SS, true, true, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse_packet<TRAV_BLOCK, SS, false, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse_packet<TRAV_BLOCK, SS, true, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_traverse_packet<TRAV_BLOCK, SS, false, T_ALL>),
+                          reinterpret_cast<const void*>(&k_traverse_packet<TRAV_BLOCK, SS, true, T_ALL>)})
         g = std::max(g, D.grid_for(k, TRAV_BLOCK));
     return g;
     }
@@ -415,7 +433,8 @@ struct SceneObj
     uint32_t n_slots = 0;
     DBuf<float4> blo, bhi;
     DBuf<uint32_t> keys, vals, keys2, vals2, hist, sums, parent, flags, prefix, slots;
-    DBuf<float> partial, cb, root_box;
+    DBuf<float> partial, cb, root_box, extent;
+    float cull_margin = 0.0f; // CULL_K * 3 * largest primitive half-extent
     DBuf<Node> nodes;
     uint32_t n_prims = 0;
     bool bvh_valid = false;
@@ -574,6 +593,21 @@ static void build_bvh(SceneObj& sc, bool refit_only)
                  k_prim_bounds<<<blocks(N, 256), 256, 0, s>>>(sc.dgeoms.p, sc.prefix.p, sc.slots.p, na, N,
                                                                 sc.blo.p, sc.bhi.p);
              });
+    {
+    // largest primitive half-extent -> the traversal's cull margin
+    const uint32_t n_part = std::min<uint32_t>(blocks(N, 256), 1024);
+    sc.extent.ensure(n_part + 1);
+    D.launch("lbvh_extent",
+             [&]
+             {
+                 k_extent_partial<<<n_part, 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, sc.extent.p + 1);
+                 k_extent_final<<<1, 256, 0, s>>>(sc.extent.p + 1, n_part, sc.extent.p);
+             });
+    float r_max = 0.0f;
+    HIPCHECK(hipMemcpyAsync(&r_max, sc.extent.p, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    sc.cull_margin = CULL_K * (3.0f * r_max);
+    }
     sc.nodes.ensure(std::max<uint32_t>(N - 1, 1));
     sc.root_box.ensure(6);
     if (N == 1)
@@ -758,7 +792,7 @@ static SceneDev scene_dev(const SceneObj& sc)
     d.root = sc.n_prims ? 0u : EMPTY;
     d.n_geoms = sc.n_slots;
     d.n_prims = sc.n_prims;
-    d.pad = 0;
+    d.cull_margin = sc.cull_margin;
     return d;
     }
 
@@ -2298,6 +2332,15 @@ extern "C"
                         {
                         REQUIRE(value % 64 == 0 && value <= 65536, "trav_chunk must be a multiple of 64, at most 65536");
                         D.trav_chunk = (uint32_t)value;
+                        }
+                    }
+                else if (k == "trav_packet")
+                    {
+                    prev = D.trav_packet;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 2, "trav_packet must be 1 (per ray) or 2 (wavefront packets)");
+                        D.trav_packet = (uint32_t)value;
                         }
                     }
                 else if (k == "trav_refill")

@@ -257,8 +257,9 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * multiple of 64 up to 65536 (default 256); "trav_refill": a traversal lane
  * whose ray is done takes a new one once this many lanes of its wavefront are
  * idle, 1..64, or 65 for the batch-synchronous kernel that traces 64 rays at
- * a time (default 65).  value <= 0 only reads the current value into
- * *previous. */
+ * a time (default 65); "trav_packet": primary rays traced as wavefront
+ * packets (2, default) or one by one (1).  value <= 0 only reads the current
+ * value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
 /* ------------------------------------------------------ diagnostics (no

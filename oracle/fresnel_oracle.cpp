@@ -607,16 +607,19 @@ static inline void slab(v3 o, v3 inv, const Box& b, float& tmin, float& tmax)
     }
 // Cull rule by the current closest hit.  The reference's sphere solve
 // (IntersectSphere.h: det = b*b - v.v + r*r) cancels for distant grazing hits,
-// so the computed t can lie below the box's slab entry by up to about
-// 9.2e-4 |v| (|v| = distance from the ray origin to the centre).  A box whose
-// slab entry is tmin holds no primitive with a computed distance below
-// cull_bound(); boxes are culled only when that bound exceeds the closest hit,
-// which makes the result independent of the hierarchy and of visiting order.
+// so the computed t can lie below the true entry distance by up to about
+// sqrt(14 u)|v| ~ 9.2e-4 |v| (u = 2^-24, |v| = distance from the ray origin to
+// the centre).  For a primitive inside a box whose slab interval is
+// [tmin, tmax] with tmin >= 0, a computed hit needs the ray to pass within
+// about its radius of the centre, so |v| <= tmax + 3 R, R the largest
+// primitive half-extent of the scene.  No primitive of the box then has a
+// computed distance below cull_bound(); boxes are culled only when that bound
+// exceeds the closest hit, which makes the result independent of the
+// hierarchy and of visiting order.  CULL_K = 2^-9 keeps a factor two.
 static const float CULL_K = 0x1p-9f;
-static inline float cull_bound(float tmin, const Box& b)
+static inline float cull_bound(float tmin, float tmax, float margin)
     {
-    const float size = ((b.hi.x - b.lo.x) + (b.hi.y - b.lo.y)) + (b.hi.z - b.lo.z);
-    return tmin - CULL_K * (fmaxf(tmin, 0.0f) + size);
+    return tmin - (CULL_K * fmaxf(tmax, 0.0f) + margin);
     }
 
 // ---------------------------------------------------------------- intersectors
@@ -1099,6 +1102,7 @@ struct Scene
     std::vector<PrimRef> refs;
     std::vector<Box> ref_box;
     std::vector<Node> nodes;
+    float cull_margin = 0.0f; // CULL_K * 3 * largest primitive half-extent
     bool brute = false;
     };
 
@@ -1168,8 +1172,13 @@ static void commit(Scene& s)
             boxes.push_back(prim_bounds(g, p));
             }
         }
+    s.cull_margin = 0.0f;
     if (s.refs.empty())
         return;
+    float r_max = 0.0f;
+    for (const Box& b : boxes)
+        r_max = fmaxf(r_max, fmaxf(fmaxf(b.hi.x - b.lo.x, b.hi.y - b.lo.y), b.hi.z - b.lo.z) * 0.5f);
+    s.cull_margin = CULL_K * (3.0f * r_max);
     std::vector<unsigned> idx(s.refs.size());
     for (unsigned k = 0; k < idx.size(); k++)
         idx[k] = k;
@@ -1214,7 +1223,7 @@ static inline void test_ref(const Scene& s, unsigned r, v3 o, v3 d, v3 inv, floa
     slab(o, inv, s.ref_box[r], tmin, tmax);
     if (!(tmin <= tmax && tmax >= tnear))
         return;
-    if (cull && best.hit && cull_bound(tmin, s.ref_box[r]) > best.t)
+    if (cull && best.hit && cull_bound(tmin, tmax, s.cull_margin) > best.t)
         return;
     const PrimRef& pr = s.refs[r];
     const Geom& g = s.geoms[pr.geom];
@@ -1254,7 +1263,7 @@ static HitRec closest_hit(const Scene& s, v3 o, v3 d, float tnear)
         slab(o, inv, n.box, tmin, tmax);
         if (!(tmin <= tmax && tmax >= tnear))
             continue;
-        if (best.hit && cull_bound(tmin, n.box) > best.t)
+        if (best.hit && cull_bound(tmin, tmax, s.cull_margin) > best.t)
             continue;
         if (n.left < 0)
             {
