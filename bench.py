@@ -291,19 +291,15 @@ def run_plain(args):
         print(f"# warmup {k}: {time.perf_counter() - t0:.3f} s", file=sys.stderr, flush=True)
 
     times = []
-    trav_ms = []
-    trav_launches = []
-    stats = {}
+    kstats = []
     for k in range(args.steps):
         t0 = time.perf_counter()
         tracer.sample(scene, samples=S, light_samples=L)
         dt = time.perf_counter() - t0
         stats = d.statsLast()
         times.append(dt)
-        trav_ms.append(stats.get('traverse', (0.0, 0))[0])
-        trav_launches.append(stats.get('traverse', (0.0, 0))[1])
-        print(f"# step {k}: {dt:.3f} s, traverse {trav_ms[-1]:.1f} ms in {trav_launches[-1]} launches",
-              file=sys.stderr, flush=True)
+        kstats.append(stats)
+        log(f"step {k}: {dt:.3f} s, " + ", ".join(f"{n} {v[0]:.1f} ms/{v[1]}" for n, v in stats.items()))
     mean_s = sum(times) / len(times)
     paths = W * H * S * L
     value = paths / mean_s / 1e6
@@ -314,26 +310,40 @@ def run_plain(args):
     ctr = d.countersLast()
     d.countersEnable(False)
     pb = wl['prim_bytes']
-    # dominant kernel: secondary-ray traversal.  Algorithmic bytes per ray:
-    # 64 B per BVH node visited, prim_bytes per primitive tested, 32 B of ray
-    # read from the queue and 16 B of hit written.
-    rays, nodes, prims = ctr[0], ctr[1], ctr[2]
-    alg_bytes = 64 * nodes + pb * prims + 48 * rays
-    t_trav_s = (sum(trav_ms) / len(trav_ms)) / 1e3
-    n_launch = max(1, trav_launches[-1])
-    achieved = alg_bytes / t_trav_s / 1e9 if t_trav_s > 0 else 0.0
+    traffic = {}
+    tf = os.path.join(ROOT, 'profiles', f'traffic_{args.workload}.json')
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get('hbm_bytes_per_launch', {})
+
+    def kernel_roofline(name, rays, nodes, prims, ray_bytes, what):
+        """Algorithmic bytes per launch (64 B per BVH node visited, prim_bytes per
+        primitive tested, ray_bytes per ray) over the kernel's mean launch time."""
+        ms = sum(st.get(name, (0.0, 0))[0] for st in kstats) / len(kstats)
+        launches = max(1, kstats[-1].get(name, (0.0, 1))[1])
+        alg = 64 * nodes + pb * prims + ray_bytes * rays
+        achieved = alg / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(name),
+                "kernel": what, "alg_bytes_per_launch": int(alg / launches), "launches_per_step": int(launches),
+                "avg_launch_ms": round(ms / launches, 3), "rays_per_launch": int(rays / launches),
+                "nodes_per_ray": round(nodes / max(rays, 1), 2), "prims_per_ray": round(prims / max(rays, 1), 2),
+                "ms_per_step": round(ms, 3)}
+
+    # primary rays are generated in the kernel and write a 16 B hit; secondary
+    # rays read 32 B of ray from the queue and write a 16 B hit
+    roof = {"traverse_primary": kernel_roofline("traverse_primary", ctr[3], ctr[4], ctr[5], 16,
+                                                "primary-ray traversal (k_traverse_packet / k_traverse)"),
+            "traverse": kernel_roofline("traverse", ctr[0], ctr[1], ctr[2], 48,
+                                        "secondary-ray traversal (k_traverse)")}
+    dominant = max(roof, key=lambda k: roof[k]["ms_per_step"])
     # whole path: every traced segment (primary, secondary, tail) plus 36 B of
     # accumulation per pixel-sample, over the frame time
     seg_rays = ctr[0] + ctr[3] + ctr[6]
     path_bytes = (64 * (ctr[1] + ctr[4] + ctr[7]) + pb * (ctr[2] + ctr[5] + ctr[8])
                   + 36 * W * H * S)
-    traffic = None
-    tf = os.path.join(ROOT, 'profiles', f'traffic_{args.workload}.json')
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get('hbm_bytes_per_launch')
-    kernel_ms = {k: round(v[0], 3) for k, v in stats.items()}
-    kernel_launches = {k: int(v[1]) for k, v in stats.items()}
+    kernel_ms = {k: round(v[0], 3) for k, v in kstats[-1].items()}
+    kernel_launches = {k: int(v[1]) for k, v in kstats[-1].items()}
 
     out = base_record(round(value, 3), "Mpaths/s", args.steps, args.warmup, round(mean_s * 1e3, 2),
                       {"workload": wl['name'], "mode": "plain", "paths_per_frame": paths,
@@ -341,20 +351,11 @@ def run_plain(args):
     out.update({
         "latency_ms": {"p50": round(float(numpy.percentile(times, 50)) * 1e3, 2),
                        "p99": round(float(numpy.percentile(times, 99)) * 1e3, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_traverse (secondary rays)", "alg_bytes_per_launch": int(alg_bytes / n_launch),
-                     "launches_per_step": int(n_launch),
-                     "avg_launch_ms": round(t_trav_s * 1e3 / n_launch, 3),
-                     "rays_per_launch": int(rays / n_launch),
-                     "nodes_per_ray": round(nodes / max(rays, 1), 2),
-                     "prims_per_ray": round(prims / max(rays, 1), 2)},
+        "roofline": roof[dominant],
+        "roofline_other": {k: v for k, v in roof.items() if k != dominant},
         "path_roofline": {"alg_bytes_per_path": round(path_bytes / paths, 1),
                           "achieved_GBs": round(path_bytes / mean_s / 1e9, 1),
                           "frac": round(path_bytes / mean_s / 1e9 / HBM_PEAK_GBS, 4)},
-        "primary_traversal": {"rays": int(ctr[3]), "nodes_per_ray": round(ctr[4] / max(ctr[3], 1), 2),
-                              "prims_per_ray": round(ctr[5] / max(ctr[3], 1), 2),
-                              "ms": kernel_ms.get('traverse_primary')},
         "tail": {"rays": int(ctr[6]), "nodes_per_ray": round(ctr[7] / max(ctr[6], 1), 2)},
         "kernel_ms_last_step": kernel_ms,
         "kernel_launches_last_step": kernel_launches,
@@ -614,7 +615,7 @@ def main():
     ap.add_argument('--frames', type=int, default=64, help='c4: trajectory frames')
     ap.add_argument('--requests', type=int, default=24, help='c5: requests per stream')
     ap.add_argument('--rates', default='5,10,20,50,100,200', help='c5: arrival rates (req/s)')
-    ap.add_argument('--target-paths', type=int, default=32 << 20, help='c5: paths per server step')
+    ap.add_argument('--target-paths', type=int, default=64 << 20, help='c5: paths per server step')
     ap.add_argument('--no-loop', action='store_true', help='c5: skip the render()-per-sample plain form')
     args = ap.parse_args()
     if args.gpus != 1:

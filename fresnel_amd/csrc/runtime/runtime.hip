@@ -1028,6 +1028,28 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
     HIPCHECK(hipStreamSynchronize(s));
     }
 
+// Run jobs whose scenes hold spheres only and the other jobs as two batches,
+// so the spheres-only jobs use the spheres-only kernels (fewer registers).
+static void run_grouped(DeviceObj& D, const std::vector<JobDev>& jobs, const std::vector<uint32_t>& types)
+    {
+    std::vector<JobDev> sph, mixed;
+    uint32_t mixed_types = 0;
+    for (size_t j = 0; j < jobs.size(); j++)
+        {
+        if ((types[j] & ~T_SPHERES) == 0)
+            sph.push_back(jobs[j]);
+        else
+            {
+            mixed.push_back(jobs[j]);
+            mixed_types |= types[j];
+            }
+        }
+    if (!sph.empty())
+        run_batch(D, sph, T_SPHERES);
+    if (!mixed.empty())
+        run_batch(D, mixed, mixed_types);
+    }
+
 // Render `samples` Path passes for each (scene, tracer) pair.
 static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, const std::vector<TracerObj*>& tracers,
                         uint32_t samples)
@@ -1050,9 +1072,9 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
     for (auto* tr : tracers)
         per_sample += (uint64_t)tr->W * tr->H * tr->L;
     uint32_t K = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(samples, D.batch_paths / std::max<uint64_t>(per_sample, 1)));
-    uint32_t types = 0;
-    for (auto* sc : uniq)
-        types |= scene_types(*sc);
+    std::vector<uint32_t> types;
+    for (auto* sc : scenes)
+        types.push_back(scene_types(*sc));
     uint32_t done = 0;
     while (done < samples)
         {
@@ -1064,7 +1086,7 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
             J.K = k;
             jobs.push_back(J);
             }
-        run_batch(D, jobs, types);
+        run_grouped(D, jobs, types);
         for (auto* tr : tracers)
             tr->n_samples += k;
         done += k;
@@ -1124,7 +1146,7 @@ struct RequestObj
 struct ServerObj
     {
     DeviceObj* dev = nullptr;
-    uint64_t budget = 0, in_use = 0, target_paths = 8ull << 20;
+    uint64_t budget = 0, in_use = 0, target_paths = 64ull << 20;
     std::deque<PolicyItem> queue;
     std::vector<uint64_t> active; // admission order
     std::map<uint64_t, RequestObj> reqs;
@@ -1181,10 +1203,10 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
             J.K = k[i];
             jobs.push_back(J);
             }
-        uint32_t types = 0;
+        std::vector<uint32_t> types;
         for (uint64_t id : S.active)
-            types |= scene_types(*S.reqs[id].scene);
-        run_batch(D, jobs, types);
+            types.push_back(scene_types(*S.reqs[id].scene));
+        run_grouped(D, jobs, types);
         D.end_stats();
         if (D.last_counters[CTR_OVERFLOW])
             throw Error(FH_ERR_STATE, "traversal stack overflow");

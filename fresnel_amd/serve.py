@@ -50,7 +50,7 @@ class Server:
         target_paths (int): light-sample paths traced per step.
     """
 
-    def __init__(self, device, hbm_budget=240 << 30, target_paths=32 << 20):
+    def __init__(self, device, hbm_budget=240 << 30, target_paths=64 << 20):
         if device.mode != 'gpu':
             raise RuntimeError("the render server needs a GPU device")
         self.device = device
