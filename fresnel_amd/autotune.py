@@ -10,10 +10,14 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
                   amortise host synchronisation and the per-batch tail.
 * ``tail_rays``   live-ray count below which a batch finishes its remaining
                   paths in one kernel.
-* ``trav_chunk``  rays a traversal wavefront claims per work-counter atomic.
+* ``trav_chunk``  rays a secondary-ray traversal wavefront claims per
+                  work-counter atomic.
+* ``primary_chunk`` the same for primary rays.
 * ``trav_refill`` idle lanes of a traversal wavefront that trigger taking new
                   rays (65: trace 64 rays at a time).
 * ``trav_packet`` primary rays traced as wavefront packets (2) or per ray (1).
+* ``primary_tiles`` primary rays tested against per-tile candidate lists (2)
+                  or searched in the BVH (1).
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -30,8 +34,10 @@ CANDIDATES = {
     'batch_paths': [8 << 20, 32 << 20, 64 << 20],
     'tail_rays': [8192, 32768, 131072],
     'trav_chunk': [64, 256, 1024],
+    'primary_chunk': [256, 1024, 4096],
     'trav_refill': [16, 32, 48, 65],
     'trav_packet': [1, 2],
+    'primary_tiles': [1, 2],
 }
 
 
@@ -91,7 +97,7 @@ def store(device, key, params):
         json.dump(data, f, indent=1)
 
 
-def tune(device, scene, w, h, samples=16, light_samples=1, repeats=2, use_cache=True, log=None):
+def tune(device, scene, w, h, samples=16, light_samples=1, repeats=3, use_cache=True, log=None):
     """Pick the fastest knob values for rendering `scene` at w x h; applies and returns them."""
     from . import tracer as _tracer
     key = scene_class(scene, w, h)

@@ -1,0 +1,206 @@
+// tiles.hip -- per-tile candidate lists for primary rays.
+//
+// For a pinhole perspective camera (aperture 0) or an orthographic camera,
+// every primary ray of an 8x8 pixel tile passes through the tile's pixel
+// footprint (widened by the anti-aliasing jitter of at most 0.7072 pixel).  A
+// primitive whose bounding box projects outside that footprint cannot pass the
+// slab test of any of those rays, so a list of the primitives whose projected
+// box overlaps the tile, sorted by a lower bound of their computed hit
+// distances, replaces the BVH search for these rays: each lane tests the list
+// in order (slab test on the primitive's bounds, then its intersector) and the
+// wavefront stops once every lane's closest hit lies below the next entry's
+// bound.  The closest hit is the same (t, geometry, primitive) minimum over
+// the primitives whose bounds pass the slab test as in the BVH search.
+#include "rt/isect.h"
+
+namespace fh
+{
+constexpr uint32_t TILE = 8;
+
+// Projection constants of a camera (host-derived from CamDev).
+struct TileCam
+    {
+    v3 p, u, v, w;     // camera position and basis (w points backwards)
+    float scale;       // perspective: focal_d / focal_h; orthographic: 1 / focal_h
+    float W, H;        // image size
+    int32_t model;     // 0 orthographic, 1 perspective
+    uint32_t tiles_x, tiles_y;
+    float margin;      // CULL_K * 3 * largest primitive half-extent
+    };
+
+// Tile rectangle [tx0, tx1] x [ty0, ty1] of a primitive box and the lower
+// bound q of its computed hit distance for rays of those tiles.  Returns false
+// when no ray of a full tile can reach the box; *straddle is set when a
+// perspective camera's plane cuts the box (such scenes use the BVH search).
+__device__ __forceinline__ bool tile_rect(const TileCam& c, float4 lo, float4 hi, int& tx0, int& tx1, int& ty0,
+                                          int& ty1, float& q, bool& straddle)
+    {
+    float imin = __builtin_huge_valf(), imax = -__builtin_huge_valf();
+    float jmin = __builtin_huge_valf(), jmax = -__builtin_huge_valf();
+    float zmin = __builtin_huge_valf(), zmax = -__builtin_huge_valf();
+    float dmax2 = 0.0f;
+    for (int k = 0; k < 8; k++)
+        {
+        const v3 X = mk3((k & 1) ? hi.x : lo.x, (k & 2) ? hi.y : lo.y, (k & 4) ? hi.z : lo.z);
+        const v3 d = X - c.p;
+        const float z = -dot(d, c.w);
+        zmin = fminf(zmin, z);
+        zmax = fmaxf(zmax, z);
+        dmax2 = fmaxf(dmax2, dot(d, d));
+        const float s = (c.model == 1) ? c.scale / z : c.scale;
+        const float xs = dot(d, c.u) * s;
+        const float ys = dot(d, c.v) * s;
+        const float i_f = xs * c.H + 0.5f * c.W;
+        const float j_f = (0.5f - ys) * c.H;
+        imin = fminf(imin, i_f);
+        imax = fmaxf(imax, i_f);
+        jmin = fminf(jmin, j_f);
+        jmax = fmaxf(jmax, j_f);
+        }
+    if (zmax <= 0.0f)
+        return false; // behind the camera (rays advance along -w)
+    if (c.model == 1 && zmin <= 0.0f)
+        {
+        straddle = true;
+        return false;
+        }
+    // pixel i sees jittered positions in (i + 0.5 - 0.7072, i + 0.5 + 0.7072];
+    // one more pixel covers rounding of the projection and of the camera rays
+    const float lim = 1.0e6f;
+    const int x0 = (int)floorf(fmaxf(imin - 2.25f, -lim)), x1 = (int)floorf(fminf(imax + 1.25f, lim));
+    const int y0 = (int)floorf(fmaxf(jmin - 2.25f, -lim)), y1 = (int)floorf(fminf(jmax + 1.25f, lim));
+    const int nx = (int)(c.tiles_x * TILE), ny = (int)(c.tiles_y * TILE);
+    if (x1 < 0 || y1 < 0 || x0 >= nx || y0 >= ny)
+        return false;
+    tx0 = max(x0, 0) / (int)TILE;
+    tx1 = min(x1, nx - 1) / (int)TILE;
+    ty0 = max(y0, 0) / (int)TILE;
+    ty1 = min(y1, ny - 1) / (int)TILE;
+    // hit distances: perspective rays start at p with unit directions, so a
+    // hit inside the box lies at least the box's distance from p; orthographic
+    // rays start on the image plane and t is the depth along -w.  The computed
+    // distance can be lower by CULL_K (|v| + 3R) with |v| at most the box's
+    // farthest corner distance (plus its size for orthographic rays).
+    float base, far;
+    if (c.model == 1)
+        {
+        const float dx = fmaxf(fmaxf(lo.x - c.p.x, c.p.x - hi.x), 0.0f);
+        const float dy = fmaxf(fmaxf(lo.y - c.p.y, c.p.y - hi.y), 0.0f);
+        const float dz = fmaxf(fmaxf(lo.z - c.p.z, c.p.z - hi.z), 0.0f);
+        base = sqrtf(dx * dx + dy * dy + dz * dz);
+        far = sqrtf(dmax2);
+        }
+    else
+        {
+        base = zmin;
+        far = zmax + ((hi.x - lo.x) + (hi.y - lo.y) + (hi.z - lo.z));
+        }
+    q = fmaxf(base * (1.0f - 0x1p-20f) - (CULL_K * far + c.margin), 0.0f);
+    return true;
+    }
+
+__global__ void k_tile_count(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t N, TileCam c,
+                             uint32_t* counts, uint32_t* flags)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > N)
+        return;
+    if (i == N)
+        {
+        counts[N] = 0; // the scan's last element gives the total
+        return;
+        }
+    int tx0, tx1, ty0, ty1;
+    float q;
+    bool straddle = false;
+    uint32_t n = 0;
+    if (tile_rect(c, lo[i], hi[i], tx0, tx1, ty0, ty1, q, straddle))
+        n = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+    counts[i] = n;
+    if (straddle)
+        atomicOr(flags, 1u);
+    }
+
+// Entries of primitive i at offsets[i]: tile index, bound bits, primitive.
+__global__ void k_tile_emit(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t N, TileCam c,
+                            const uint32_t* __restrict__ offsets, uint32_t* tile, uint32_t* qbits, uint32_t* ref,
+                            uint32_t* order)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    int tx0, tx1, ty0, ty1;
+    float q;
+    bool straddle = false;
+    if (!tile_rect(c, lo[i], hi[i], tx0, tx1, ty0, ty1, q, straddle))
+        return;
+    uint32_t e = offsets[i];
+    const uint32_t e_end = offsets[i + 1]; // the count pass saw the same rectangle
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1 && e < e_end; tx++, e++)
+            {
+            tile[e] = (uint32_t)ty * c.tiles_x + (uint32_t)tx;
+            qbits[e] = __float_as_uint(q); // q >= 0: the bits order like the values
+            ref[e] = i;
+            order[e] = e;
+            }
+    }
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t n,
+                             uint32_t* dst)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = src[idx[i]];
+    }
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+    }
+
+// Sorted entries packed as 8 words each, so a wavefront reads an entry with
+// one scalar load: [0] bound bits, then for spheres-only scenes the sphere
+// (x, y, z, r), geometry slot and primitive, otherwise the primitive's box
+// (lo.xyz, hi.xyz) and its index into the bounds arrays.
+template<bool SPHERES>
+__global__ void k_tile_pack(const uint32_t* __restrict__ order, const uint32_t* __restrict__ ref0,
+                            const uint32_t* __restrict__ q0, const float4* __restrict__ lo,
+                            const float4* __restrict__ hi, const GeomDev* __restrict__ geoms, uint32_t n,
+                            uint4* __restrict__ out)
+    {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n)
+        return;
+    const uint32_t e = order[j];
+    const uint32_t r = ref0[e];
+    const float4 a = lo[r], b = hi[r];
+    if (SPHERES)
+        {
+        const uint32_t slot = __float_as_uint(a.w), prim = __float_as_uint(b.w);
+        const float4 sp = geoms[slot].a[prim];
+        out[2 * j] = make_uint4(q0[e], __float_as_uint(sp.x), __float_as_uint(sp.y), __float_as_uint(sp.z));
+        out[2 * j + 1] = make_uint4(__float_as_uint(sp.w), slot, prim, 0u);
+        }
+    else
+        {
+        out[2 * j] = make_uint4(q0[e], __float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(a.z));
+        out[2 * j + 1] = make_uint4(__float_as_uint(b.x), __float_as_uint(b.y), __float_as_uint(b.z), r);
+        }
+    }
+
+// [start, end) of each tile's run in the tile-sorted entries.
+__global__ void k_tile_bounds(const uint32_t* __restrict__ tile, uint32_t n, uint32_t* start, uint32_t* end)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint32_t t = tile[i];
+    if (i == 0 || tile[i - 1] != t)
+        start[t] = i;
+    if (i == n - 1 || tile[i + 1] != t)
+        end[t] = i + 1;
+    }
+    } // namespace fh

@@ -12,6 +12,7 @@
 // not depend on how paths were scheduled.
 #include "rt/isect.h"
 #include "rt/shade.h"
+#include "kernels/tiles.hip"
 
 namespace fh
 {
@@ -392,14 +393,20 @@ __device__ __forceinline__ void primary_order(const JobDev& J, uint32_t local, u
     }
 
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
-                                            uint32_t& k, uint32_t& pixel, v3& o, v3& d)
+                                            uint32_t& k, uint32_t& px, uint32_t& py, v3& o, v3& d)
     {
     job = find_job(jobs, n_jobs, i);
     const JobDev& J = jobs[job];
-    uint32_t px, py;
     primary_order(J, i - J.ray_base, k, px, py);
-    pixel = py * J.W + px;
     camera_ray(J.cam, J.W, J.H, J.seed, px, py, J.n_base + k + 1, true, o, d);
+    }
+
+__device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
+                                            uint32_t& k, uint32_t& pixel, v3& o, v3& d)
+    {
+    uint32_t px, py;
+    primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
+    pixel = py * jobs[job].W + px;
     }
 
 // Persistent traversal: each wavefront claims `chunk` rays (a multiple of 64)
@@ -691,19 +698,122 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
             const uint32_t i = sub + lane;
             const bool valid = i < end;
             v3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
-            uint32_t job = 0;
+            uint32_t job = 0, px = 0, py = 0;
             if (valid)
                 {
-                uint32_t k, pixel;
-                primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+                uint32_t k;
+                primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
                 }
             const uint32_t job0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);
             HitT best;
-            if (__ballot(valid && job != job0) != 0)
+            const bool one_job = __ballot(valid && job != job0) == 0;
+            // the tile of this group, if all 64 rays belong to one full 8x8 tile
+            uint32_t tile = EMPTY;
+            if (one_job && jobs[job0].use_tiles)
+                {
+                const JobDev& J = jobs[job0];
+                if (valid && px < J.tiles_x * TILE && py < J.tiles_y * TILE)
+                    tile = (py / TILE) * J.tiles_x + px / TILE;
+                const uint32_t t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile);
+                if (__ballot(tile != t0) != 0)
+                    tile = EMPTY;
+                tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile);
+                }
+            if (!one_job)
                 {
                 if (valid)
                     trace_closest<LdsStack<SS>, COUNT, TYPES>(jobs[job].scene, o, d, 1e-3f, best, st, n_nodes,
                                                               n_prims, err);
+                }
+            else if (tile != EMPTY)
+                {
+                // candidate list of the tile (kernels/tiles.hip), nearest bound
+                // first; the next entry is loaded while the current one is tested
+                const JobDev& J = jobs[job0];
+                const GeomDev* __restrict__ geoms = J.scene.geoms;
+                const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                const float tnear = 1e-3f;
+                best.t = __builtin_huge_valf();
+                best.geom = EMPTY;
+                best.prim = EMPTY;
+                const uint32_t e0 = ((const ConstU*)J.tile_start)[tile];
+                const uint32_t e1 = ((const ConstU*)J.tile_end)[tile];
+                const ConstU* ent = (const ConstU*)J.tile_entries;
+                uint32_t cur[8], nxt[8];
+                if (e0 < e1)
+                    for (int k = 0; k < 8; k++)
+                        cur[k] = ent[8 * (size_t)e0 + k];
+                for (uint32_t e = e0; e < e1; e++)
+                    {
+                    const size_t en = (e + 1 < e1) ? (size_t)e + 1 : (size_t)e;
+                    for (int k = 0; k < 8; k++)
+                        nxt[k] = ent[8 * en + k];
+                    const float q = __uint_as_float(cur[0]);
+                    const bool want = !(q > best.t);
+                    if (__ballot(want) == 0)
+                        break;
+                    if (want)
+                        {
+                        float lx, ly, lz, hx, hy, hz;
+                        float4 sph;
+                        uint32_t slot, prim;
+                        if constexpr (TYPES == T_SPHERES)
+                            {
+                            sph = make_float4(__uint_as_float(cur[1]), __uint_as_float(cur[2]),
+                                              __uint_as_float(cur[3]), __uint_as_float(cur[4]));
+                            // the sphere's bounds, as the builder computes them (GeometrySphere.cc:60-73)
+                            lx = sph.x - sph.w;
+                            ly = sph.y - sph.w;
+                            lz = sph.z - sph.w;
+                            hx = sph.x + sph.w;
+                            hy = sph.y + sph.w;
+                            hz = sph.z + sph.w;
+                            slot = cur[5];
+                            prim = cur[6];
+                            }
+                        else
+                            {
+                            lx = __uint_as_float(cur[1]);
+                            ly = __uint_as_float(cur[2]);
+                            lz = __uint_as_float(cur[3]);
+                            hx = __uint_as_float(cur[4]);
+                            hy = __uint_as_float(cur[5]);
+                            hz = __uint_as_float(cur[6]);
+                            slot = 0;
+                            prim = 0;
+                            }
+                        float tmin, tmax;
+                        slab(o, inv, lx, ly, lz, hx, hy, hz, tmin, tmax);
+                        if ((tmin <= tmax) && (tmax >= tnear))
+                            {
+                            float t;
+                            bool hit;
+                            if (COUNT)
+                                n_prims++;
+                            if constexpr (TYPES == T_SPHERES)
+                                {
+                                float de;
+                                bool first;
+                                hit = sphere_t(o, d, mk3(sph.x, sph.y, sph.z), sph.w, t, de, first) && (tnear < t);
+                                }
+                            else
+                                {
+                                const uint32_t r = cur[7];
+                                slot = __float_as_uint(J.prim_lo[r].w);
+                                prim = __float_as_uint(J.prim_hi[r].w);
+                                hit = leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
+                                }
+                            if (hit && better(t, slot, prim, best))
+                                {
+                                best.t = t;
+                                best.geom = slot;
+                                best.prim = prim;
+                                }
+                            }
+                        }
+                    for (int k = 0; k < 8; k++)
+                        cur[k] = nxt[k];
+                    }
                 }
             else
                 {

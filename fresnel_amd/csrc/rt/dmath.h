@@ -143,24 +143,18 @@ struct u4
     uint32_t v[4];
     };
 
-FH_HD uint32_t mulhi32(uint32_t a, uint32_t b)
-    {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return __umulhi(a, b);
-#else
-    return (uint32_t)(((uint64_t)a * b) >> 32);
-#endif
-    }
-
 FH_HD u4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
     {
 #pragma unroll
     for (int r = 0; r < 10; r++)
         {
-        const uint32_t hi0 = mulhi32(0xD2511F53u, c0);
-        const uint32_t lo0 = 0xD2511F53u * c0;
-        const uint32_t hi1 = mulhi32(0xCD9E8D57u, c2);
-        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        // one 32x32->64 multiply gives both halves (v_mad_u64_u32 on the device)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32);
+        const uint32_t lo1 = (uint32_t)p1;
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0;

@@ -127,6 +127,19 @@ struct JobDev
     uint64_t rec_base;    // first termination record of this job
     float4* linear;       // running mean (h, w) RGBA
     uchar4* srgb;         // sRGB output (h, w) RGBA
+    // per-tile candidate lists of the primary rays (kernels/tiles.hip), used
+    // when use_tiles is set: entries [tile_start[t], tile_end[t]) of tile
+    // t = (y / 8) * tiles_x + x / 8, 8 words each (k_tile_pack), ascending in
+    // their hit-distance lower bound; prim_lo/hi .w carry the geometry slot
+    // and primitive of a box entry's index
+    const uint32_t* tile_start;
+    const uint32_t* tile_end;
+    const uint4* tile_entries;
+    const float4* prim_lo;
+    const float4* prim_hi;
+    uint32_t tiles_x, tiles_y;
+    uint32_t use_tiles;
+    uint32_t pad_tiles;
     };
 
 // Termination record of one light-sample path.

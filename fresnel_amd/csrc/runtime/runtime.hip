@@ -160,8 +160,10 @@ struct DeviceObj
     bool count_on = false;
     uint64_t batch_paths = 64ull << 20; // light-sample paths per batch (sizes the queues)
     uint32_t tail_rays = 131072;        // finish paths in one kernel below this many rays
-    uint32_t trav_chunk = 256;          // rays a traversal wavefront claims per work-counter atomic
+    uint32_t trav_chunk = 256;          // rays a secondary-ray traversal wavefront claims per atomic
+    uint32_t primary_chunk = 1024;      // the same for primary rays
     uint32_t trav_packet = 2;           // primary rays: 2 wavefront packets, 1 per ray
+    uint32_t primary_tiles = 2;         // primary rays of pinhole/orthographic cameras: 2 tile lists, 1 BVH
     uint32_t trav_refill = 65;          // lanes take new rays once this many of a wavefront are idle
                                         // (65: batch-synchronous kernel, 64 rays at a time)
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
@@ -277,8 +279,8 @@ static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const T
         D.launch(nm,
                  [&]
                  {
-                     kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.n, a.hits, a.work, a.chunk, a.gstack,
-                                                     a.gstride, a.counters);
+                     kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.n, a.hits, a.work, D.primary_chunk,
+                                                     a.gstack, a.gstride, a.counters);
                  });
         return;
         }
@@ -432,7 +434,8 @@ struct SceneObj
     DBuf<GeomDev> dgeoms;
     uint32_t n_slots = 0;
     DBuf<float4> blo, bhi;
-    DBuf<uint32_t> keys, vals, keys2, vals2, hist, sums, parent, flags, prefix, slots;
+    DBuf<uint32_t> keys, vals, keys2, vals2, hist, parent, flags, prefix, slots;
+    std::deque<DBuf<uint32_t>> scan_tmp;
     DBuf<float> partial, cb, root_box, extent;
     float cull_margin = 0.0f; // CULL_K * 3 * largest primitive half-extent
     DBuf<Node> nodes;
@@ -555,6 +558,51 @@ static uint32_t prim_count(const GeometryObj& g)
     }
 
 // Build or refit the LBVH over the enabled geometries.
+// Exclusive prefix sum of n uint32 in place (any n; tmp holds one buffer per level).
+static void scan_exclusive(DeviceObj& D, hipStream_t s, uint32_t* data, uint32_t n,
+                           std::deque<DBuf<uint32_t>>& tmp, size_t level = 0)
+    {
+    const uint32_t n_sums = blocks(n, 2048);
+    while (tmp.size() <= level)
+        tmp.emplace_back();
+    tmp[level].ensure(std::max<uint32_t>(n_sums, 1));
+    uint32_t* sums = tmp[level].p;
+    k_scan_blocks<<<n_sums, 256, 0, s>>>(data, n, sums);
+    if (n_sums <= 2048)
+        k_scan_top<<<1, 256, 0, s>>>(sums, n_sums);
+    else
+        scan_exclusive(D, s, sums, n_sums, tmp, level + 1);
+    k_scan_add<<<blocks(n, 256), 256, 0, s>>>(data, n, sums);
+    }
+
+// Stable LSD radix sort of (key, value) pairs on key bits [bit0, bit1) in
+// 8-bit digits, ping-ponging between (k0, v0) and (k1, v1); returns whether
+// the result is in (k1, v1).
+static bool radix_sort_pairs(DeviceObj& D, hipStream_t s, const char* name, uint32_t* k0, uint32_t* v0, uint32_t* k1,
+                             uint32_t* v1, uint32_t N, uint32_t bit0, uint32_t bit1, DBuf<uint32_t>& hist,
+                             std::deque<DBuf<uint32_t>>& tmp)
+    {
+    const uint32_t nb = blocks(N, SORT_TILE);
+    const uint32_t nh = 256 * nb;
+    hist.ensure(nh);
+    uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+    bool flipped = false;
+    for (uint32_t shift = bit0; shift < bit1; shift += 8)
+        {
+        D.launch(name,
+                 [&]
+                 {
+                     k_radix_hist<<<nb, SORT_BLOCK, 0, s>>>(kin, N, shift, nb, hist.p);
+                     scan_exclusive(D, s, hist.p, nh, tmp);
+                     k_radix_scatter<<<nb, SORT_BLOCK, 0, s>>>(kin, vin, kout, vout, N, shift, nb, hist.p);
+                 });
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        flipped = !flipped;
+        }
+    return flipped;
+    }
+
 static void build_bvh(SceneObj& sc, bool refit_only)
     {
     DeviceObj& D = *sc.dev;
@@ -635,28 +683,9 @@ static void build_bvh(SceneObj& sc, bool refit_only)
         sc.vals2.ensure(N);
         D.launch("lbvh_morton",
                  [&] { k_morton<<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, sc.cb.p, sc.keys.p, sc.vals.p); });
-        const uint32_t nb = blocks(N, SORT_TILE);
-        const uint32_t nh = 256 * nb;
-        const uint32_t n_sums = blocks(nh, 2048);
-        REQUIRE(n_sums <= 2048, "scene too large for the radix sort scan");
-        sc.hist.ensure(nh);
-        sc.sums.ensure(n_sums);
-        uint32_t *kin = sc.keys.p, *vin = sc.vals.p, *kout = sc.keys2.p, *vout = sc.vals2.p;
-        for (uint32_t shift = 0; shift < 32; shift += 8)
-            {
-            D.launch("lbvh_sort",
-                     [&]
-                     {
-                         k_radix_hist<<<nb, SORT_BLOCK, 0, s>>>(kin, N, shift, nb, sc.hist.p);
-                         k_scan_blocks<<<n_sums, 256, 0, s>>>(sc.hist.p, nh, sc.sums.p);
-                         k_scan_top<<<1, 256, 0, s>>>(sc.sums.p, n_sums);
-                         k_scan_add<<<blocks(nh, 256), 256, 0, s>>>(sc.hist.p, nh, sc.sums.p);
-                         k_radix_scatter<<<nb, SORT_BLOCK, 0, s>>>(kin, vin, kout, vout, N, shift, nb, sc.hist.p);
-                     });
-            std::swap(kin, kout);
-            std::swap(vin, vout);
-            }
-        // after 4 passes the sorted data is back in keys/vals
+        // four 8-bit passes: the sorted data is back in keys/vals
+        radix_sort_pairs(D, s, "lbvh_sort", sc.keys.p, sc.vals.p, sc.keys2.p, sc.vals2.p, N, 0, 32, sc.hist,
+                         sc.scan_tmp);
         sc.parent.ensure(2 * N - 1);
         D.launch("lbvh_karras",
                  [&]
@@ -849,9 +878,27 @@ enum TracerType
     T_DIRECT
     };
 
+// Per-tile candidate lists of a tracer's primary rays (kernels/tiles.hip),
+// cached for the (scene, hierarchy version, camera, image size) they were
+// built for.
+struct TileLists
+    {
+    bool valid = false;        // lists usable for the key below
+    const void* scene = nullptr;
+    uint64_t builds = ~0ull, refits = ~0ull;
+    CamDev cam {};
+    uint32_t W = 0, H = 0;
+    uint32_t tiles_x = 0, tiles_y = 0;
+    uint32_t n_entries = 0;
+    DBuf<uint32_t> counts, flags, tile0, q0, ref0, keysA, valsA, keysB, valsB, start, end, hist;
+    DBuf<uint4> entries; // 2 per entry
+    std::deque<DBuf<uint32_t>> scan_tmp;
+    };
+
 struct TracerObj
     {
     DeviceObj* dev = nullptr;
+    TileLists tiles;
     TracerType type = T_PATH;
     uint32_t W = 0, H = 0;
     uint32_t seed = 0;
@@ -887,6 +934,114 @@ struct TracerObj
         }
     };
 
+static uint32_t scene_types(const SceneObj& sc);
+
+// Bring a tracer's per-tile candidate lists up to date for scene `sc` seen
+// through `cam` (pinhole perspective or orthographic cameras only; a scene a
+// perspective camera's plane cuts keeps the BVH search).
+static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev& cam)
+    {
+    TileLists& T = tr.tiles;
+    const bool pinhole = (cam.model == 1 && cam.a == 0.0f) || cam.model == 0;
+    if (D.primary_tiles != 2 || !pinhole || sc.n_prims == 0 || tr.W < TILE || tr.H < TILE)
+        {
+        T.valid = false;
+        T.scene = nullptr;
+        return;
+        }
+    if (T.scene == &sc && T.builds == sc.builds && T.refits == sc.refits && T.W == tr.W && T.H == tr.H
+        && std::memcmp(&T.cam, &cam, sizeof(CamDev)) == 0)
+        return; // cached, usable or known unusable
+    T.scene = &sc;
+    T.builds = sc.builds;
+    T.refits = sc.refits;
+    T.cam = cam;
+    T.W = tr.W;
+    T.H = tr.H;
+    T.valid = false;
+    hipStream_t s = D.stream;
+    const uint32_t N = sc.n_prims;
+    TileCam c;
+    c.p = cam.p;
+    c.u = cam.u;
+    c.v = cam.v;
+    c.w = cam.w;
+    c.scale = (cam.model == 1) ? cam.focal_d / cam.focal_h : 1.0f / cam.focal_h;
+    c.W = (float)tr.W;
+    c.H = (float)tr.H;
+    c.model = cam.model;
+    c.tiles_x = tr.W / TILE;
+    c.tiles_y = tr.H / TILE;
+    c.margin = sc.cull_margin;
+    const uint32_t n_tiles = c.tiles_x * c.tiles_y;
+    T.counts.ensure((size_t)N + 1);
+    T.flags.ensure(1);
+    uint32_t h[2] = {0, 0};
+    D.launch("tiles_count",
+             [&]
+             {
+                 k_fill_u32<<<1, 64, 0, s>>>(T.flags.p, 1, 0u);
+                 k_tile_count<<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, c, T.counts.p, T.flags.p);
+                 scan_exclusive(D, s, T.counts.p, N + 1, T.scan_tmp);
+             });
+    HIPCHECK(hipMemcpyAsync(&h[0], T.counts.p + N, 4, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(&h[1], T.flags.p, 4, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    if (h[1])
+        return; // the camera plane cuts a primitive's box
+    const uint32_t M = h[0];
+    T.start.ensure(n_tiles);
+    T.end.ensure(n_tiles);
+    if (M > 0)
+        {
+        for (auto* b : {&T.tile0, &T.q0, &T.ref0, &T.keysA, &T.valsA, &T.keysB, &T.valsB})
+            b->ensure(M);
+        T.entries.ensure(2 * (size_t)M);
+        D.launch("tiles_emit",
+                 [&]
+                 {
+                     k_tile_emit<<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, c, T.counts.p, T.tile0.p,
+                                                                T.q0.p, T.ref0.p, T.valsA.p);
+                     HIPCHECK(hipMemcpyAsync(T.keysA.p, T.q0.p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+                 });
+        // by bound (4 passes: the result is back in keysA/valsA), then stably by tile
+        radix_sort_pairs(D, s, "tiles_sort", T.keysA.p, T.valsA.p, T.keysB.p, T.valsB.p, M, 0, 32, T.hist,
+                         T.scan_tmp);
+        uint32_t bits = 8;
+        while (bits < 32 && (n_tiles - 1) >> bits)
+            bits += 8;
+        D.launch("tiles_sort", [&] { k_gather_u32<<<blocks(M, 256), 256, 0, s>>>(T.tile0.p, T.valsA.p, M, T.keysB.p); });
+        const bool flipped = radix_sort_pairs(D, s, "tiles_sort", T.keysB.p, T.valsA.p, T.keysA.p, T.valsB.p, M, 0,
+                                              bits, T.hist, T.scan_tmp);
+        const uint32_t* kt = flipped ? T.keysA.p : T.keysB.p;
+        const uint32_t* vt = flipped ? T.valsB.p : T.valsA.p;
+        D.launch("tiles_bounds",
+                 [&]
+                 {
+                     if (scene_types(sc) == T_SPHERES)
+                         k_tile_pack<true><<<blocks(M, 256), 256, 0, s>>>(vt, T.ref0.p, T.q0.p, sc.blo.p, sc.bhi.p,
+                                                                          sc.dgeoms.p, M, T.entries.p);
+                     else
+                         k_tile_pack<false><<<blocks(M, 256), 256, 0, s>>>(vt, T.ref0.p, T.q0.p, sc.blo.p, sc.bhi.p,
+                                                                           sc.dgeoms.p, M, T.entries.p);
+                     k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.start.p, n_tiles, 0u);
+                     k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.end.p, n_tiles, 0u);
+                     k_tile_bounds<<<blocks(M, 256), 256, 0, s>>>(kt, M, T.start.p, T.end.p);
+                 });
+        }
+    else
+        D.launch("tiles_bounds",
+                 [&]
+                 {
+                     k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.start.p, n_tiles, 0u);
+                     k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.end.p, n_tiles, 0u);
+                 });
+    T.tiles_x = c.tiles_x;
+    T.tiles_y = c.tiles_y;
+    T.n_entries = M;
+    T.valid = true;
+    }
+
 static JobDev make_job(SceneObj& sc, TracerObj& tr, const fh_user_camera& cam_user)
     {
     JobDev J;
@@ -906,6 +1061,18 @@ static JobDev make_job(SceneObj& sc, TracerObj& tr, const fh_user_camera& cam_us
     J.hw_color = mk3(tr.hw_color.r, tr.hw_color.g, tr.hw_color.b);
     J.linear = tr.linear.p;
     J.srgb = tr.srgb.p;
+    const TileLists& T = tr.tiles;
+    if (T.valid && T.scene == &sc && T.W == tr.W && T.H == tr.H && std::memcmp(&T.cam, &J.cam, sizeof(CamDev)) == 0)
+        {
+        J.use_tiles = 1;
+        J.tiles_x = T.tiles_x;
+        J.tiles_y = T.tiles_y;
+        J.tile_start = T.start.p;
+        J.tile_end = T.end.p;
+        J.tile_entries = T.entries.p;
+        J.prim_lo = sc.blo.p;
+        J.prim_hi = sc.bhi.p;
+        }
     return J;
     }
 
@@ -1068,6 +1235,8 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
             uniq.push_back(sc);
     for (auto* sc : uniq)
         sync_scene(*sc);
+    for (size_t k = 0; k < scenes.size(); k++)
+        update_tiles(D, *scenes[k], *tracers[k], make_cam(scenes[k]->camera));
     uint64_t per_sample = 0;
     for (auto* tr : tracers)
         per_sample += (uint64_t)tr->W * tr->H * tr->L;
@@ -1195,6 +1364,11 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         std::vector<uint32_t> k = policy_allocate(items, S.target_paths);
         std::lock_guard<std::mutex> lock(D.mu);
         D.begin_stats();
+        for (uint64_t id : S.active)
+            {
+            RequestObj& r = S.reqs[id];
+            update_tiles(D, *r.scene, *r.tracer, make_cam(r.scene->camera));
+            }
         std::vector<JobDev> jobs;
         for (size_t i = 0; i < S.active.size(); i++)
             {
@@ -2356,6 +2530,15 @@ extern "C"
                         D.trav_chunk = (uint32_t)value;
                         }
                     }
+                else if (k == "primary_tiles")
+                    {
+                    prev = D.primary_tiles;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 2, "primary_tiles must be 1 (BVH search) or 2 (tile lists)");
+                        D.primary_tiles = (uint32_t)value;
+                        }
+                    }
                 else if (k == "trav_packet")
                     {
                     prev = D.trav_packet;
@@ -2363,6 +2546,16 @@ extern "C"
                         {
                         REQUIRE(value <= 2, "trav_packet must be 1 (per ray) or 2 (wavefront packets)");
                         D.trav_packet = (uint32_t)value;
+                        }
+                    }
+                else if (k == "primary_chunk")
+                    {
+                    prev = D.primary_chunk;
+                    if (value > 0)
+                        {
+                        REQUIRE(value % 64 == 0 && value <= 65536,
+                                "primary_chunk must be a multiple of 64, at most 65536");
+                        D.primary_chunk = (uint32_t)value;
                         }
                     }
                 else if (k == "trav_refill")

@@ -253,13 +253,16 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * (default 64 Mi; sizes the ray queues); "tail_rays": below this many live
  * rays a batch finishes its paths in one kernel (default 131072); "stack_lds":
  * traversal stack entries kept in LDS per thread, 8 or 16 (default 8);
- * "trav_chunk": rays a traversal wavefront claims per work-counter atomic, a
- * multiple of 64 up to 65536 (default 256); "trav_refill": a traversal lane
+ * "trav_chunk": rays a secondary-ray traversal wavefront claims per
+ * work-counter atomic, a multiple of 64 up to 65536 (default 256);
+ * "primary_chunk": the same for primary rays (default 1024); "trav_refill": a traversal lane
  * whose ray is done takes a new one once this many lanes of its wavefront are
  * idle, 1..64, or 65 for the batch-synchronous kernel that traces 64 rays at
  * a time (default 65); "trav_packet": primary rays traced as wavefront
- * packets (2, default) or one by one (1).  value <= 0 only reads the current
- * value into *previous. */
+ * packets (2, default) or one by one (1); "primary_tiles": primary rays of
+ * pinhole-perspective and orthographic cameras tested against per-8x8-tile
+ * candidate lists (2, default) or searched in the BVH (1).  value <= 0 only
+ * reads the current value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
 /* ------------------------------------------------------ diagnostics (no
