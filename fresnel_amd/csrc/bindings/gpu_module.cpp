@@ -619,6 +619,19 @@ PYBIND11_MODULE(_gpu, m)
         .def("reset", &TracerPath::reset)
         .def("setLightSamples", &TracerPath::setLightSamples)
         .def("renderSamples", &TracerPath::renderSamples)
+        .def("debugPrimaryRender",
+             [](TracerPath& t, std::shared_ptr<Scene> s, unsigned sample)
+             {
+                 uint32_t w, hh;
+                 check(fh_tracer_get_size(t.h, &w, &hh));
+                 const py::ssize_t n = (py::ssize_t)w * hh;
+                 py::array_t<uint32_t> gid(n), prim(n);
+                 py::array_t<float> tt(n);
+                 s->push();
+                 check(fh_debug_primary_render(t.h, s->h, sample, (uint32_t*)gid.request().ptr,
+                                               (uint32_t*)prim.request().ptr, (float*)tt.request().ptr));
+                 return py::make_tuple(gid, prim, tt);
+             })
         .def("debugPrimary",
              [](TracerPath& t, std::shared_ptr<Scene> s, unsigned sample)
              {

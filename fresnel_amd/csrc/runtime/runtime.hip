@@ -2671,6 +2671,81 @@ extern "C"
                     }
             });
         }
+    int fh_debug_primary_render(fh_tracer tr, fh_scene scene, uint32_t sample, uint32_t* geom_id, uint32_t* prim,
+                                float* t)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(scene);
+                REQUIRE(sample >= 1, "sample numbers start at 1");
+                TracerObj& T = tr->obj;
+                SceneObj& sc = scene->obj;
+                DeviceObj& D = *T.dev;
+                REQUIRE(sc.dev == T.dev, "Scene and Tracer devices do not match");
+                std::lock_guard<std::mutex> lock(D.mu);
+                sync_scene(sc);
+                update_tiles(D, sc, T, make_cam(sc.camera));
+                JobDev J = make_job(sc, T, sc.camera);
+                J.K = 1;
+                J.n_base = sample - 1;
+                const uint32_t npix = J.npix;
+                RenderScratch& R = D.rs;
+                R.jobs.ensure(1);
+                R.hits.ensure(npix);
+                R.counters32.ensure(4);
+                const uint32_t gstride = D.trav_grid * TRAV_BLOCK;
+                R.gstack.ensure((size_t)(STACK_MAX - 8) * gstride);
+                D.begin_stats();
+                HIPCHECK(hipMemcpyAsync(R.jobs.p, &J, sizeof(J), hipMemcpyHostToDevice, D.stream));
+                HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), D.stream));
+                TravArgs a {R.jobs.p, 1, Queue {}, npix, R.hits.p, R.counters32.p, D.trav_chunk, R.gstack.p, gstride,
+                            D.counters.p};
+                dispatch_traverse(D, "traverse_primary", D.stream, a, true,
+                                  (scene_types(sc) & ~T_SPHERES) == 0);
+                D.end_stats();
+                if (D.last_counters[CTR_OVERFLOW])
+                    throw Error(FH_ERR_STATE, "traversal stack overflow");
+                std::vector<uint4> h(npix);
+                HIPCHECK(hipMemcpy(h.data(), R.hits.p, npix * sizeof(uint4), hipMemcpyDeviceToHost));
+                // ray j of the batch belongs to the pixel primary_order gives (one sample)
+                const uint32_t W8 = T.W & ~7u, H8 = T.H & ~7u;
+                for (uint32_t j = 0; j < npix; j++)
+                    {
+                    uint32_t px, py;
+                    if (j < W8 * H8)
+                        {
+                        const uint32_t tile = j >> 6, within = j & 63u, ty = tile / (W8 >> 3);
+                        px = (tile - ty * (W8 >> 3)) * 8u + (within & 7u);
+                        py = ty * 8u + (within >> 3);
+                        }
+                    else
+                        {
+                        uint32_t r = j - W8 * H8;
+                        const uint32_t rw = T.W - W8;
+                        if (r < rw * H8)
+                            {
+                            py = r / rw;
+                            px = W8 + (r - py * rw);
+                            }
+                        else
+                            {
+                            r -= rw * H8;
+                            py = H8 + r / T.W;
+                            px = r % T.W;
+                            }
+                        }
+                    const uint32_t p = py * T.W + px;
+                    if (geom_id)
+                        geom_id[p] = h[j].y;
+                    if (prim)
+                        prim[p] = h[j].z;
+                    if (t)
+                        std::memcpy(&t[p], &h[j].x, 4);
+                    }
+            });
+        }
     int fh_debug_bvh(fh_scene scene, void* nodes, uint64_t cap, uint64_t* n_nodes)
         {
         return guard(

@@ -279,6 +279,10 @@ int fh_debug_philox(fh_device dev, const uint32_t* in, uint32_t n, uint32_t* out
  * miss; t = +inf on a miss.  Any pointer may be NULL. */
 int fh_debug_primary(fh_tracer tr, fh_scene scene, uint32_t sample, float* rays,
                      uint32_t* geom_id, uint32_t* prim, float* t);
+/* The same hits through the renderer's own primary-ray kernel (tile lists or
+ * BVH packets, per the tuning knobs), for camera sample number `sample` >= 1. */
+int fh_debug_primary_render(fh_tracer tr, fh_scene scene, uint32_t sample, uint32_t* geom_id,
+                            uint32_t* prim, float* t);
 /* Closest hits for arbitrary rays (6 floats each) with tnear. */
 int fh_debug_trace(fh_scene scene, const float* rays, uint32_t n, float tnear,
                    uint32_t* geom_id, uint32_t* prim, float* t);

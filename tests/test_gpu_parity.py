@@ -101,6 +101,45 @@ def test_primary_rays_and_hits(name, gpu_device, cpu_device):
         assert (gid != MISS).any(), "scene not visible"
 
 
+@pytest.mark.parametrize('tiles', [2, 1], ids=['tile_lists', 'bvh_packets'])
+@pytest.mark.parametrize('name', sorted(PRIMARY_SCENES))
+def test_render_primary_hits_vs_checker(name, tiles, gpu_device, cpu_device):
+    """The renderer's own primary-ray kernel (tile lists, or BVH wavefront packets) returns the
+    checker's primary hits, bit for bit, except counted exact ties."""
+    W, H = 100, 60  # not multiples of 8: the edge strips take the per-ray search
+    d = gpu_device._device
+    saved = d.tune('primary_tiles', 0)
+    try:
+        d.tune('primary_tiles', tiles)
+        gs = PRIMARY_SCENES[name](gpu_device)
+        cs = PRIMARY_SCENES[name](cpu_device)
+        tr = fresnel.tracer.Path(gpu_device, w=W, h=H)
+        tr.seed = 321
+        for sample in (1, 5):
+            gid, prim, t = tr._tracer.debugPrimaryRender(gs._scene, sample)
+            h = cs._scene.handle()
+            L = oracle.lib()
+            ref_rays = numpy.zeros((W * H, 6), dtype=numpy.float32)
+            L.oracle_camera_rays(h, W, H, tr.seed, sample, 1, ptr(ref_rays))
+            n = W * H
+            rg = numpy.zeros(n, numpy.uint32)
+            rp = numpy.zeros(n, numpy.uint32)
+            rt = numpy.zeros(n, numpy.float32)
+            L.oracle_trace(h, ptr(ref_rays), n, 1e-3, ptr(rg, ctypes.c_uint32), ptr(rp, ctypes.c_uint32), ptr(rt),
+                           None, 0)
+            diff = (gid != rg) | (prim != rp)
+            if diff.any():
+                sel = numpy.ascontiguousarray(ref_rays[diff])
+                tsel = numpy.zeros(len(sel), numpy.uint32)
+                L.oracle_count_ties(h, ptr(sel), len(sel), 1e-3, ptr(tsel, ctypes.c_uint32))
+                assert numpy.all(tsel >= 2), f"{int(numpy.sum(tsel < 2))} primary hits differ without an exact tie"
+            same = ~diff
+            assert numpy.array_equal(t[same].view(numpy.uint32), rt[same].view(numpy.uint32)), "hit distances differ"
+            assert (gid != MISS).any(), "scene not visible"
+    finally:
+        d.tune('primary_tiles', saved)
+
+
 def _linear(device, builder, W, H, samples, L, seed=None):
     s = builder(device)
     t = fresnel.tracer.Path(device, w=W, h=H)
