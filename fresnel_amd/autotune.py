@@ -18,6 +18,8 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
 * ``trav_packet`` primary rays traced as wavefront packets (2) or per ray (1).
 * ``primary_tiles`` primary rays tested against per-tile candidate lists (2)
                   or searched in the BVH (1).
+* ``fuse_primary`` primary traversal and depth-0 shading in one kernel (2) or
+                  two (1).
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -38,6 +40,7 @@ CANDIDATES = {
     'trav_refill': [16, 32, 48, 65],
     'trav_packet': [1, 2],
     'primary_tiles': [1, 2],
+    'fuse_primary': [1, 2],
 }
 
 

@@ -494,29 +494,18 @@ __device__ __forceinline__ void hit_attributes(const SceneDev& sc, const HitT& h
 
 // Depth-0 shading: one primary hit spawns the L light-sample paths
 // (fresnel/cpu/TracerPath.cc:141-155 reuse the cached primary hit).
+// Depth-0 shading of a workgroup's primary hits: each hit spawns the job's L
+// light-sample paths (fresnel/cpu/TracerPath.cc:141-155 reuse the cached
+// primary hit); continuing paths are appended to `out` with one atomic per
+// workgroup and light sample, terminated ones write their record.  Every
+// thread of the workgroup must call it.
 template<int BS, uint32_t TYPES>
-__global__ __launch_bounds__(BS) void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs,
-                                                      const uint4* __restrict__ hits, uint32_t n, Queue out,
-                                                      uint32_t* out_count, uint4* records)
+__device__ __forceinline__ void shade_primary_block(const JobDev* __restrict__ jobs, bool valid, uint32_t job,
+                                                    uint32_t k, uint32_t pixel, v3 o, v3 d, const HitT& h, Queue out,
+                                                    uint32_t* out_count, uint4* records, uint32_t* sh)
     {
-    __shared__ uint32_t sh[BS / 64 + 1];
-    const uint32_t i = blockIdx.x * BS + threadIdx.x;
-    const bool valid = i < n;
-    uint32_t L = 0;
-    uint32_t job = 0, pixel = 0, k = 0;
-    HitT h;
-    h.geom = EMPTY;
-    v3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
     const float tnear = 1e-3f;
-    if (valid)
-        {
-        // the primary ray is regenerated rather than stored (bit-identical)
-        primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
-        const uint4 hh = hits[i];
-        h.t = __uint_as_float(hh.x);
-        h.geom = hh.y;
-        h.prim = hh.z;
-        }
+    uint32_t L = 0;
     const JobDev& J = jobs[job];
     if (valid)
         {
@@ -558,6 +547,30 @@ __global__ __launch_bounds__(BS) void k_shade_primary(const JobDev* __restrict__
             out.meta[slot] = make_uint4(job, pixel, k | (ls << 16), 1);
             }
         }
+    }
+
+template<int BS, uint32_t TYPES>
+__global__ __launch_bounds__(BS) void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs,
+                                                      const uint4* __restrict__ hits, uint32_t n, Queue out,
+                                                      uint32_t* out_count, uint4* records)
+    {
+    __shared__ uint32_t sh[BS / 64 + 1];
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t job = 0, pixel = 0, k = 0;
+    HitT h;
+    h.geom = EMPTY;
+    v3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
+    if (valid)
+        {
+        // the primary ray is regenerated rather than stored (bit-identical)
+        primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+        const uint4 hh = hits[i];
+        h.t = __uint_as_float(hh.x);
+        h.geom = hh.y;
+        h.prim = hh.z;
+        }
+    shade_primary_block<BS, TYPES>(jobs, valid, job, k, pixel, o, d, h, out, out_count, records, sh);
     }
 
 // Shade one hit or miss of a path at depth >= 1.  Returns true when the path
@@ -667,6 +680,238 @@ __device__ __forceinline__ bool leaf_test_uniform(const GeomDev* geoms, uint32_t
         return leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
     }
 
+// Closest hits of one wavefront's 64 primary rays (lane ray o, d of job `job`
+// at pixel px, py; `valid` lanes only): tile lists when all 64 rays form one
+// full tile of a job that has them, wavefront-packet BVH search when they
+// belong to one job, per-ray search otherwise.
+template<int SS, bool COUNT, uint32_t TYPES>
+__device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ jobs, bool valid, uint32_t job,
+                                                  uint32_t px, uint32_t py, v3 o, v3 d, LdsStack<SS>& st, HitT& best,
+                                                  uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
+    {
+    const uint32_t job0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);
+    const bool one_job = __ballot(valid && job != job0) == 0;
+    // the tile of this group, if all 64 rays belong to one full 8x8 tile
+    uint32_t tile = EMPTY;
+    if (one_job && jobs[job0].use_tiles)
+        {
+        const JobDev& J = jobs[job0];
+        if (valid && px < J.tiles_x * TILE && py < J.tiles_y * TILE)
+            tile = (py / TILE) * J.tiles_x + px / TILE;
+        const uint32_t t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile);
+        if (__ballot(tile != t0) != 0)
+            tile = EMPTY;
+        tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile);
+        }
+    if (!one_job)
+        {
+        if (valid)
+            trace_closest<LdsStack<SS>, COUNT, TYPES>(jobs[job].scene, o, d, 1e-3f, best, st, n_nodes,
+                                                      n_prims, err);
+        }
+    else if (tile != EMPTY)
+        {
+        // candidate list of the tile (kernels/tiles.hip), nearest bound
+        // first; the next entry is loaded while the current one is tested
+        const JobDev& J = jobs[job0];
+        const GeomDev* __restrict__ geoms = J.scene.geoms;
+        const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        const float tnear = 1e-3f;
+        best.t = __builtin_huge_valf();
+        best.geom = EMPTY;
+        best.prim = EMPTY;
+        const uint32_t e0 = ((const ConstU*)J.tile_start)[tile];
+        const uint32_t e1 = ((const ConstU*)J.tile_end)[tile];
+        const ConstU* ent = (const ConstU*)J.tile_entries;
+        uint32_t cur[8], nxt[8];
+        if (e0 < e1)
+            for (int k = 0; k < 8; k++)
+                cur[k] = ent[8 * (size_t)e0 + k];
+        for (uint32_t e = e0; e < e1; e++)
+            {
+            const size_t en = (e + 1 < e1) ? (size_t)e + 1 : (size_t)e;
+            for (int k = 0; k < 8; k++)
+                nxt[k] = ent[8 * en + k];
+            const float q = __uint_as_float(cur[0]);
+            const bool want = !(q > best.t);
+            if (__ballot(want) == 0)
+                break;
+            if (want)
+                {
+                float lx, ly, lz, hx, hy, hz;
+                float4 sph;
+                uint32_t slot, prim;
+                if constexpr (TYPES == T_SPHERES)
+                    {
+                    sph = make_float4(__uint_as_float(cur[1]), __uint_as_float(cur[2]),
+                                      __uint_as_float(cur[3]), __uint_as_float(cur[4]));
+                    // the sphere's bounds, as the builder computes them (GeometrySphere.cc:60-73)
+                    lx = sph.x - sph.w;
+                    ly = sph.y - sph.w;
+                    lz = sph.z - sph.w;
+                    hx = sph.x + sph.w;
+                    hy = sph.y + sph.w;
+                    hz = sph.z + sph.w;
+                    slot = cur[5];
+                    prim = cur[6];
+                    }
+                else
+                    {
+                    lx = __uint_as_float(cur[1]);
+                    ly = __uint_as_float(cur[2]);
+                    lz = __uint_as_float(cur[3]);
+                    hx = __uint_as_float(cur[4]);
+                    hy = __uint_as_float(cur[5]);
+                    hz = __uint_as_float(cur[6]);
+                    slot = 0;
+                    prim = 0;
+                    }
+                float tmin, tmax;
+                slab(o, inv, lx, ly, lz, hx, hy, hz, tmin, tmax);
+                if ((tmin <= tmax) && (tmax >= tnear))
+                    {
+                    float t;
+                    bool hit;
+                    if (COUNT)
+                        n_prims++;
+                    if constexpr (TYPES == T_SPHERES)
+                        {
+                        float de;
+                        bool first;
+                        hit = sphere_t(o, d, mk3(sph.x, sph.y, sph.z), sph.w, t, de, first) && (tnear < t);
+                        }
+                    else
+                        {
+                        const uint32_t r = cur[7];
+                        slot = __float_as_uint(J.prim_lo[r].w);
+                        prim = __float_as_uint(J.prim_hi[r].w);
+                        hit = leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
+                        }
+                    if (hit && better(t, slot, prim, best))
+                        {
+                        best.t = t;
+                        best.geom = slot;
+                        best.prim = prim;
+                        }
+                    }
+                }
+            for (int k = 0; k < 8; k++)
+                cur[k] = nxt[k];
+            }
+        }
+    else
+        {
+        const SceneDev& sc = jobs[job0].scene;
+        const Node* __restrict__ nodes = sc.nodes;
+        const GeomDev* __restrict__ geoms = sc.geoms;
+        const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+        const float tnear = 1e-3f;
+        const float margin = sc.cull_margin;
+        best.t = __builtin_huge_valf();
+        best.geom = EMPTY;
+        best.prim = EMPTY;
+        st.sp = 0;
+        uint32_t node = sc.root;
+        bool want = valid;
+        while (node != EMPTY)
+            {
+            // the node index is uniform: fetch through the constant address
+            // space so the node arrives in scalar registers
+            node = (uint32_t)__builtin_amdgcn_readfirstlane((int)node);
+            const ConstF* nf = (const ConstF*)(nodes + node);
+            const ConstU* nu = (const ConstU*)(nodes + node);
+            const float4 A = make_float4(nf[0], nf[1], nf[2], nf[3]);
+            const float4 B = make_float4(nf[4], nf[5], nf[6], nf[7]);
+            const float4 C = make_float4(nf[8], nf[9], nf[10], nf[11]);
+            const uint4 lk = make_uint4(nu[12], nu[13], nu[14], nu[15]);
+            if (COUNT && want)
+                n_nodes++;
+            float t0min, t0max, t1min, t1max;
+            slab(o, inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
+            slab(o, inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
+            const float c0 = cull_bound(t0min, t0max, margin);
+            const float c1 = cull_bound(t1min, t1max, margin);
+            bool w0 = want && (t0min <= t0max) && (t0max >= tnear) && !(c0 > best.t);
+            bool w1 = want && (t1min <= t1max) && (t1max >= tnear) && !(c1 > best.t);
+            if (lk.x & LEAF)
+                {
+                if (w0)
+                    {
+                    const uint32_t prim = lk.x & ~LEAF;
+                    float t;
+                    if (COUNT)
+                        n_prims++;
+                    if (leaf_test_uniform<TYPES>(geoms, lk.z, prim, o, d, tnear, t) && better(t, lk.z, prim, best))
+                        {
+                        best.t = t;
+                        best.geom = lk.z;
+                        best.prim = prim;
+                        }
+                    }
+                w0 = false;
+                }
+            if (lk.y & LEAF)
+                {
+                if (w1)
+                    {
+                    const uint32_t prim = lk.y & ~LEAF;
+                    float t;
+                    if (COUNT)
+                        n_prims++;
+                    if (leaf_test_uniform<TYPES>(geoms, lk.w, prim, o, d, tnear, t) && better(t, lk.w, prim, best))
+                        {
+                        best.t = t;
+                        best.geom = lk.w;
+                        best.prim = prim;
+                        }
+                    }
+                w1 = false;
+                }
+            w0 = w0 && !(c0 > best.t);
+            w1 = w1 && !(c1 > best.t);
+            const uint64_t b0 = __ballot(w0), b1 = __ballot(w1);
+            if (b0 && b1)
+                {
+                const uint64_t both = b0 & b1;
+                const uint64_t pref1 = __ballot(w0 && w1 && t1min < t0min);
+                const bool near1 = 2 * __popcll(pref1) > __popcll(both);
+                const uint32_t far_node = near1 ? lk.x : lk.y;
+                const bool far_want = near1 ? w0 : w1;
+                st.push(far_want ? far_node : (far_node | UNWANTED), near1 ? c0 : c1, err);
+                node = near1 ? lk.y : lk.x;
+                want = near1 ? w1 : w0;
+                continue;
+                }
+            if (b0)
+                {
+                node = lk.x;
+                want = w0;
+                continue;
+                }
+            if (b1)
+                {
+                node = lk.y;
+                want = w1;
+                continue;
+                }
+            node = EMPTY;
+            while (st.sp > 0)
+                {
+                uint32_t e;
+                float cv;
+                st.pop(e, cv);
+                const bool w = !(e & UNWANTED) && !(cv > best.t);
+                if (__ballot(w) != 0)
+                    {
+                    node = (uint32_t)__builtin_amdgcn_readfirstlane((int)e) & ~UNWANTED;
+                    want = w;
+                    break;
+                    }
+                }
+            }
+        }
+    }
+
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restrict__ jobs, uint32_t n_jobs,
                                                            uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
@@ -704,228 +949,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
                 uint32_t k;
                 primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
                 }
-            const uint32_t job0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);
             HitT best;
-            const bool one_job = __ballot(valid && job != job0) == 0;
-            // the tile of this group, if all 64 rays belong to one full 8x8 tile
-            uint32_t tile = EMPTY;
-            if (one_job && jobs[job0].use_tiles)
-                {
-                const JobDev& J = jobs[job0];
-                if (valid && px < J.tiles_x * TILE && py < J.tiles_y * TILE)
-                    tile = (py / TILE) * J.tiles_x + px / TILE;
-                const uint32_t t0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile);
-                if (__ballot(tile != t0) != 0)
-                    tile = EMPTY;
-                tile = (uint32_t)__builtin_amdgcn_readfirstlane((int)tile);
-                }
-            if (!one_job)
-                {
-                if (valid)
-                    trace_closest<LdsStack<SS>, COUNT, TYPES>(jobs[job].scene, o, d, 1e-3f, best, st, n_nodes,
-                                                              n_prims, err);
-                }
-            else if (tile != EMPTY)
-                {
-                // candidate list of the tile (kernels/tiles.hip), nearest bound
-                // first; the next entry is loaded while the current one is tested
-                const JobDev& J = jobs[job0];
-                const GeomDev* __restrict__ geoms = J.scene.geoms;
-                const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-                const float tnear = 1e-3f;
-                best.t = __builtin_huge_valf();
-                best.geom = EMPTY;
-                best.prim = EMPTY;
-                const uint32_t e0 = ((const ConstU*)J.tile_start)[tile];
-                const uint32_t e1 = ((const ConstU*)J.tile_end)[tile];
-                const ConstU* ent = (const ConstU*)J.tile_entries;
-                uint32_t cur[8], nxt[8];
-                if (e0 < e1)
-                    for (int k = 0; k < 8; k++)
-                        cur[k] = ent[8 * (size_t)e0 + k];
-                for (uint32_t e = e0; e < e1; e++)
-                    {
-                    const size_t en = (e + 1 < e1) ? (size_t)e + 1 : (size_t)e;
-                    for (int k = 0; k < 8; k++)
-                        nxt[k] = ent[8 * en + k];
-                    const float q = __uint_as_float(cur[0]);
-                    const bool want = !(q > best.t);
-                    if (__ballot(want) == 0)
-                        break;
-                    if (want)
-                        {
-                        float lx, ly, lz, hx, hy, hz;
-                        float4 sph;
-                        uint32_t slot, prim;
-                        if constexpr (TYPES == T_SPHERES)
-                            {
-                            sph = make_float4(__uint_as_float(cur[1]), __uint_as_float(cur[2]),
-                                              __uint_as_float(cur[3]), __uint_as_float(cur[4]));
-                            // the sphere's bounds, as the builder computes them (GeometrySphere.cc:60-73)
-                            lx = sph.x - sph.w;
-                            ly = sph.y - sph.w;
-                            lz = sph.z - sph.w;
-                            hx = sph.x + sph.w;
-                            hy = sph.y + sph.w;
-                            hz = sph.z + sph.w;
-                            slot = cur[5];
-                            prim = cur[6];
-                            }
-                        else
-                            {
-                            lx = __uint_as_float(cur[1]);
-                            ly = __uint_as_float(cur[2]);
-                            lz = __uint_as_float(cur[3]);
-                            hx = __uint_as_float(cur[4]);
-                            hy = __uint_as_float(cur[5]);
-                            hz = __uint_as_float(cur[6]);
-                            slot = 0;
-                            prim = 0;
-                            }
-                        float tmin, tmax;
-                        slab(o, inv, lx, ly, lz, hx, hy, hz, tmin, tmax);
-                        if ((tmin <= tmax) && (tmax >= tnear))
-                            {
-                            float t;
-                            bool hit;
-                            if (COUNT)
-                                n_prims++;
-                            if constexpr (TYPES == T_SPHERES)
-                                {
-                                float de;
-                                bool first;
-                                hit = sphere_t(o, d, mk3(sph.x, sph.y, sph.z), sph.w, t, de, first) && (tnear < t);
-                                }
-                            else
-                                {
-                                const uint32_t r = cur[7];
-                                slot = __float_as_uint(J.prim_lo[r].w);
-                                prim = __float_as_uint(J.prim_hi[r].w);
-                                hit = leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
-                                }
-                            if (hit && better(t, slot, prim, best))
-                                {
-                                best.t = t;
-                                best.geom = slot;
-                                best.prim = prim;
-                                }
-                            }
-                        }
-                    for (int k = 0; k < 8; k++)
-                        cur[k] = nxt[k];
-                    }
-                }
-            else
-                {
-                const SceneDev& sc = jobs[job0].scene;
-                const Node* __restrict__ nodes = sc.nodes;
-                const GeomDev* __restrict__ geoms = sc.geoms;
-                const v3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-                const float tnear = 1e-3f;
-                const float margin = sc.cull_margin;
-                best.t = __builtin_huge_valf();
-                best.geom = EMPTY;
-                best.prim = EMPTY;
-                st.sp = 0;
-                uint32_t node = sc.root;
-                bool want = valid;
-                while (node != EMPTY)
-                    {
-                    // the node index is uniform: fetch through the constant address
-                    // space so the node arrives in scalar registers
-                    node = (uint32_t)__builtin_amdgcn_readfirstlane((int)node);
-                    const ConstF* nf = (const ConstF*)(nodes + node);
-                    const ConstU* nu = (const ConstU*)(nodes + node);
-                    const float4 A = make_float4(nf[0], nf[1], nf[2], nf[3]);
-                    const float4 B = make_float4(nf[4], nf[5], nf[6], nf[7]);
-                    const float4 C = make_float4(nf[8], nf[9], nf[10], nf[11]);
-                    const uint4 lk = make_uint4(nu[12], nu[13], nu[14], nu[15]);
-                    if (COUNT && want)
-                        n_nodes++;
-                    float t0min, t0max, t1min, t1max;
-                    slab(o, inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
-                    slab(o, inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
-                    const float c0 = cull_bound(t0min, t0max, margin);
-                    const float c1 = cull_bound(t1min, t1max, margin);
-                    bool w0 = want && (t0min <= t0max) && (t0max >= tnear) && !(c0 > best.t);
-                    bool w1 = want && (t1min <= t1max) && (t1max >= tnear) && !(c1 > best.t);
-                    if (lk.x & LEAF)
-                        {
-                        if (w0)
-                            {
-                            const uint32_t prim = lk.x & ~LEAF;
-                            float t;
-                            if (COUNT)
-                                n_prims++;
-                            if (leaf_test_uniform<TYPES>(geoms, lk.z, prim, o, d, tnear, t) && better(t, lk.z, prim, best))
-                                {
-                                best.t = t;
-                                best.geom = lk.z;
-                                best.prim = prim;
-                                }
-                            }
-                        w0 = false;
-                        }
-                    if (lk.y & LEAF)
-                        {
-                        if (w1)
-                            {
-                            const uint32_t prim = lk.y & ~LEAF;
-                            float t;
-                            if (COUNT)
-                                n_prims++;
-                            if (leaf_test_uniform<TYPES>(geoms, lk.w, prim, o, d, tnear, t) && better(t, lk.w, prim, best))
-                                {
-                                best.t = t;
-                                best.geom = lk.w;
-                                best.prim = prim;
-                                }
-                            }
-                        w1 = false;
-                        }
-                    w0 = w0 && !(c0 > best.t);
-                    w1 = w1 && !(c1 > best.t);
-                    const uint64_t b0 = __ballot(w0), b1 = __ballot(w1);
-                    if (b0 && b1)
-                        {
-                        const uint64_t both = b0 & b1;
-                        const uint64_t pref1 = __ballot(w0 && w1 && t1min < t0min);
-                        const bool near1 = 2 * __popcll(pref1) > __popcll(both);
-                        const uint32_t far_node = near1 ? lk.x : lk.y;
-                        const bool far_want = near1 ? w0 : w1;
-                        st.push(far_want ? far_node : (far_node | UNWANTED), near1 ? c0 : c1, err);
-                        node = near1 ? lk.y : lk.x;
-                        want = near1 ? w1 : w0;
-                        continue;
-                        }
-                    if (b0)
-                        {
-                        node = lk.x;
-                        want = w0;
-                        continue;
-                        }
-                    if (b1)
-                        {
-                        node = lk.y;
-                        want = w1;
-                        continue;
-                        }
-                    node = EMPTY;
-                    while (st.sp > 0)
-                        {
-                        uint32_t e;
-                        float cv;
-                        st.pop(e, cv);
-                        const bool w = !(e & UNWANTED) && !(cv > best.t);
-                        if (__ballot(w) != 0)
-                            {
-                            node = (uint32_t)__builtin_amdgcn_readfirstlane((int)e) & ~UNWANTED;
-                            want = w;
-                            break;
-                            }
-                        }
-                    }
-                }
+            primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, err);
             if (valid)
                 {
                 hits[i] = make_uint4(__float_as_uint(best.t), best.geom, best.prim, 0);
@@ -934,6 +959,72 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
                 }
             }
         base = __shfl(next, 0, 64);
+        }
+    if (COUNT)
+        {
+        atomicAdd(counters + CTR_PRIMARY + 0, (unsigned long long)n_rays);
+        atomicAdd(counters + CTR_PRIMARY + 1, (unsigned long long)n_nodes);
+        atomicAdd(counters + CTR_PRIMARY + 2, (unsigned long long)n_prims);
+        }
+    if (err)
+        atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
+    }
+
+// Primary rays traced and shaded in one persistent kernel: the workgroup
+// claims `groups` x its size rays per atomic, each wavefront traces `groups`
+// groups of 64 rays (primary_group_hit) and the workgroup shades them
+// together (shade_primary_block), so no hit is stored and no primary ray is
+// generated twice.  Same arithmetic as k_traverse_packet + k_shade_primary.
+template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
+__global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t n,
+                                                   uint32_t* work, uint32_t groups, uint2* gstack, uint32_t gstride,
+                                                   Queue out, uint32_t* out_count, uint4* records,
+                                                   unsigned long long* counters)
+    {
+    __shared__ uint2 lds[SS * BLOCK];
+    __shared__ uint32_t sh[BLOCK / 64 + 1];
+    __shared__ uint32_t s_claim;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    LdsStack<SS> st;
+    st.lds = lds + threadIdx.x;
+    st.lstride = BLOCK;
+    st.glob = gstack + gtid;
+    st.gstride = gstride;
+    st.sp = 0;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t chunk = BLOCK * groups;
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    if (threadIdx.x == 0)
+        s_claim = atomicAdd(work, chunk);
+    __syncthreads();
+    uint32_t base = s_claim;
+    __syncthreads();
+    while (base < n)
+        {
+        if (threadIdx.x == 0)
+            s_claim = atomicAdd(work, chunk); // the next chunk, claimed ahead
+        for (uint32_t g = 0; g < groups; g++)
+            {
+            const uint32_t i = base + (wave * groups + g) * 64 + lane;
+            const bool valid = i < n;
+            v3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
+            uint32_t job = 0, k = 0, px = 0, py = 0;
+            if (valid)
+                primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
+            HitT best;
+            best.t = __builtin_huge_valf();
+            best.geom = EMPTY;
+            best.prim = EMPTY;
+            primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, err);
+            if (COUNT && valid)
+                n_rays++;
+            shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, o, d, best, out,
+                                              out_count, records, sh);
+            }
+        __syncthreads();
+        base = s_claim;
+        __syncthreads();
         }
     if (COUNT)
         {

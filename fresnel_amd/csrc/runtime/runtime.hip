@@ -164,6 +164,7 @@ struct DeviceObj
     uint32_t primary_chunk = 1024;      // the same for primary rays
     uint32_t trav_packet = 2;           // primary rays: 2 wavefront packets, 1 per ray
     uint32_t primary_tiles = 2;         // primary rays of pinhole/orthographic cameras: 2 tile lists, 1 BVH
+    uint32_t fuse_primary = 2;          // 2: primary traversal and depth-0 shading in one kernel, 1: two
     uint32_t trav_refill = 65;          // lanes take new rays once this many of a wavefront are idle
                                         // (65: batch-synchronous kernel, 64 rays at a time)
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
@@ -328,6 +329,56 @@ static void dispatch_ss(DeviceObj& D, const char* nm, hipStream_t s, const TravA
         }
     }
 
+// Fused primary traversal + depth-0 shading (k_primary), 256-thread workgroups.
+constexpr int PRIMARY_BLOCK = 256;
+struct PrimaryArgs
+    {
+    const JobDev* jobs;
+    uint32_t n_jobs;
+    uint32_t n;
+    uint32_t* work;
+    uint32_t groups;
+    uint2* gstack;
+    uint32_t gstride;
+    Queue out;
+    uint32_t* out_count;
+    uint4* records;
+    unsigned long long* counters;
+    };
+
+template<int SS, bool C, uint32_t T>
+static void launch_primary(DeviceObj& D, hipStream_t s, const PrimaryArgs& a)
+    {
+    auto kern = &k_primary<PRIMARY_BLOCK, SS, C, T>;
+    const uint32_t grid = D.grid_for(reinterpret_cast<const void*>(kern), PRIMARY_BLOCK);
+    REQUIRE((uint64_t)grid * PRIMARY_BLOCK <= a.gstride, "primary grid exceeds the overflow stacks");
+    D.launch("primary",
+             [&]
+             {
+                 kern<<<grid, PRIMARY_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.n, a.work, a.groups, a.gstack, a.gstride,
+                                                     a.out, a.out_count, a.records, a.counters);
+             });
+    }
+
+static void dispatch_primary(DeviceObj& D, hipStream_t s, const PrimaryArgs& a, bool spheres)
+    {
+    const bool c = D.count_on;
+    if (D.stack_lds == 8)
+        {
+        if (spheres)
+            c ? launch_primary<8, true, T_SPHERES>(D, s, a) : launch_primary<8, false, T_SPHERES>(D, s, a);
+        else
+            c ? launch_primary<8, true, T_ALL>(D, s, a) : launch_primary<8, false, T_ALL>(D, s, a);
+        }
+    else
+        {
+        if (spheres)
+            c ? launch_primary<16, true, T_SPHERES>(D, s, a) : launch_primary<16, false, T_SPHERES>(D, s, a);
+        else
+            c ? launch_primary<16, true, T_ALL>(D, s, a) : launch_primary<16, false, T_ALL>(D, s, a);
+        }
+    }
+
 static void dispatch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const TravArgs& a, bool primary, bool spheres)
     {
     if (D.stack_lds == 8)
@@ -361,6 +412,12 @@ template<int SS> static uint32_t max_grid_ss(DeviceObj& D)
                           reinterpret_cast<const void*>(&k_traverse_packet<TRAV_BLOCK, SS, false, T_ALL>),
                           reinterpret_cast<const void*>(&k_traverse_packet<TRAV_BLOCK, SS, true, T_ALL>)})
         g = std::max(g, D.grid_for(k, TRAV_BLOCK));
+    // fused primary kernel: its workgroups are PRIMARY_BLOCK / TRAV_BLOCK traversal workgroups
+    for (const void* k : {reinterpret_cast<const void*>(&k_primary<PRIMARY_BLOCK, SS, false, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_primary<PRIMARY_BLOCK, SS, true, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_primary<PRIMARY_BLOCK, SS, false, T_ALL>),
+                          reinterpret_cast<const void*>(&k_primary<PRIMARY_BLOCK, SS, true, T_ALL>)})
+        g = std::max(g, D.grid_for(k, PRIMARY_BLOCK) * (PRIMARY_BLOCK / TRAV_BLOCK));
     return g;
     }
 
@@ -1145,15 +1202,25 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
         dispatch_traverse(D, primary ? "traverse_primary" : "traverse", s, a, primary, spheres);
     };
 
-    traverse(Q[0], (uint32_t)n_pri, true);
-    HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
-    D.launch("shade_primary",
-             [&]
-             {
-                 auto* f = spheres ? k_shade_primary<SHADE_BLOCK, T_SPHERES> : k_shade_primary<SHADE_BLOCK, T_ALL>;
-                 f<<<blocks(n_pri, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri, Q[1],
-                                                                      qcount, R.records.p);
-             });
+    if (D.fuse_primary == 2)
+        {
+        HIPCHECK(hipMemsetAsync(work, 0, 2 * sizeof(uint32_t), s)); // work and qcount
+        PrimaryArgs a {R.jobs.p, nj, (uint32_t)n_pri, work, std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK),
+                       R.gstack.p, gstride, Q[1], qcount, R.records.p, ctr};
+        dispatch_primary(D, s, a, spheres);
+        }
+    else
+        {
+        traverse(Q[0], (uint32_t)n_pri, true);
+        HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
+        D.launch("shade_primary",
+                 [&]
+                 {
+                     auto* f = spheres ? k_shade_primary<SHADE_BLOCK, T_SPHERES> : k_shade_primary<SHADE_BLOCK, T_ALL>;
+                     f<<<blocks(n_pri, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri,
+                                                                          Q[1], qcount, R.records.p);
+                 });
+        }
     uint32_t n = 0;
     read_count(D, qcount, n);
     int cur = 1;
@@ -2528,6 +2595,15 @@ extern "C"
                         {
                         REQUIRE(value % 64 == 0 && value <= 65536, "trav_chunk must be a multiple of 64, at most 65536");
                         D.trav_chunk = (uint32_t)value;
+                        }
+                    }
+                else if (k == "fuse_primary")
+                    {
+                    prev = D.fuse_primary;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 2, "fuse_primary must be 1 (two kernels) or 2 (one kernel)");
+                        D.fuse_primary = (uint32_t)value;
                         }
                     }
                 else if (k == "primary_tiles")

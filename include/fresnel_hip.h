@@ -261,8 +261,9 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * a time (default 65); "trav_packet": primary rays traced as wavefront
  * packets (2, default) or one by one (1); "primary_tiles": primary rays of
  * pinhole-perspective and orthographic cameras tested against per-8x8-tile
- * candidate lists (2, default) or searched in the BVH (1).  value <= 0 only
- * reads the current value into *previous. */
+ * candidate lists (2, default) or searched in the BVH (1); "fuse_primary":
+ * primary traversal and depth-0 shading in one kernel (2, default) or two (1).
+ * value <= 0 only reads the current value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
 /* ------------------------------------------------------ diagnostics (no
