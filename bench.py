@@ -280,7 +280,7 @@ def run_plain(args):
     if not args.no_tune:
         from fresnel_amd import autotune
         t0 = time.perf_counter()
-        tuned = autotune.tune(dev, scene, W, H, samples=max(1, S // 4), light_samples=L, use_cache=False,
+        tuned = autotune.tune(dev, scene, W, H, samples=max(1, S // 2), light_samples=L, use_cache=False,
                               log=lambda m: print("# " + m, file=sys.stderr, flush=True))
         print(f"# autotune: {tuned} in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     d.statsEnable(True)
@@ -330,10 +330,15 @@ def run_plain(args):
                 "nodes_per_ray": round(nodes / max(rays, 1), 2), "prims_per_ray": round(prims / max(rays, 1), 2),
                 "ms_per_step": round(ms, 3)}
 
-    # primary rays are generated in the kernel and write a 16 B hit; secondary
-    # rays read 32 B of ray from the queue and write a 16 B hit
-    roof = {"traverse_primary": kernel_roofline("traverse_primary", ctr[3], ctr[4], ctr[5], 16,
-                                                "primary-ray traversal (k_traverse_packet / k_traverse)"),
+    # primary rays are generated in the kernel; the fused primary kernel
+    # (k_primary) writes at least a 16 B record or queue entry per ray, the
+    # unfused one a 16 B hit.  Secondary rays read 32 B of ray from the queue
+    # and write a 16 B hit.
+    primary_name = "primary" if "primary" in kstats[-1] else "traverse_primary"
+    roof = {primary_name: kernel_roofline(primary_name, ctr[3], ctr[4], ctr[5], 16,
+                                          "primary rays: tile lists or BVH packets, fused with depth-0 shading "
+                                          "(k_primary)" if primary_name == "primary" else
+                                          "primary-ray traversal (k_traverse_packet / k_traverse)"),
             "traverse": kernel_roofline("traverse", ctr[0], ctr[1], ctr[2], 48,
                                         "secondary-ray traversal (k_traverse)")}
     dominant = max(roof, key=lambda k: roof[k]["ms_per_step"])
@@ -357,6 +362,7 @@ def run_plain(args):
                           "achieved_GBs": round(path_bytes / mean_s / 1e9, 1),
                           "frac": round(path_bytes / mean_s / 1e9 / HBM_PEAK_GBS, 4)},
         "tail": {"rays": int(ctr[6]), "nodes_per_ray": round(ctr[7] / max(ctr[6], 1), 2)},
+        "primary_tile_entries_per_ray": round(ctr[10] / max(ctr[3], 1), 2),
         "kernel_ms_last_step": kernel_ms,
         "kernel_launches_last_step": kernel_launches,
         "tuning": tuned,

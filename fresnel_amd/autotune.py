@@ -20,6 +20,7 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
                   or searched in the BVH (1).
 * ``fuse_primary`` primary traversal and depth-0 shading in one kernel (2) or
                   two (1).
+* ``fuse_bounce`` the same for each bounce of the secondary rays.
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -41,7 +42,11 @@ CANDIDATES = {
     'trav_packet': [1, 2],
     'primary_tiles': [1, 2],
     'fuse_primary': [1, 2],
+    'fuse_bounce': [1, 2],
 }
+
+
+MIN_GAIN = 0.02  # relative improvement a knob change must show to be kept
 
 
 def cache_path():
@@ -123,6 +128,8 @@ def tune(device, scene, w, h, samples=16, light_samples=1, repeats=3, use_cache=
         return min(dt)
 
     best_t = timed(best)
+    if log:
+        log(f"autotune {key}: {best} -> {best_t * 1e3:.2f} ms (start)")
     for k, values in CANDIDATES.items():
         for v in values:
             if v == best[k]:
@@ -132,7 +139,8 @@ def tune(device, scene, w, h, samples=16, light_samples=1, repeats=3, use_cache=
             dt = timed(trial)
             if log:
                 log(f"autotune {key}: {trial} -> {dt * 1e3:.2f} ms")
-            if dt < best_t:
+            # a change must win by more than the timing noise of short renders
+            if dt < best_t * (1.0 - MIN_GAIN):
                 best, best_t = trial, dt
     apply(device, best)
     store(device, key, best)

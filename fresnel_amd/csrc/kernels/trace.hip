@@ -398,7 +398,7 @@ __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uin
     job = find_job(jobs, n_jobs, i);
     const JobDev& J = jobs[job];
     primary_order(J, i - J.ray_base, k, px, py);
-    camera_ray(J.cam, J.W, J.H, J.seed, px, py, J.n_base + k + 1, true, o, d);
+    camera_ray(J.cam, J.W, J.H, J.xs_off, J.seed, px, py, J.n_base + k + 1, true, o, d);
     }
 
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
@@ -687,7 +687,8 @@ __device__ __forceinline__ bool leaf_test_uniform(const GeomDev* geoms, uint32_t
 template<int SS, bool COUNT, uint32_t TYPES>
 __device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ jobs, bool valid, uint32_t job,
                                                   uint32_t px, uint32_t py, v3 o, v3 d, LdsStack<SS>& st, HitT& best,
-                                                  uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
+                                                  uint32_t& n_nodes, uint32_t& n_prims, uint32_t& n_entries,
+                                                  uint32_t& err)
     {
     const uint32_t job0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);
     const bool one_job = __ballot(valid && job != job0) == 0;
@@ -736,6 +737,8 @@ __device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ job
             const bool want = !(q > best.t);
             if (__ballot(want) == 0)
                 break;
+            if (COUNT)
+                n_entries++;
             if (want)
                 {
                 float lx, ly, lz, hx, hy, hz;
@@ -927,7 +930,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
     st.gstride = gstride;
     st.sp = 0;
     const uint32_t lane = lane_id();
-    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0, n_entries = 0;
     uint32_t claim = 0;
     if (lane == 0)
         claim = atomicAdd(work, chunk);
@@ -950,7 +953,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
                 primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
                 }
             HitT best;
-            primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, err);
+            primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, n_entries,
+                                                err);
             if (valid)
                 {
                 hits[i] = make_uint4(__float_as_uint(best.t), best.geom, best.prim, 0);
@@ -965,6 +969,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
         atomicAdd(counters + CTR_PRIMARY + 0, (unsigned long long)n_rays);
         atomicAdd(counters + CTR_PRIMARY + 1, (unsigned long long)n_nodes);
         atomicAdd(counters + CTR_PRIMARY + 2, (unsigned long long)n_prims);
+        atomicAdd(counters + CTR_TILE_ENTRIES, (unsigned long long)n_entries);
         }
     if (err)
         atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
@@ -994,7 +999,7 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t chunk = BLOCK * groups;
-    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0, n_entries = 0;
     if (threadIdx.x == 0)
         s_claim = atomicAdd(work, chunk);
     __syncthreads();
@@ -1016,7 +1021,8 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
             best.t = __builtin_huge_valf();
             best.geom = EMPTY;
             best.prim = EMPTY;
-            primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, err);
+            primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, n_entries,
+                                                err);
             if (COUNT && valid)
                 n_rays++;
             shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, o, d, best, out,
@@ -1031,6 +1037,91 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
         atomicAdd(counters + CTR_PRIMARY + 0, (unsigned long long)n_rays);
         atomicAdd(counters + CTR_PRIMARY + 1, (unsigned long long)n_nodes);
         atomicAdd(counters + CTR_PRIMARY + 2, (unsigned long long)n_prims);
+        atomicAdd(counters + CTR_TILE_ENTRIES, (unsigned long long)n_entries);
+        }
+    if (err)
+        atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
+    }
+
+// One bounce of the secondary rays in one persistent kernel: each wavefront
+// claims `chunk` rays of the queue per atomic (the next chunk ahead), traces
+// each ray and shades its hit or miss (shade_bounce); continuing paths are
+// appended to `out` with one atomic per 64 rays, terminated ones write their
+// record.  Same arithmetic as k_traverse + k_shade.
+template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
+__global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ jobs, Queue in, uint32_t n,
+                                                  uint32_t* work, uint32_t chunk, uint2* gstack, uint32_t gstride,
+                                                  Queue out, uint32_t* out_count, uint4* records, uint32_t max_depth,
+                                                  unsigned long long* counters)
+    {
+    __shared__ uint2 lds[SS * BLOCK];
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    LdsStack<SS> st;
+    st.lds = lds + threadIdx.x;
+    st.lstride = BLOCK;
+    st.glob = gstack + gtid;
+    st.gstride = gstride;
+    st.sp = 0;
+    const uint32_t lane = lane_id();
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    uint32_t claim = 0;
+    if (lane == 0)
+        claim = atomicAdd(work, chunk);
+    uint32_t base = __shfl(claim, 0, 64);
+    while (base < n)
+        {
+        uint32_t next = 0;
+        if (lane == 0)
+            next = atomicAdd(work, chunk);
+        const uint32_t end = min(n, base + chunk);
+        for (uint32_t sub = base; sub < end; sub += 64)
+            {
+            const uint32_t i = sub + lane;
+            bool cont = false;
+            v3 att, no, nd;
+            uint4 mt = make_uint4(0, 0, 0, 0);
+            if (i < end)
+                {
+                mt = in.meta[i];
+                const JobDev& J = jobs[mt.x];
+                const float4 a = in.o[i];
+                const float4 b = in.d[i];
+                const float4 at4 = in.att[i];
+                const v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z);
+                HitT h;
+                trace_closest<LdsStack<SS>, COUNT, TYPES>(J.scene, o, d, a.w, h, st, n_nodes, n_prims, err);
+                if (COUNT)
+                    n_rays++;
+                att = mk3(at4.x, at4.y, at4.z);
+                const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
+                uint4 rec;
+                cont = shade_bounce<TYPES>(J, h, o, d, a.w, mt.y, k, ls, mt.w, att, no, nd, rec);
+                if (cont && mt.w + 1 >= max_depth)
+                    {
+                    // depth cap (the reference has none): stop without contribution
+                    cont = false;
+                    rec = make_rec(mk3(0, 0, 0), REC_NONE);
+                    atomicAdd(counters + CTR_CAPPED, 1ull);
+                    }
+                if (!cont)
+                    records[rec_index(J, k, mt.y, ls)] = rec;
+                }
+            const uint32_t slot = wave_append(cont, out_count);
+            if (cont)
+                {
+                out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
+                out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
+                out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
+                out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
+                }
+            }
+        base = __shfl(next, 0, 64);
+        }
+    if (COUNT)
+        {
+        atomicAdd(counters + CTR_SECONDARY + 0, (unsigned long long)n_rays);
+        atomicAdd(counters + CTR_SECONDARY + 1, (unsigned long long)n_nodes);
+        atomicAdd(counters + CTR_SECONDARY + 2, (unsigned long long)n_prims);
         }
     if (err)
         atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
@@ -1271,7 +1362,7 @@ __global__ void k_preview(const JobDev* __restrict__ jobp, uint32_t aa_n)
     for (uint32_t sample = 0; sample < aa_n * aa_n; sample++)
         {
         v3 o, d;
-        camera_ray(J.cam, J.W, J.H, J.seed, px, py, sample, aa_n != 1, o, d);
+        camera_ray(J.cam, J.W, J.H, J.xs_off, J.seed, px, py, sample, aa_n != 1, o, d);
         HitT h;
         trace_closest<LocalStack, false>(J.scene, o, d, 0.0f, h, st, dummy, dummy, err);
         v3 c = J.bg;
@@ -1357,7 +1448,7 @@ __global__ void k_debug_primary(const JobDev* __restrict__ jobp, uint32_t sample
         return;
     const uint32_t py = pixel / J.W, px = pixel - py * J.W;
     v3 o, d;
-    camera_ray(J.cam, J.W, J.H, J.seed, px, py, sample, true, o, d);
+    camera_ray(J.cam, J.W, J.H, J.xs_off, J.seed, px, py, sample, true, o, d);
     LocalStack st;
     uint32_t dummy = 0, err = 0;
     HitT h;

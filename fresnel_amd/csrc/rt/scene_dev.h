@@ -62,8 +62,11 @@ struct GeomDev
 // EMPTY with a NaN box that every slab test rejects.
 // Device work-counter slots (64-bit each): rays traced, BVH nodes visited
 // and primitive tests of the primary, secondary and tail traversals; paths
-// stopped at the depth cap; traversal-stack overflow flag.
-constexpr int CTR_SECONDARY = 0, CTR_PRIMARY = 3, CTR_TAIL = 6, CTR_CAPPED = 9, CTR_OVERFLOW = 15, N_CTR = 16;
+// stopped at the depth cap; tile-list entries examined by primary rays (per
+// ray, each entry a wavefront reads counts once per lane); traversal-stack
+// overflow flag.
+constexpr int CTR_SECONDARY = 0, CTR_PRIMARY = 3, CTR_TAIL = 6, CTR_CAPPED = 9, CTR_TILE_ENTRIES = 10,
+              CTR_OVERFLOW = 15, N_CTR = 16;
 
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t EMPTY = 0xffffffffu;
@@ -117,6 +120,7 @@ struct JobDev
     v3 bg;
     float bg_alpha;
     uint32_t W, H, npix;
+    float xs_off;         // 0.5f * float(W) / float(H)
     uint32_t seed;
     uint32_t n_base;      // passes accumulated before this batch
     uint32_t K;           // camera samples in this batch

@@ -203,8 +203,10 @@ __device__ __forceinline__ float pdf_diffuse(v3 l, v3 n)
 
 // ---------------------------------------------------------------- camera
 // Camera::importanceSampleAA + generateRay, fresnel/common/Camera.h:169-269
-__device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, uint32_t seed, uint32_t i, uint32_t j,
-                                  uint32_t sample, bool aa, v3& o, v3& d)
+// xs_off = 0.5f * float(W) / float(H), evaluated once per image by the host
+// (the same IEEE float operations, so the same value).
+__device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, float xs_off, uint32_t seed, uint32_t i,
+                                  uint32_t j, uint32_t sample, bool aa, v3& o, v3& d)
     {
     const uint32_t key0 = j * W + i;
     float i_f, j_f;
@@ -225,7 +227,7 @@ __device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, uint3
         j_f = float(j) + 0.5f;
         }
     const float ys = -1.0f * (j_f / float(H) - 0.5f);
-    const float xs = i_f / float(H) - 0.5f * float(W) / float(H);
+    const float xs = i_f / float(H) - xs_off;
     if (c.model == 0)
         {
         o = c.p + (ys * c.v + xs * c.u) * c.focal_h;

@@ -165,6 +165,7 @@ struct DeviceObj
     uint32_t trav_packet = 2;           // primary rays: 2 wavefront packets, 1 per ray
     uint32_t primary_tiles = 2;         // primary rays of pinhole/orthographic cameras: 2 tile lists, 1 BVH
     uint32_t fuse_primary = 2;          // 2: primary traversal and depth-0 shading in one kernel, 1: two
+    uint32_t fuse_bounce = 1;           // 2: secondary traversal and shading in one kernel, 1: two
     uint32_t trav_refill = 65;          // lanes take new rays once this many of a wavefront are idle
                                         // (65: batch-synchronous kernel, 64 rays at a time)
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
@@ -379,6 +380,55 @@ static void dispatch_primary(DeviceObj& D, hipStream_t s, const PrimaryArgs& a, 
         }
     }
 
+// Fused secondary traversal + shading (k_bounce).
+struct BounceArgs
+    {
+    const JobDev* jobs;
+    Queue in;
+    uint32_t n;
+    uint32_t* work;
+    uint32_t chunk;
+    uint2* gstack;
+    uint32_t gstride;
+    Queue out;
+    uint32_t* out_count;
+    uint4* records;
+    unsigned long long* counters;
+    };
+
+template<int SS, bool C, uint32_t T>
+static void launch_bounce(DeviceObj& D, hipStream_t s, const BounceArgs& a)
+    {
+    auto kern = &k_bounce<TRAV_BLOCK, SS, C, T>;
+    const uint32_t grid = D.grid_for(reinterpret_cast<const void*>(kern), TRAV_BLOCK);
+    REQUIRE((uint64_t)grid * TRAV_BLOCK <= a.gstride, "bounce grid exceeds the overflow stacks");
+    D.launch("bounce",
+             [&]
+             {
+                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.in, a.n, a.work, a.chunk, a.gstack, a.gstride, a.out,
+                                                 a.out_count, a.records, MAX_DEPTH, a.counters);
+             });
+    }
+
+static void dispatch_bounce(DeviceObj& D, hipStream_t s, const BounceArgs& a, bool spheres)
+    {
+    const bool c = D.count_on;
+    if (D.stack_lds == 8)
+        {
+        if (spheres)
+            c ? launch_bounce<8, true, T_SPHERES>(D, s, a) : launch_bounce<8, false, T_SPHERES>(D, s, a);
+        else
+            c ? launch_bounce<8, true, T_ALL>(D, s, a) : launch_bounce<8, false, T_ALL>(D, s, a);
+        }
+    else
+        {
+        if (spheres)
+            c ? launch_bounce<16, true, T_SPHERES>(D, s, a) : launch_bounce<16, false, T_SPHERES>(D, s, a);
+        else
+            c ? launch_bounce<16, true, T_ALL>(D, s, a) : launch_bounce<16, false, T_ALL>(D, s, a);
+        }
+    }
+
 static void dispatch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const TravArgs& a, bool primary, bool spheres)
     {
     if (D.stack_lds == 8)
@@ -418,6 +468,11 @@ template<int SS> static uint32_t max_grid_ss(DeviceObj& D)
                           reinterpret_cast<const void*>(&k_primary<PRIMARY_BLOCK, SS, false, T_ALL>),
                           reinterpret_cast<const void*>(&k_primary<PRIMARY_BLOCK, SS, true, T_ALL>)})
         g = std::max(g, D.grid_for(k, PRIMARY_BLOCK) * (PRIMARY_BLOCK / TRAV_BLOCK));
+    for (const void* k : {reinterpret_cast<const void*>(&k_bounce<TRAV_BLOCK, SS, false, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_bounce<TRAV_BLOCK, SS, true, T_SPHERES>),
+                          reinterpret_cast<const void*>(&k_bounce<TRAV_BLOCK, SS, false, T_ALL>),
+                          reinterpret_cast<const void*>(&k_bounce<TRAV_BLOCK, SS, true, T_ALL>)})
+        g = std::max(g, D.grid_for(k, TRAV_BLOCK));
     return g;
     }
 
@@ -1111,6 +1166,7 @@ static JobDev make_job(SceneObj& sc, TracerObj& tr, const fh_user_camera& cam_us
     J.W = tr.W;
     J.H = tr.H;
     J.npix = tr.W * tr.H;
+    J.xs_off = 0.5f * float(tr.W) / float(tr.H); // as fresnel/common/Camera.h:267 evaluates it per ray
     J.seed = tr.seed;
     J.n_base = tr.n_samples;
     J.L = tr.L;
@@ -1241,15 +1297,25 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
                      });
             break;
             }
-        traverse(Q[cur], n, false);
-        HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
-        D.launch("shade",
-                 [&]
-                 {
-                     auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
-                     f<<<blocks(n, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, n, Q[1 - cur],
-                                                                      qcount, R.records.p, MAX_DEPTH, ctr);
-                 });
+        if (D.fuse_bounce == 2)
+            {
+            HIPCHECK(hipMemsetAsync(work, 0, 2 * sizeof(uint32_t), s)); // work and qcount
+            BounceArgs a {R.jobs.p, Q[cur], n, work, D.trav_chunk, R.gstack.p, gstride, Q[1 - cur], qcount,
+                          R.records.p, ctr};
+            dispatch_bounce(D, s, a, spheres);
+            }
+        else
+            {
+            traverse(Q[cur], n, false);
+            HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
+            D.launch("shade",
+                     [&]
+                     {
+                         auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
+                         f<<<blocks(n, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, n, Q[1 - cur],
+                                                                          qcount, R.records.p, MAX_DEPTH, ctr);
+                     });
+            }
         read_count(D, qcount, n);
         cur = 1 - cur;
         }
@@ -2595,6 +2661,15 @@ extern "C"
                         {
                         REQUIRE(value % 64 == 0 && value <= 65536, "trav_chunk must be a multiple of 64, at most 65536");
                         D.trav_chunk = (uint32_t)value;
+                        }
+                    }
+                else if (k == "fuse_bounce")
+                    {
+                    prev = D.fuse_bounce;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 2, "fuse_bounce must be 1 (two kernels) or 2 (one kernel)");
+                        D.fuse_bounce = (uint32_t)value;
                         }
                     }
                 else if (k == "fuse_primary")

@@ -262,7 +262,10 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * packets (2, default) or one by one (1); "primary_tiles": primary rays of
  * pinhole-perspective and orthographic cameras tested against per-8x8-tile
  * candidate lists (2, default) or searched in the BVH (1); "fuse_primary":
- * primary traversal and depth-0 shading in one kernel (2, default) or two (1).
+ * primary traversal and depth-0 shading in one kernel (2, default) or two (1);
+ * "fuse_bounce": the same for each bounce of the secondary rays (default 1:
+ * the fused kernel holds 88 registers, and the latency-bound secondary
+ * traversal runs faster at the 8 wavefronts per SIMD of the separate kernel).
  * value <= 0 only reads the current value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
@@ -304,8 +307,8 @@ int fh_stats_enable(fh_device dev, int enable);
  * nodes visited and primitive tests of the secondary-ray traversal kernel
  * ([0], [1], [2]), the primary-ray traversal kernel ([3], [4], [5]) and the
  * tail kernel that finishes the last paths of a batch ([6], [7], [8]); [9]
- * paths stopped at the depth cap; [15] non-zero if a traversal stack
- * overflowed.  Counting is enabled with fh_counters_enable and uses
+ * paths stopped at the depth cap; [10] tile-list entries examined by primary
+ * rays; [15] non-zero if a traversal stack overflowed.  Counting is enabled with fh_counters_enable and uses
  * instrumented kernels (slower; the counts do not depend on timing). */
 int fh_counters_enable(fh_device dev, int enable);
 int fh_counters_last(fh_device dev, uint64_t out[16]);
