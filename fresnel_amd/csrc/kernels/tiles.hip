@@ -26,6 +26,7 @@ struct TileCam
     int32_t model;     // 0 orthographic, 1 perspective
     uint32_t tiles_x, tiles_y;
     float margin;      // CULL_K * 3 * largest primitive half-extent
+    float plane_r;     // orthographic: largest distance of a ray origin from p; perspective: 0
     };
 
 // Tile rectangle [tx0, tx1] x [ty0, ty1] of a primitive box and the lower
@@ -99,8 +100,101 @@ __device__ __forceinline__ bool tile_rect(const TileCam& c, float4 lo, float4 hi
     return true;
     }
 
-__global__ void k_tile_count(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t N, TileCam c,
-                             uint32_t* counts, uint32_t* flags)
+// Pixel interval [a0, a1] (before the jitter margins) of the directions or
+// origins whose rays pass within rho of a centre at camera coordinates
+// (x, z): tangent slopes x/z of the circle for perspective rays, which needs
+// z > 2 rho (false otherwise), the shifted interval for orthographic rays.
+// The interval is widened by 2^-16 (1 + |slope|), far above the float error of
+// the formula at z > 2 rho.
+__device__ __forceinline__ bool ball_interval(const TileCam& c, float x, float z, float rho, float& a0, float& a1)
+    {
+    float m0, m1;
+    if (c.model == 1)
+        {
+        if (!(z > 2.0f * rho))
+            return false;
+        const float den = z * z - rho * rho;
+        const float root = rho * sqrtf(x * x + den);
+        m0 = (x * z - root) / den;
+        m1 = (x * z + root) / den;
+        }
+    else
+        {
+        m0 = x - rho;
+        m1 = x + rho;
+        }
+    m0 -= 0x1p-16f * (1.0f + fabsf(m0));
+    m1 += 0x1p-16f * (1.0f + fabsf(m1));
+    a0 = m0 * c.scale;
+    a1 = m1 * c.scale;
+    return true;
+    }
+
+// Tighter rectangle and bound for a sphere (centre, radius s), intersected
+// with the box rule's.  A ray whose computed sphere solve reports a hit passes
+// the true centre within rho, rho^2 = r^2 (1 + 2^-20) + 2^-18 far^2: the float
+// discriminant b*b - v.v + r*r errs by about 19 u |v|^2 (u = 2^-24, |v| <= far
+// the distance from the ray origin to the centre).  Its computed distance is
+// at least (distance of the centre, or its depth for orthographic rays) - r
+// - CULL_K |v|, the same sqrt(14 u) |v| cancellation bound as the BVH cull.
+__device__ __forceinline__ bool sphere_tile_rect(const TileCam& c, float4 s, float4 lo, float4 hi, int& tx0, int& tx1,
+                                                 int& ty0, int& ty1, float& q, bool& straddle)
+    {
+    if (!tile_rect(c, lo, hi, tx0, tx1, ty0, ty1, q, straddle))
+        return false;
+    const v3 e = mk3(s.x, s.y, s.z) - c.p;
+    const float dist = sqrtf(dot(e, e));
+    const float far = dist * (1.0f + 0x1p-20f) + c.plane_r;
+    const float rho = sqrtf(s.w * s.w * (1.0f + 0x1p-20f) + 0x1p-18f * far * far) * (1.0f + 0x1p-20f);
+    const float cx = dot(e, c.u), cy = dot(e, c.v), cz = -dot(e, c.w);
+    float i0, i1, j0, j1;
+    if (ball_interval(c, cx, cz, rho, i0, i1) && ball_interval(c, cy, cz, rho, j0, j1))
+        {
+        // screen position -> pixel: i = xs H + W / 2, j = (1/2 - ys) H
+        const float imin = i0 * c.H + 0.5f * c.W, imax = i1 * c.H + 0.5f * c.W;
+        const float jmin = (0.5f - j1) * c.H, jmax = (0.5f - j0) * c.H;
+        const float lim = 1.0e6f;
+        const int x0 = (int)floorf(fmaxf(imin - 2.25f, -lim)), x1 = (int)floorf(fminf(imax + 1.25f, lim));
+        const int y0 = (int)floorf(fmaxf(jmin - 2.25f, -lim)), y1 = (int)floorf(fminf(jmax + 1.25f, lim));
+        const int nx = (int)(c.tiles_x * TILE), ny = (int)(c.tiles_y * TILE);
+        if (x1 < 0 || y1 < 0 || x0 >= nx || y0 >= ny)
+            return false;
+        tx0 = max(tx0, max(x0, 0) / (int)TILE);
+        tx1 = min(tx1, min(x1, nx - 1) / (int)TILE);
+        ty0 = max(ty0, max(y0, 0) / (int)TILE);
+        ty1 = min(ty1, min(y1, ny - 1) / (int)TILE);
+        if (tx0 > tx1 || ty0 > ty1)
+            return false;
+        }
+    const float base = ((c.model == 1) ? dist : cz) - s.w;
+    const float qs = base - 0x1p-20f * (far + s.w) - (CULL_K * far + c.margin);
+    q = fmaxf(q, fmaxf(qs, 0.0f));
+    return true;
+    }
+
+// Tile rectangle and bound of primitive i: the box rule, tightened by the
+// sphere rule in spheres-only scenes (the sphere through the bounds' slot and
+// primitive bits).
+template<bool SPHERES>
+__device__ __forceinline__ bool prim_tile_rect(const TileCam& c, const float4* __restrict__ lo,
+                                               const float4* __restrict__ hi, const GeomDev* __restrict__ geoms,
+                                               uint32_t i, int& tx0, int& tx1, int& ty0, int& ty1, float& q,
+                                               bool& straddle)
+    {
+    const float4 a = lo[i], b = hi[i];
+    if constexpr (SPHERES)
+        {
+        const float4 s = geoms[__float_as_uint(a.w)].a[__float_as_uint(b.w)];
+        return sphere_tile_rect(c, s, a, b, tx0, tx1, ty0, ty1, q, straddle);
+        }
+    else
+        return tile_rect(c, a, b, tx0, tx1, ty0, ty1, q, straddle);
+    }
+
+template<bool SPHERES>
+__global__ void k_tile_count(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                             const GeomDev* __restrict__ geoms, uint32_t N, TileCam c, uint32_t* counts,
+                             uint32_t* flags)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > N)
@@ -114,7 +208,7 @@ __global__ void k_tile_count(const float4* __restrict__ lo, const float4* __rest
     float q;
     bool straddle = false;
     uint32_t n = 0;
-    if (tile_rect(c, lo[i], hi[i], tx0, tx1, ty0, ty1, q, straddle))
+    if (prim_tile_rect<SPHERES>(c, lo, hi, geoms, i, tx0, tx1, ty0, ty1, q, straddle))
         n = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
     counts[i] = n;
     if (straddle)
@@ -122,7 +216,9 @@ __global__ void k_tile_count(const float4* __restrict__ lo, const float4* __rest
     }
 
 // Entries of primitive i at offsets[i]: tile index, bound bits, primitive.
-__global__ void k_tile_emit(const float4* __restrict__ lo, const float4* __restrict__ hi, uint32_t N, TileCam c,
+template<bool SPHERES>
+__global__ void k_tile_emit(const float4* __restrict__ lo, const float4* __restrict__ hi,
+                            const GeomDev* __restrict__ geoms, uint32_t N, TileCam c,
                             const uint32_t* __restrict__ offsets, uint32_t* tile, uint32_t* qbits, uint32_t* ref,
                             uint32_t* order)
     {
@@ -132,7 +228,7 @@ __global__ void k_tile_emit(const float4* __restrict__ lo, const float4* __restr
     int tx0, tx1, ty0, ty1;
     float q;
     bool straddle = false;
-    if (!tile_rect(c, lo[i], hi[i], tx0, tx1, ty0, ty1, q, straddle))
+    if (!prim_tile_rect<SPHERES>(c, lo, hi, geoms, i, tx0, tx1, ty0, ty1, q, straddle))
         return;
     uint32_t e = offsets[i];
     const uint32_t e_end = offsets[i + 1]; // the count pass saw the same rectangle

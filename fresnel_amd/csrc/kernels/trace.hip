@@ -248,6 +248,64 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
     best = r.best;
     }
 
+// A ray queue is cut into QSEG segments with one append counter each, the
+// counters QSEG_STRIDE words (256 bytes) apart: a producer unit (a claimed
+// chunk of a persistent kernel, or a workgroup) appends to segment
+// unit % QSEG, so the append atomics spread over QSEG addresses instead of
+// queueing on one (one counter's atomic rate bounded the shading kernels).
+// Segment s keeps its entries at [s seg_cap, s seg_cap + count_s); the host
+// sizes seg_cap from the producers' per-unit output bounds, so no segment can
+// overflow (a full segment raises a flag instead of writing past it).
+// Consumers see the entries as one dense index range: entry i of the queue
+// lives at slot i + adj[s] for the last segment s with pre[s] <= i.
+constexpr uint32_t QSEG = 16, QSEG_STRIDE = 64;
+constexpr uint32_t QSEG_FLAG = QSEG * QSEG_STRIDE; // overflow flag word after the counters
+
+struct Queue
+    {
+    float4* o;   // origin, tnear
+    float4* d;   // direction, job (bits)
+    float4* att; // attenuation
+    uint4* meta; // job, pixel, sample-in-batch | light sample << 16, depth
+    uint32_t* count;      // producer: append counters (count[s * QSEG_STRIDE]) and flag
+    uint32_t seg_cap;     // slots per segment
+    uint32_t pre[QSEG];   // consumer: first dense index of each segment
+    uint32_t adj[QSEG];   // consumer: slot - index within segment s (s seg_cap - pre[s])
+    };
+
+// slot - index for dense index i (pre is non-decreasing, pre[0] = 0)
+__device__ __forceinline__ uint32_t seg_adj(const Queue& q, uint32_t i)
+    {
+    uint32_t a = 0;
+#pragma unroll
+    for (uint32_t s = 1; s < QSEG; s++)
+        a = (i >= q.pre[s]) ? q.adj[s] : a;
+    return a;
+    }
+
+// Slot of dense index i, for lanes whose indices lie in [lo, hi] (both
+// wavefront-uniform): one scalar offset when the range is in one segment.
+__device__ __forceinline__ uint32_t q_slot(const Queue& q, uint32_t i, uint32_t lo, uint32_t hi)
+    {
+    lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)lo);
+    hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)hi);
+    const uint32_t a = seg_adj(q, lo);
+    if (a == seg_adj(q, hi))
+        return i + a;
+    return i + seg_adj(q, i);
+    }
+
+// First slot of a reservation of `total` entries in segment seg, or EMPTY
+// (and the flag set) when the segment is full.
+__device__ __forceinline__ uint32_t seg_reserve(const Queue& q, uint32_t seg, uint32_t total)
+    {
+    const uint32_t p = atomicAdd(q.count + seg * QSEG_STRIDE, total);
+    if ((uint64_t)p + total <= q.seg_cap)
+        return seg * q.seg_cap + p;
+    atomicOr(q.count + QSEG_FLAG, 1u);
+    return EMPTY;
+    }
+
 __device__ __forceinline__ uint32_t lane_id()
     {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -255,22 +313,23 @@ __device__ __forceinline__ uint32_t lane_id()
 
 // Append slot for lanes with pred set: one atomic per wavefront.  Every lane
 // of the wavefront must call it.
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+__device__ __forceinline__ uint32_t wave_append(bool pred, const Queue& q, uint32_t seg)
     {
     const uint64_t m = __ballot(pred);
     const uint32_t lane = lane_id();
-    uint32_t base = 0;
+    uint32_t base = EMPTY;
     if (lane == 0 && m)
-        base = atomicAdd(counter, (uint32_t)__popcll(m));
+        base = seg_reserve(q, seg, (uint32_t)__popcll(m));
     base = __shfl(base, 0, 64);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    return (base == EMPTY) ? EMPTY : base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     }
 
-// Append slot for lanes with pred set: one atomic per workgroup of BS
-// threads (wavefront counts combined in LDS).  Every thread of the
-// workgroup must call it; sh holds BS / 64 + 1 words.
+// Append slot in segment seg of q for lanes with pred set (EMPTY if the
+// segment is full): one atomic per workgroup of BS threads (wavefront counts
+// combined in LDS).  Every thread of the workgroup must call it; sh holds
+// BS / 64 + 1 words.
 template<int BS>
-__device__ __forceinline__ uint32_t block_append(bool pred, uint32_t* counter, uint32_t* sh)
+__device__ __forceinline__ uint32_t block_append(bool pred, const Queue& q, uint32_t seg, uint32_t* sh)
     {
     const uint64_t m = __ballot(pred);
     const uint32_t lane = lane_id();
@@ -287,10 +346,11 @@ __device__ __forceinline__ uint32_t block_append(bool pred, uint32_t* counter, u
             sh[w] = total;
             total += c;
             }
-        sh[BS / 64] = total ? atomicAdd(counter, total) : 0u;
+        sh[BS / 64] = total ? seg_reserve(q, seg, total) : EMPTY;
         }
     __syncthreads();
-    const uint32_t slot = sh[BS / 64] + sh[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const uint32_t b = sh[BS / 64];
+    const uint32_t slot = (b == EMPTY) ? EMPTY : b + sh[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     __syncthreads(); // sh is reused by the next call
     return slot;
     }
@@ -342,13 +402,6 @@ __device__ __forceinline__ uint64_t rec_index(const JobDev& J, uint32_t k, uint3
     return J.rec_base + ((uint64_t)k * J.npix + pixel) * J.L + ls;
     }
 
-struct Queue
-    {
-    float4* o;   // origin, tnear
-    float4* d;   // direction, job (bits)
-    float4* att; // attenuation
-    uint4* meta; // job, pixel, sample-in-batch | light sample << 16, depth
-    };
 
 // ---------------------------------------------------------------- traverse
 // Persistent threads: each wavefront takes 64 rays at a time from a counter
@@ -364,16 +417,16 @@ struct Queue
 __device__ __forceinline__ void primary_order(const JobDev& J, uint32_t local, uint32_t& k, uint32_t& px,
                                               uint32_t& py)
     {
-    const uint32_t chunk = local / (64u * J.K);
+    const uint32_t chunk = fdiv(local, J.div_chunk); // local / (64 K)
     const uint32_t rem = local - chunk * 64u * J.K;
     const uint32_t size = min(64u, J.npix - chunk * 64u);
-    k = rem / size;
+    k = (size == 64u) ? (rem >> 6) : rem / size;
     const uint32_t j = chunk * 64u + (rem - k * size);
     const uint32_t W8 = J.W & ~7u, H8 = J.H & ~7u;
     if (j < W8 * H8)
         {
         const uint32_t tile = j >> 6, within = j & 63u, tiles_x = W8 >> 3;
-        const uint32_t ty = tile / tiles_x;
+        const uint32_t ty = fdiv(tile, J.div_tiles); // tile / tiles_x
         px = (tile - ty * tiles_x) * 8u + (within & 7u);
         py = ty * 8u + (within >> 3);
         return;
@@ -398,7 +451,7 @@ __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uin
     job = find_job(jobs, n_jobs, i);
     const JobDev& J = jobs[job];
     primary_order(J, i - J.ray_base, k, px, py);
-    camera_ray(J.cam, J.W, J.H, J.xs_off, J.seed, px, py, J.n_base + k + 1, true, o, d);
+    camera_ray(J.cam, J.W, J.H, J.xs_off, J.inv_h, J.seed, px, py, J.n_base + k + 1, true, o, d);
     }
 
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
@@ -440,6 +493,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
         for (uint32_t sub = base; sub < end; sub += 64)
         {
         const uint32_t i = sub + lane;
+        const uint32_t sl = PRIMARY ? 0u : q_slot(q, i, sub, min(end, sub + 64u) - 1u);
         if (i < end)
             {
             v3 o, d;
@@ -453,8 +507,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
                 }
             else
                 {
-                const float4 a = q.o[i];
-                const float4 b = q.d[i];
+                const float4 a = q.o[sl];
+                const float4 b = q.d[sl];
                 o = mk3(a.x, a.y, a.z);
                 d = mk3(b.x, b.y, b.z);
                 tnear = a.w;
@@ -488,6 +542,18 @@ __device__ __forceinline__ void hit_attributes(const SceneDev& sc, const HitT& h
                                                const GeomDev*& g)
     {
     g = sc.geoms + h.geom;
+    if (TYPES == T_SPHERES || g->type == G_SPHERE)
+        {
+        // the search's t is the sphere solve's: no second solve
+        // (GeometrySphere.cc:81-117 attributes, unflipped Ng)
+        const float4 s = g->a[h.prim];
+        const v3 p = xyz(s);
+        const v3 w = cross(p - o, d);
+        at.Ng = mk3(o.x + h.t * d.x - p.x, o.y + h.t * d.y - p.y, o.z + h.t * d.z - p.z);
+        at.d = s.w - sqrtf(dot(w, w));
+        at.color = ld3(g->color, h.prim);
+        return;
+        }
     float t2;
     intersect_prim<true, TYPES>(*g, h.prim, o, d, tnear, t2, &at);
     }
@@ -501,8 +567,8 @@ __device__ __forceinline__ void hit_attributes(const SceneDev& sc, const HitT& h
 // thread of the workgroup must call it.
 template<int BS, uint32_t TYPES>
 __device__ __forceinline__ void shade_primary_block(const JobDev* __restrict__ jobs, bool valid, uint32_t job,
-                                                    uint32_t k, uint32_t pixel, v3 o, v3 d, const HitT& h, Queue out,
-                                                    uint32_t* out_count, uint4* records, uint32_t* sh)
+                                                    uint32_t k, uint32_t pixel, v3 o, v3 d, const HitT& h,
+                                                    const Queue& out, uint32_t seg, uint4* records, uint32_t* sh)
     {
     const float tnear = 1e-3f;
     uint32_t L = 0;
@@ -538,8 +604,8 @@ __device__ __forceinline__ void shade_primary_block(const JobDev* __restrict__ j
             if (!cont)
                 records[rec_index(J, k, pixel, ls)] = make_rec(addv ? add : mk3(0, 0, 0), addv ? REC_ADD : REC_NONE);
             }
-        const uint32_t slot = block_append<BS>(cont, out_count, sh);
-        if (cont)
+        const uint32_t slot = block_append<BS>(cont, out, seg, sh);
+        if (cont && slot != EMPTY)
             {
             out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
             out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(job));
@@ -552,7 +618,7 @@ __device__ __forceinline__ void shade_primary_block(const JobDev* __restrict__ j
 template<int BS, uint32_t TYPES>
 __global__ __launch_bounds__(BS) void k_shade_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs,
                                                       const uint4* __restrict__ hits, uint32_t n, Queue out,
-                                                      uint32_t* out_count, uint4* records)
+                                                      uint4* records)
     {
     __shared__ uint32_t sh[BS / 64 + 1];
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
@@ -570,7 +636,7 @@ __global__ __launch_bounds__(BS) void k_shade_primary(const JobDev* __restrict__
         h.geom = hh.y;
         h.prim = hh.z;
         }
-    shade_primary_block<BS, TYPES>(jobs, valid, job, k, pixel, o, d, h, out, out_count, records, sh);
+    shade_primary_block<BS, TYPES>(jobs, valid, job, k, pixel, o, d, h, out, blockIdx.x % QSEG, records, sh);
     }
 
 // Shade one hit or miss of a path at depth >= 1.  Returns true when the path
@@ -602,26 +668,27 @@ __device__ inline bool shade_bounce(const JobDev& J, const HitT& h, v3 o, v3 d, 
 template<int BS, uint32_t TYPES>
 __global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, Queue in,
                                               const uint4* __restrict__ hits, uint32_t n, Queue out,
-                                              uint32_t* out_count, uint4* records, uint32_t max_depth,
-                                              unsigned long long* counters)
+                                              uint4* records, uint32_t max_depth, unsigned long long* counters)
     {
     __shared__ uint32_t sh[BS / 64 + 1];
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
+    const uint32_t first = blockIdx.x * BS;
+    const uint32_t sl = q_slot(in, i, first, min(n, first + BS) - 1u);
     bool cont = false;
     v3 att, no, nd;
     uint4 mt = make_uint4(0, 0, 0, 0);
     if (i < n)
         {
-        mt = in.meta[i];
+        mt = in.meta[sl];
         const JobDev& J = jobs[mt.x];
         const uint4 hh = hits[i];
         HitT h;
         h.t = __uint_as_float(hh.x);
         h.geom = hh.y;
         h.prim = hh.z;
-        const float4 a = in.o[i];
-        const float4 b = in.d[i];
-        const float4 at4 = in.att[i];
+        const float4 a = in.o[sl];
+        const float4 b = in.d[sl];
+        const float4 at4 = in.att[sl];
         att = mk3(at4.x, at4.y, at4.z);
         const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
         uint4 rec;
@@ -637,8 +704,8 @@ __global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, Q
         if (!cont)
             records[rec_index(J, k, mt.y, ls)] = rec;
         }
-    const uint32_t slot = block_append<BS>(cont, out_count, sh);
-    if (cont)
+    const uint32_t slot = block_append<BS>(cont, out, blockIdx.x % QSEG, sh);
+    if (cont && slot != EMPTY)
         {
         out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
         out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
@@ -983,8 +1050,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t n,
                                                    uint32_t* work, uint32_t groups, uint2* gstack, uint32_t gstride,
-                                                   Queue out, uint32_t* out_count, uint4* records,
-                                                   unsigned long long* counters)
+                                                   Queue out, uint4* records, unsigned long long* counters)
     {
     __shared__ uint2 lds[SS * BLOCK];
     __shared__ uint32_t sh[BLOCK / 64 + 1];
@@ -1009,6 +1075,7 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
         {
         if (threadIdx.x == 0)
             s_claim = atomicAdd(work, chunk); // the next chunk, claimed ahead
+        const uint32_t seg = (base / chunk) % QSEG; // the chunk's output segment
         for (uint32_t g = 0; g < groups; g++)
             {
             const uint32_t i = base + (wave * groups + g) * 64 + lane;
@@ -1025,8 +1092,8 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
                                                 err);
             if (COUNT && valid)
                 n_rays++;
-            shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, o, d, best, out,
-                                              out_count, records, sh);
+            shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, o, d, best, out, seg,
+                                              records, sh);
             }
         __syncthreads();
         base = s_claim;
@@ -1051,7 +1118,7 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ jobs, Queue in, uint32_t n,
                                                   uint32_t* work, uint32_t chunk, uint2* gstack, uint32_t gstride,
-                                                  Queue out, uint32_t* out_count, uint4* records, uint32_t max_depth,
+                                                  Queue out, uint4* records, uint32_t max_depth,
                                                   unsigned long long* counters)
     {
     __shared__ uint2 lds[SS * BLOCK];
@@ -1074,19 +1141,21 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ job
         if (lane == 0)
             next = atomicAdd(work, chunk);
         const uint32_t end = min(n, base + chunk);
+        const uint32_t seg = (base / chunk) % QSEG; // the chunk's output segment
         for (uint32_t sub = base; sub < end; sub += 64)
             {
             const uint32_t i = sub + lane;
+            const uint32_t sl = q_slot(in, i, sub, min(end, sub + 64u) - 1u);
             bool cont = false;
             v3 att, no, nd;
             uint4 mt = make_uint4(0, 0, 0, 0);
             if (i < end)
                 {
-                mt = in.meta[i];
+                mt = in.meta[sl];
                 const JobDev& J = jobs[mt.x];
-                const float4 a = in.o[i];
-                const float4 b = in.d[i];
-                const float4 at4 = in.att[i];
+                const float4 a = in.o[sl];
+                const float4 b = in.d[sl];
+                const float4 at4 = in.att[sl];
                 const v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z);
                 HitT h;
                 trace_closest<LdsStack<SS>, COUNT, TYPES>(J.scene, o, d, a.w, h, st, n_nodes, n_prims, err);
@@ -1106,8 +1175,8 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ job
                 if (!cont)
                     records[rec_index(J, k, mt.y, ls)] = rec;
                 }
-            const uint32_t slot = wave_append(cont, out_count);
-            if (cont)
+            const uint32_t slot = wave_append(cont, out, seg);
+            if (cont && slot != EMPTY)
                 {
                 out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
                 out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
@@ -1184,8 +1253,9 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
                     }
                 else
                     {
-                    const float4 a = q.o[i];
-                    const float4 b = q.d[i];
+                    const uint32_t sl = q_slot(q, i, pool, pool_end - 1u);
+                    const float4 a = q.o[sl];
+                    const float4 b = q.d[sl];
                     o = mk3(a.x, a.y, a.z);
                     d = mk3(b.x, b.y, b.z);
                     tnear = a.w;
@@ -1243,11 +1313,13 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
     st.glob = gstack + i;
     st.gstride = gstride;
     st.sp = 0;
-    const uint4 mt = in.meta[i];
+    const uint32_t first = blockIdx.x * BLOCK;
+    const uint32_t sl = q_slot(in, i, first, min(n, first + BLOCK) - 1u);
+    const uint4 mt = in.meta[sl];
     const JobDev& J = jobs[mt.x];
-    const float4 a = in.o[i];
-    const float4 b = in.d[i];
-    const float4 at4 = in.att[i];
+    const float4 a = in.o[sl];
+    const float4 b = in.d[sl];
+    const float4 at4 = in.att[sl];
     v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
     const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
     uint32_t depth = mt.w;
@@ -1362,7 +1434,7 @@ __global__ void k_preview(const JobDev* __restrict__ jobp, uint32_t aa_n)
     for (uint32_t sample = 0; sample < aa_n * aa_n; sample++)
         {
         v3 o, d;
-        camera_ray(J.cam, J.W, J.H, J.xs_off, J.seed, px, py, sample, aa_n != 1, o, d);
+        camera_ray(J.cam, J.W, J.H, J.xs_off, J.inv_h, J.seed, px, py, sample, aa_n != 1, o, d);
         HitT h;
         trace_closest<LocalStack, false>(J.scene, o, d, 0.0f, h, st, dummy, dummy, err);
         v3 c = J.bg;
@@ -1448,7 +1520,7 @@ __global__ void k_debug_primary(const JobDev* __restrict__ jobp, uint32_t sample
         return;
     const uint32_t py = pixel / J.W, px = pixel - py * J.W;
     v3 o, d;
-    camera_ray(J.cam, J.W, J.H, J.xs_off, J.seed, px, py, sample, true, o, d);
+    camera_ray(J.cam, J.W, J.H, J.xs_off, J.inv_h, J.seed, px, py, sample, true, o, d);
     LocalStack st;
     uint32_t dummy = 0, err = 0;
     HitT h;

@@ -134,6 +134,35 @@ FH_HD float smax(float a, float b)
     return (a < b) ? b : a;
     }
 
+// ------------------------------------------------------------ division by a constant
+// n / d for a divisor d >= 1 fixed per image, by a multiply-high and two
+// shifts: the round-up method of Granlund and Montgomery ("Division by
+// invariant integers using multiplication", PLDI 1994, section 4) with
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1, exact for every
+// 32-bit n.
+struct FastDiv
+    {
+    uint32_t m, s1, s2;
+    };
+
+inline FastDiv make_fastdiv(uint32_t d)
+    {
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d)
+        l++;
+    FastDiv f;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    f.s1 = l ? 1u : 0u;
+    f.s2 = l ? l - 1 : 0u;
+    return f;
+    }
+
+FH_HD uint32_t fdiv(uint32_t n, FastDiv f)
+    {
+    const uint32_t t = (uint32_t)(((uint64_t)f.m * n) >> 32);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+    }
+
 // ------------------------------------------------------------ Philox4x32-10
 // Counter-based RNG of Salmon, Moraes, Dror and Shaw (SC'11), 10 rounds,
 // multipliers 0xD2511F53 / 0xCD9E8D57 and Weyl increments 0x9E3779B9 /
@@ -142,6 +171,16 @@ struct u4
     {
     uint32_t v[4];
     };
+
+// a ^ b ^ c; one v_bitop3_b32 (truth table 0x96) on gfx950
+FH_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+    }
 
 FH_HD u4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
     {
@@ -155,8 +194,8 @@ FH_HD u4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
         const uint32_t lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32);
         const uint32_t lo1 = (uint32_t)p1;
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        const uint32_t n0 = xor3(hi1, c1, k0);
+        const uint32_t n2 = xor3(hi0, c3, k1);
         c0 = n0;
         c1 = lo1;
         c2 = n2;

@@ -95,6 +95,9 @@ struct CamDev
     v3 p, u, v, w;
     float focal_h, focal_d, a;
     int32_t model;
+    // perspective with aperture 0 and no component of p equal to -0: the lens
+    // offset is a signed zero and every ray origin is exactly p
+    int32_t pinhole;
     };
 
 // Lights in scene space plus per-light constants evaluated once on the host:
@@ -128,7 +131,10 @@ struct JobDev
     uint32_t highlight;
     v3 hw_color;
     uint32_t ray_base;    // first primary ray of this job in the batch
+    FastDiv div_chunk;    // division by 64 K (primary rays per 64-pixel chunk)
+    FastDiv div_tiles;    // division by (W / 8) (full 8x8 tiles per row)
     uint64_t rec_base;    // first termination record of this job
+    double inv_h;         // 1 / H
     float4* linear;       // running mean (h, w) RGBA
     uchar4* srgb;         // sRGB output (h, w) RGBA
     // per-tile candidate lists of the primary rays (kernels/tiles.hip), used

@@ -204,9 +204,17 @@ __device__ __forceinline__ float pdf_diffuse(v3 l, v3 n)
 // ---------------------------------------------------------------- camera
 // Camera::importanceSampleAA + generateRay, fresnel/common/Camera.h:169-269
 // xs_off = 0.5f * float(W) / float(H), evaluated once per image by the host
-// (the same IEEE float operations, so the same value).
-__device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, float xs_off, uint32_t seed, uint32_t i,
-                                  uint32_t j, uint32_t sample, bool aa, v3& o, v3& d)
+// (the same IEEE float operations, so the same value); inv_h = 1 / H in double.
+//
+// x / H as (float)(x * inv_h) in double is the correctly rounded float
+// quotient for H <= 2^20: with x = A 2^j (2^23 <= |A| < 2^24) a float
+// midpoint M 2^k (M odd, 25 bits) near x / H satisfies x - H M 2^k != 0 (the
+// odd part of H M has 25 or more bits, that of A fewer), so
+// |x / H - M 2^k| >= 2^k / H and, as j - k <= log2 H + 2, the quotient lies at
+// least 2^-46 (relative) from every midpoint, while the double product errs by
+// at most 2^-52.  Larger H take the float division.
+__device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, float xs_off, double inv_h, uint32_t seed,
+                                  uint32_t i, uint32_t j, uint32_t sample, bool aa, v3& o, v3& d)
     {
     const uint32_t key0 = j * W + i;
     float i_f, j_f;
@@ -215,8 +223,12 @@ __device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, float
         const u4 r = philox(0, 0, sample, RNG_AA, key0, seed);
         const float r1 = u01(r.v[0]) * 2.0f;
         const float r2 = u01(r.v[1]) * 2.0f;
-        const float dx = (r1 < 1.0f) ? (sqrtf(r1) - 1.0f) : (1.0f - sqrtf(2.0f - r1));
-        const float dy = (r2 < 1.0f) ? (sqrtf(r2) - 1.0f) : (1.0f - sqrtf(2.0f - r2));
+        // tent filter: r < 1 ? sqrt(r) - 1 : 1 - sqrt(2 - r), one square root
+        const bool low1 = r1 < 1.0f, low2 = r2 < 1.0f;
+        const float s1 = sqrtf(low1 ? r1 : 2.0f - r1);
+        const float s2 = sqrtf(low2 ? r2 : 2.0f - r2);
+        const float dx = low1 ? (s1 - 1.0f) : (1.0f - s1);
+        const float dy = low2 ? (s2 - 1.0f) : (1.0f - s2);
         const float aa_w = 0.707106781f;
         i_f = float(i) + 0.5f + dx * aa_w;
         j_f = float(j) + 0.5f + dy * aa_w;
@@ -226,8 +238,19 @@ __device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, float
         i_f = float(i) + 0.5f;
         j_f = float(j) + 0.5f;
         }
-    const float ys = -1.0f * (j_f / float(H) - 0.5f);
-    const float xs = i_f / float(H) - xs_off;
+    float i_h, j_h;
+    if (H <= (1u << 20))
+        {
+        i_h = (float)((double)i_f * inv_h);
+        j_h = (float)((double)j_f * inv_h);
+        }
+    else
+        {
+        i_h = i_f / float(H);
+        j_h = j_f / float(H);
+        }
+    const float ys = -1.0f * (j_h - 0.5f);
+    const float xs = i_h - xs_off;
     if (c.model == 0)
         {
         o = c.p + (ys * c.v + xs * c.u) * c.focal_h;
@@ -236,11 +259,16 @@ __device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, float
         }
     const v3 F = c.p - c.w * c.focal_d;
     const v3 C = F + (ys * c.v + xs * c.u) * c.focal_h;
-    const u4 r = philox(0, 0, sample, RNG_APERTURE, key0, seed);
-    const float theta = u01(r.v[0]) * 2.0f * PI_F;
-    const float rr = u01(r.v[1]) * c.a * 0.5f;
-    const v3 offset = rr * cosf(theta) * c.u + rr * sinf(theta) * c.v;
-    o = c.p + offset;
+    if (c.pinhole)
+        o = c.p; // the lens offset below is (+-0, +-0, +-0)
+    else
+        {
+        const u4 r = philox(0, 0, sample, RNG_APERTURE, key0, seed);
+        const float theta = u01(r.v[0]) * 2.0f * PI_F;
+        const float rr = u01(r.v[1]) * c.a * 0.5f;
+        const v3 offset = rr * cosf(theta) * c.u + rr * sinf(theta) * c.v;
+        o = c.p + offset;
+        }
     d = C - o;
     d = d * (1.0f / sqrtf(dot(d, d)));
     }

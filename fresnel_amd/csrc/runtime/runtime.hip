@@ -138,7 +138,8 @@ struct RenderScratch
     DBuf<uint4> qmeta[2];
     DBuf<uint4> hits;
     DBuf<uint4> records;
-    DBuf<uint32_t> counters32; // [0] traverse work, [1] queue A count, [2] queue B count
+    DBuf<uint32_t> counters32; // [0] traverse work
+    DBuf<uint32_t> qseg;       // append counters of the queue being filled (QSEG_STRIDE apart) and flag
     DBuf<uint2> gstack;        // traversal stack overflow (persistent grid)
     DBuf<uint2> tail_stack;    // stack overflow of the tail kernel
     DBuf<JobDev> jobs;
@@ -156,6 +157,7 @@ struct DeviceObj
     RenderScratch rs;
     DBuf<unsigned long long> counters;
     uint32_t* h_pinned = nullptr; // 16 uint32 of pinned host memory for counts
+    uint32_t* h_counts = nullptr; // pinned copy of a queue's segment counters and flag
     bool stats_on = false;
     bool count_on = false;
     uint64_t batch_paths = 64ull << 20; // light-sample paths per batch (sizes the queues)
@@ -342,7 +344,6 @@ struct PrimaryArgs
     uint2* gstack;
     uint32_t gstride;
     Queue out;
-    uint32_t* out_count;
     uint4* records;
     unsigned long long* counters;
     };
@@ -357,7 +358,7 @@ static void launch_primary(DeviceObj& D, hipStream_t s, const PrimaryArgs& a)
              [&]
              {
                  kern<<<grid, PRIMARY_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.n, a.work, a.groups, a.gstack, a.gstride,
-                                                     a.out, a.out_count, a.records, a.counters);
+                                                     a.out, a.records, a.counters);
              });
     }
 
@@ -391,7 +392,6 @@ struct BounceArgs
     uint2* gstack;
     uint32_t gstride;
     Queue out;
-    uint32_t* out_count;
     uint4* records;
     unsigned long long* counters;
     };
@@ -406,7 +406,7 @@ static void launch_bounce(DeviceObj& D, hipStream_t s, const BounceArgs& a)
              [&]
              {
                  kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.in, a.n, a.work, a.chunk, a.gstack, a.gstride, a.out,
-                                                 a.out_count, a.records, MAX_DEPTH, a.counters);
+                                                 a.records, MAX_DEPTH, a.counters);
              });
     }
 
@@ -957,6 +957,10 @@ static CamDev make_cam(const fh_user_camera& u)
     c.a = u.f / u.f_stop;
     c.model = u.model;
     c.focal_h = (u.model == 0) ? u.h : u.h / u.f * c.focal_d;
+    // p + (+-0) == p except for p = -0 and an offset of +0
+    const bool neg_zero = (c.p.x == 0.0f && std::signbit(c.p.x)) || (c.p.y == 0.0f && std::signbit(c.p.y))
+                          || (c.p.z == 0.0f && std::signbit(c.p.z));
+    c.pinhole = (c.model == 1 && c.a == 0.0f && !neg_zero) ? 1 : 0;
     return c;
     }
 
@@ -1085,6 +1089,15 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
     c.tiles_x = tr.W / TILE;
     c.tiles_y = tr.H / TILE;
     c.margin = sc.cull_margin;
+    // orthographic ray origins lie on the image plane within this distance of p
+    // (screen coordinates |xs| <= W / 2H + 2 / H, |ys| <= 1/2 + 2 / H)
+    c.plane_r = 0.0f;
+    if (cam.model == 0)
+        {
+        const double xs = 0.5 * tr.W / tr.H + 2.0 / tr.H, ys = 0.5 + 2.0 / tr.H;
+        c.plane_r = (float)(cam.focal_h * std::sqrt(xs * xs + ys * ys) * (1.0 + 0x1p-10));
+        }
+    const bool spheres = scene_types(sc) == T_SPHERES;
     const uint32_t n_tiles = c.tiles_x * c.tiles_y;
     T.counts.ensure((size_t)N + 1);
     T.flags.ensure(1);
@@ -1093,7 +1106,12 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
              [&]
              {
                  k_fill_u32<<<1, 64, 0, s>>>(T.flags.p, 1, 0u);
-                 k_tile_count<<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, c, T.counts.p, T.flags.p);
+                 if (spheres)
+                     k_tile_count<true><<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
+                                                                           T.counts.p, T.flags.p);
+                 else
+                     k_tile_count<false><<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
+                                                                            T.counts.p, T.flags.p);
                  scan_exclusive(D, s, T.counts.p, N + 1, T.scan_tmp);
              });
     HIPCHECK(hipMemcpyAsync(&h[0], T.counts.p + N, 4, hipMemcpyDeviceToHost, s));
@@ -1112,8 +1130,14 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
         D.launch("tiles_emit",
                  [&]
                  {
-                     k_tile_emit<<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, c, T.counts.p, T.tile0.p,
-                                                                T.q0.p, T.ref0.p, T.valsA.p);
+                     if (spheres)
+                         k_tile_emit<true><<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
+                                                                          T.counts.p, T.tile0.p, T.q0.p, T.ref0.p,
+                                                                          T.valsA.p);
+                     else
+                         k_tile_emit<false><<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
+                                                                           T.counts.p, T.tile0.p, T.q0.p, T.ref0.p,
+                                                                           T.valsA.p);
                      HIPCHECK(hipMemcpyAsync(T.keysA.p, T.q0.p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
                  });
         // by bound (4 passes: the result is back in keysA/valsA), then stably by tile
@@ -1130,7 +1154,7 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
         D.launch("tiles_bounds",
                  [&]
                  {
-                     if (scene_types(sc) == T_SPHERES)
+                     if (spheres)
                          k_tile_pack<true><<<blocks(M, 256), 256, 0, s>>>(vt, T.ref0.p, T.q0.p, sc.blo.p, sc.bhi.p,
                                                                           sc.dgeoms.p, M, T.entries.p);
                      else
@@ -1167,6 +1191,7 @@ static JobDev make_job(SceneObj& sc, TracerObj& tr, const fh_user_camera& cam_us
     J.H = tr.H;
     J.npix = tr.W * tr.H;
     J.xs_off = 0.5f * float(tr.W) / float(tr.H); // as fresnel/common/Camera.h:267 evaluates it per ray
+    J.inv_h = 1.0 / (double)tr.H;
     J.seed = tr.seed;
     J.n_base = tr.n_samples;
     J.L = tr.L;
@@ -1200,11 +1225,52 @@ static uint32_t scene_types(const SceneObj& sc)
     return m;
     }
 
-static void read_count(DeviceObj& D, const uint32_t* dptr, uint32_t& out)
+// Counts of the QSEG segments a producer filled (counters QSEG_STRIDE words
+// apart, then the overflow flag): sets the consumer view of q (pre, adj) and
+// returns the number of entries.
+static uint32_t read_segments(DeviceObj& D, const uint32_t* dcount, Queue& q)
     {
-    HIPCHECK(hipMemcpyAsync(D.h_pinned, dptr, sizeof(uint32_t), hipMemcpyDeviceToHost, D.stream));
+    const size_t words = QSEG_FLAG + 1;
+    HIPCHECK(hipMemcpyAsync(D.h_counts, dcount, words * sizeof(uint32_t), hipMemcpyDeviceToHost, D.stream));
     HIPCHECK(hipStreamSynchronize(D.stream));
-    out = D.h_pinned[0];
+    REQUIRE(D.h_counts[QSEG_FLAG] == 0, "internal error: a ray queue segment overflowed");
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < QSEG; k++)
+        {
+        const uint32_t c = D.h_counts[k * QSEG_STRIDE];
+        REQUIRE(c <= q.seg_cap, "internal error: ray queue segment count exceeds its capacity");
+        q.pre[k] = (uint32_t)total;
+        q.adj[k] = k * q.seg_cap - (uint32_t)total;
+        total += c;
+        }
+    return (uint32_t)total;
+    }
+
+// Slots per queue segment for a batch: every producer appends at most
+// `per_unit` entries per unit, and segment s receives the units u with
+// u % QSEG == s.
+static uint32_t segment_capacity(uint64_t n_pri, uint64_t n_paths, uint32_t max_l, uint32_t primary_chunk,
+                                 uint32_t trav_chunk)
+    {
+    auto bound = [](uint64_t n, uint64_t unit, uint64_t per_unit)
+    {
+        const uint64_t units = (n + unit - 1) / unit;
+        return (units + QSEG - 1) / QSEG * per_unit;
+    };
+    uint64_t c = 1;
+    c = std::max(c, bound(n_pri, primary_chunk, (uint64_t)primary_chunk * max_l));  // k_primary
+    c = std::max(c, bound(n_pri, SHADE_BLOCK, (uint64_t)SHADE_BLOCK * max_l));      // k_shade_primary
+    c = std::max(c, bound(n_paths, SHADE_BLOCK, SHADE_BLOCK));                       // k_shade
+    c = std::max(c, bound(n_paths, trav_chunk, trav_chunk));                         // k_bounce
+    REQUIRE(c * QSEG < (1ull << 32), "batch too large");
+    return (uint32_t)c;
+    }
+
+// Divisors of the primary-ray order (primary_order) once K is final.
+static void set_order_divisors(JobDev& J)
+    {
+    J.div_chunk = make_fastdiv(64u * J.K);
+    J.div_tiles = make_fastdiv(std::max(1u, J.W >> 3));
     }
 
 // Run one batch: every job advances by jobs[j].K camera samples.
@@ -1216,6 +1282,7 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
     std::vector<uint32_t> pix_base;
     for (auto& J : jobs)
         {
+        set_order_divisors(J);
         J.ray_base = (uint32_t)n_pri;
         J.rec_base = n_paths;
         pix_base.push_back((uint32_t)n_pix);
@@ -1224,7 +1291,13 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
         n_pix += J.npix;
         }
     REQUIRE(n_pri < (1ull << 31) && n_paths < (1ull << 31), "batch too large");
-    const uint64_t cap = n_paths; // primary rays are generated from their index, only hits are stored
+    // primary rays are generated from their index; the queues hold paths
+    uint32_t max_l = 1;
+    for (const auto& J : jobs)
+        max_l = std::max(max_l, J.L);
+    const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
+    const uint32_t seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * groups, D.trav_chunk);
+    const uint64_t cap = (uint64_t)QSEG * seg_cap;
     R.hits.ensure(std::max(n_pri, n_paths));
     for (int q = 0; q < 2; q++)
         {
@@ -1235,6 +1308,7 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
         }
     R.records.ensure(n_paths);
     R.counters32.ensure(4);
+    R.qseg.ensure(QSEG_FLAG + 1);
     R.jobs.ensure(jobs.size());
     R.pix_base.ensure(jobs.size());
     HIPCHECK(hipMemcpyAsync(R.jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, s));
@@ -1244,11 +1318,15 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
     R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * D.tail_rays);
     Queue Q[2];
     for (int q = 0; q < 2; q++)
+        {
         Q[q] = Queue {R.qo[q].p, R.qd[q].p, R.qatt[q].p, R.qmeta[q].p};
+        Q[q].count = R.qseg.p;
+        Q[q].seg_cap = seg_cap;
+        }
     const uint32_t nj = (uint32_t)jobs.size();
     uint32_t* work = R.counters32.p;
-    uint32_t* qcount = R.counters32.p + 1;
     unsigned long long* ctr = D.counters.p;
+    auto reset_segments = [&] { HIPCHECK(hipMemsetAsync(R.qseg.p, 0, (QSEG_FLAG + 1) * sizeof(uint32_t), s)); };
 
     const bool spheres = (types == T_SPHERES);
     auto traverse = [&](const Queue& q, uint32_t n, bool primary)
@@ -1260,25 +1338,24 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
 
     if (D.fuse_primary == 2)
         {
-        HIPCHECK(hipMemsetAsync(work, 0, 2 * sizeof(uint32_t), s)); // work and qcount
-        PrimaryArgs a {R.jobs.p, nj, (uint32_t)n_pri, work, std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK),
-                       R.gstack.p, gstride, Q[1], qcount, R.records.p, ctr};
+        HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
+        reset_segments();
+        PrimaryArgs a {R.jobs.p, nj, (uint32_t)n_pri, work, groups, R.gstack.p, gstride, Q[1], R.records.p, ctr};
         dispatch_primary(D, s, a, spheres);
         }
     else
         {
         traverse(Q[0], (uint32_t)n_pri, true);
-        HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
+        reset_segments();
         D.launch("shade_primary",
                  [&]
                  {
                      auto* f = spheres ? k_shade_primary<SHADE_BLOCK, T_SPHERES> : k_shade_primary<SHADE_BLOCK, T_ALL>;
                      f<<<blocks(n_pri, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri,
-                                                                          Q[1], qcount, R.records.p);
+                                                                          Q[1], R.records.p);
                  });
         }
-    uint32_t n = 0;
-    read_count(D, qcount, n);
+    uint32_t n = read_segments(D, R.qseg.p, Q[1]);
     int cur = 1;
     while (n > 0)
         {
@@ -1299,24 +1376,25 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
             }
         if (D.fuse_bounce == 2)
             {
-            HIPCHECK(hipMemsetAsync(work, 0, 2 * sizeof(uint32_t), s)); // work and qcount
-            BounceArgs a {R.jobs.p, Q[cur], n, work, D.trav_chunk, R.gstack.p, gstride, Q[1 - cur], qcount,
-                          R.records.p, ctr};
+            HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
+            reset_segments();
+            BounceArgs a {R.jobs.p, Q[cur], n, work, D.trav_chunk, R.gstack.p, gstride, Q[1 - cur], R.records.p,
+                          ctr};
             dispatch_bounce(D, s, a, spheres);
             }
         else
             {
             traverse(Q[cur], n, false);
-            HIPCHECK(hipMemsetAsync(qcount, 0, sizeof(uint32_t), s));
+            reset_segments();
             D.launch("shade",
                      [&]
                      {
                          auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
                          f<<<blocks(n, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, n, Q[1 - cur],
-                                                                          qcount, R.records.p, MAX_DEPTH, ctr);
+                                                                          R.records.p, MAX_DEPTH, ctr);
                      });
             }
-        read_count(D, qcount, n);
+        n = read_segments(D, R.qseg.p, Q[1 - cur]);
         cur = 1 - cur;
         }
     D.launch("accumulate",
@@ -1647,6 +1725,9 @@ extern "C"
                 void* hp = nullptr;
                 HIPCHECK(hipHostMalloc(&hp, 64, hipHostMallocDefault));
                 D.h_pinned = (uint32_t*)hp;
+                hp = nullptr;
+                HIPCHECK(hipHostMalloc(&hp, (QSEG_FLAG + 1) * sizeof(uint32_t), hipHostMallocDefault));
+                D.h_counts = (uint32_t*)hp;
                 D.counters.ensure(N_CTR);
                 HIPCHECK(hipMemset(D.counters.p, 0, N_CTR * sizeof(unsigned long long)));
                 *out = d.release();
@@ -1670,6 +1751,8 @@ extern "C"
                     (void)hipEventDestroy(e);
                 if (dev->obj.h_pinned)
                     (void)hipHostFree(dev->obj.h_pinned);
+                if (dev->obj.h_counts)
+                    (void)hipHostFree(dev->obj.h_counts);
                 hipStream_t s = dev->obj.stream;
                 delete dev;
                 if (s)
@@ -2841,6 +2924,7 @@ extern "C"
                 JobDev J = make_job(sc, T, sc.camera);
                 J.K = 1;
                 J.n_base = sample - 1;
+                set_order_divisors(J);
                 const uint32_t npix = J.npix;
                 RenderScratch& R = D.rs;
                 R.jobs.ensure(1);
