@@ -413,6 +413,98 @@ __global__ void k_refit(const uint32_t* __restrict__ vals, const float4* __restr
         par = parent[node];
         }
     }
+// ---------------------------------------------------------------- 4-wide hierarchy
+// The binary hierarchy's inner nodes at even depth become the nodes of a
+// 4-wide hierarchy whose children are their grandchildren (a child that is a
+// leaf stays a child), so a search makes half as many dependent node fetches.
+// The binary tree is the one built and refit; the 4-wide nodes copy its boxes
+// (k_bvh4_fill after every build and refit, the topology from k_depth_even and
+// a scan of its flags).
+
+// even[i] = 1 for inner nodes at even depth (the root has depth 0); even[n] = 0.
+__global__ void k_depth_even(const uint32_t* __restrict__ parent, uint32_t n, uint32_t* even)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n)
+        return;
+    if (i == n)
+        {
+        even[n] = 0;
+        return;
+        }
+    uint32_t d = 0;
+    for (uint32_t k = i; k != 0; k = parent[k] >> 1)
+        d++;
+    even[i] = (d & 1u) ? 0u : 1u;
+    }
+
+__device__ __forceinline__ void child_box(const Node& n, uint32_t side, float b[6])
+    {
+    if (side == 0)
+        {
+        b[0] = n.a_lo.x; b[1] = n.a_lo.y; b[2] = n.a_lo.z;
+        b[3] = n.a_lo.w; b[4] = n.a_hi.x; b[5] = n.a_hi.y;
+        }
+    else
+        {
+        b[0] = n.a_hi.z; b[1] = n.a_hi.w; b[2] = n.b_box.x;
+        b[3] = n.b_box.y; b[4] = n.b_box.z; b[5] = n.b_box.w;
+        }
+    }
+
+// 4-wide node idx4[i] from inner node i of n, for the nodes at even depth:
+// idx4 is the exclusive scan of k_depth_even's flags (n + 1 entries).
+__global__ void k_bvh4_fill(const Node* __restrict__ nodes, uint32_t n, const uint32_t* __restrict__ idx4,
+                            Node4* __restrict__ out)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || idx4[i + 1] == idx4[i])
+        return;
+    float box[4][6];
+    uint32_t ref[4], slot[4];
+    uint32_t m = 0;
+    const Node a = nodes[i];
+    const uint32_t la[2] = {a.link.x, a.link.y}, ga[2] = {a.link.z, a.link.w};
+    for (uint32_t s = 0; s < 2; s++)
+        {
+        if (la[s] & LEAF) // a leaf or EMPTY (its NaN box rejects every ray)
+            {
+            child_box(a, s, box[m]);
+            ref[m] = la[s];
+            slot[m] = ga[s];
+            m++;
+            continue;
+            }
+        const Node b = nodes[la[s]];
+        const uint32_t lb[2] = {b.link.x, b.link.y}, gb[2] = {b.link.z, b.link.w};
+        for (uint32_t t = 0; t < 2; t++)
+            {
+            child_box(b, t, box[m]);
+            ref[m] = (lb[t] & LEAF) ? lb[t] : idx4[lb[t]];
+            slot[m] = gb[t];
+            m++;
+            }
+        }
+    const float nan = __builtin_nanf("");
+    for (; m < 4; m++)
+        {
+        for (int k = 0; k < 6; k++)
+            box[m][k] = nan;
+        ref[m] = EMPTY;
+        slot[m] = 0;
+        }
+    Node4 r;
+    r.lox = make_float4(box[0][0], box[1][0], box[2][0], box[3][0]);
+    r.loy = make_float4(box[0][1], box[1][1], box[2][1], box[3][1]);
+    r.loz = make_float4(box[0][2], box[1][2], box[2][2], box[3][2]);
+    r.hix = make_float4(box[0][3], box[1][3], box[2][3], box[3][3]);
+    r.hiy = make_float4(box[0][4], box[1][4], box[2][4], box[3][4]);
+    r.hiz = make_float4(box[0][5], box[1][5], box[2][5], box[3][5]);
+    r.child = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+    r.slot = make_uint4(slot[0], slot[1], slot[2], slot[3]);
+    out[idx4[i]] = r;
+    }
+
 // ---------------------------------------------------------------- quality
 // Surface-area cost of a hierarchy: the sum over internal nodes of their two
 // child boxes' half surface areas.  A random ray visits a node with

@@ -16,7 +16,9 @@
 
 namespace fh
 {
-constexpr uint32_t STACK_MAX = 64; // >= depth of a Karras tree over 64-bit keys
+// >= 3 pushes per level of a 4-wide tree of depth 32 (half the depth of a
+// Karras tree over 64-bit keys; the binary packet search pushes one per level)
+constexpr uint32_t STACK_MAX = 96;
 
 struct HitT
     {
@@ -120,6 +122,7 @@ __device__ __forceinline__ bool leaf_test(const GeomDev& g, uint32_t prim, v3 o,
     }
 
 typedef const Node __attribute__((address_space(1))) GlobalNode;
+typedef const Node4 __attribute__((address_space(1))) GlobalNode4;
 
 // Closest-hit traversal state of one ray.
 struct TraceState
@@ -146,79 +149,103 @@ __device__ __forceinline__ void trace_begin(const SceneDev& sc, v3 o, v3 d, floa
     st.sp = 0;
     }
 
-// One node visit of the closest-hit search: the primitives of leaf children
-// are tested, the nearer hit inner child is visited next and the farther one
-// pushed.  Returns false when the search is complete.  The closest hit is
-// the smallest (t, geometry, primitive) over the primitives whose bounds pass
-// the slab test with tmax >= tnear; a subtree is culled when the lower bound
-// of its primitives' distances (cull_bound) exceeds the closest hit.
+// Primitive test of a leaf child that passed the slab and cull tests.
+template<bool COUNT, uint32_t TYPES>
+__device__ __forceinline__ void visit_leaf(const GeomDev* __restrict__ geoms, uint32_t ref, uint32_t slot,
+                                           TraceState& r, uint32_t& n_prims)
+    {
+    const uint32_t prim = ref & ~LEAF;
+    float t;
+    if (COUNT)
+        n_prims++;
+    if (leaf_test<TYPES>(geoms[slot], prim, r.o, r.d, r.tnear, t) && better(t, slot, prim, r.best))
+        {
+        r.best.t = t;
+        r.best.geom = slot;
+        r.best.prim = prim;
+        }
+    }
+
+// Sort two (key, node, bound) entries by key.
+__device__ __forceinline__ void order2(float& ka, uint32_t& na, float& ca, float& kb, uint32_t& nb, float& cb)
+    {
+    const bool sw = kb < ka;
+    const float k = sw ? kb : ka, c = sw ? cb : ca;
+    const uint32_t n = sw ? nb : na;
+    kb = sw ? ka : kb;
+    nb = sw ? na : nb;
+    cb = sw ? ca : cb;
+    ka = k;
+    na = n;
+    ca = c;
+    }
+
+// One node visit of the closest-hit search over the 4-wide hierarchy: the
+// primitives of leaf children are tested, the nearest hit inner child is
+// visited next and the others are pushed, farthest first.  Returns false when
+// the search is complete.  The closest hit is the smallest (t, geometry,
+// primitive) over the primitives whose bounds pass the slab test with
+// tmax >= tnear; a subtree is culled when the lower bound of its primitives'
+// distances (cull_bound) exceeds the closest hit, so neither the hierarchy
+// nor the visiting order changes the result.
 template<class Stack, bool COUNT, uint32_t TYPES>
-__device__ __forceinline__ bool trace_visit(const Node* __restrict__ nodes, const GeomDev* __restrict__ geoms,
+__device__ __forceinline__ bool trace_visit(const Node4* __restrict__ nodes, const GeomDev* __restrict__ geoms,
                                             TraceState& r, Stack& st, uint32_t& n_nodes, uint32_t& n_prims,
                                             uint32_t& err)
     {
     if (r.node == EMPTY)
         return false;
     // global address space: direct global loads instead of flat ones
-    const GlobalNode* np = (const GlobalNode*)(nodes + r.node);
-    const float4 A = make_float4(np->a_lo.x, np->a_lo.y, np->a_lo.z, np->a_lo.w);
-    const float4 B = make_float4(np->a_hi.x, np->a_hi.y, np->a_hi.z, np->a_hi.w);
-    const float4 C = make_float4(np->b_box.x, np->b_box.y, np->b_box.z, np->b_box.w);
-    const uint4 lk = make_uint4(np->link.x, np->link.y, np->link.z, np->link.w);
+    const GlobalNode4* np = (const GlobalNode4*)(nodes + r.node);
+    const float lx[4] = {np->lox.x, np->lox.y, np->lox.z, np->lox.w};
+    const float ly[4] = {np->loy.x, np->loy.y, np->loy.z, np->loy.w};
+    const float lz[4] = {np->loz.x, np->loz.y, np->loz.z, np->loz.w};
+    const float hx[4] = {np->hix.x, np->hix.y, np->hix.z, np->hix.w};
+    const float hy[4] = {np->hiy.x, np->hiy.y, np->hiy.z, np->hiy.w};
+    const float hz[4] = {np->hiz.x, np->hiz.y, np->hiz.z, np->hiz.w};
+    const uint32_t ch[4] = {np->child.x, np->child.y, np->child.z, np->child.w};
+    const uint32_t sl[4] = {np->slot.x, np->slot.y, np->slot.z, np->slot.w};
     if (COUNT)
         n_nodes++;
-    float t0min, t0max, t1min, t1max;
-    slab(r.o, r.inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
-    slab(r.o, r.inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
-    const float c0 = cull_bound(t0min, t0max, r.margin);
-    const float c1 = cull_bound(t1min, t1max, r.margin);
-    bool h0 = (t0min <= t0max) && (t0max >= r.tnear) && !(c0 > r.best.t);
-    bool h1 = (t1min <= t1max) && (t1max >= r.tnear) && !(c1 > r.best.t);
-    if (h0 && (lk.x & LEAF))
+    float tmin[4], cb[4];
+    bool h[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
         {
-        const uint32_t prim = lk.x & ~LEAF;
-        float t;
-        if (COUNT)
-            n_prims++;
-        if (leaf_test<TYPES>(geoms[lk.z], prim, r.o, r.d, r.tnear, t) && better(t, lk.z, prim, r.best))
+        float tmax;
+        slab(r.o, r.inv, lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], tmin[k], tmax);
+        cb[k] = cull_bound(tmin[k], tmax, r.margin);
+        h[k] = (tmin[k] <= tmax) && (tmax >= r.tnear) && !(cb[k] > r.best.t);
+        }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (h[k] && (ch[k] & LEAF))
             {
-            r.best.t = t;
-            r.best.geom = lk.z;
-            r.best.prim = prim;
+            visit_leaf<COUNT, TYPES>(geoms, ch[k], sl[k], r, n_prims);
+            h[k] = false;
             }
-        h0 = false;
-        }
-    if (h1 && (lk.y & LEAF))
+    // inner children still open, nearest entry first
+    float key[4];
+    uint32_t nd[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
         {
-        const uint32_t prim = lk.y & ~LEAF;
-        float t;
-        if (COUNT)
-            n_prims++;
-        if (leaf_test<TYPES>(geoms[lk.w], prim, r.o, r.d, r.tnear, t) && better(t, lk.w, prim, r.best))
-            {
-            r.best.t = t;
-            r.best.geom = lk.w;
-            r.best.prim = prim;
-            }
-        h1 = false;
+        const bool open = h[k] && !(cb[k] > r.best.t);
+        key[k] = open ? tmin[k] : __builtin_huge_valf();
+        nd[k] = open ? ch[k] : EMPTY;
         }
-    h0 = h0 && !(c0 > r.best.t);
-    h1 = h1 && !(c1 > r.best.t);
-    if (h0 && h1)
+    order2(key[0], nd[0], cb[0], key[1], nd[1], cb[1]);
+    order2(key[2], nd[2], cb[2], key[3], nd[3], cb[3]);
+    order2(key[0], nd[0], cb[0], key[2], nd[2], cb[2]);
+    order2(key[1], nd[1], cb[1], key[3], nd[3], cb[3]);
+    order2(key[1], nd[1], cb[1], key[2], nd[2], cb[2]);
+    if (nd[0] != EMPTY)
         {
-        const bool swap = t1min < t0min;
-        st.push(swap ? lk.x : lk.y, swap ? c0 : c1, err);
-        r.node = swap ? lk.y : lk.x;
-        return true;
-        }
-    if (h0)
-        {
-        r.node = lk.x;
-        return true;
-        }
-    if (h1)
-        {
-        r.node = lk.y;
+#pragma unroll
+        for (int k = 3; k >= 1; k--)
+            if (nd[k] != EMPTY)
+                st.push(nd[k], cb[k], err);
+        r.node = nd[0];
         return true;
         }
     while (st.sp > 0)
@@ -242,7 +269,7 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
     {
     TraceState r;
     trace_begin(sc, o, d, tnear, r, st);
-    while (trace_visit<Stack, COUNT, TYPES>(sc.nodes, sc.geoms, r, st, n_nodes, n_prims, err))
+    while (trace_visit<Stack, COUNT, TYPES>(sc.nodes4, sc.geoms, r, st, n_nodes, n_prims, err))
         {
         }
     best = r.best;
@@ -1229,7 +1256,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
     bool active = false;
     uint32_t i = 0;
     TraceState r;
-    const Node* nodes = nullptr;
+    const Node4* nodes = nullptr;
     const GeomDev* geoms = nullptr;
     for (;;)
         {
@@ -1262,7 +1289,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
                     job = __float_as_uint(b.w);
                     }
                 const SceneDev& sc = jobs[job].scene;
-                nodes = sc.nodes;
+                nodes = sc.nodes4;
                 geoms = sc.geoms;
                 trace_begin(sc, o, d, tnear, r, st);
                 active = true;

@@ -78,9 +78,20 @@ struct __attribute__((aligned(64))) Node
     uint4 link;   // c0, c1, g0, g1
     };
 
+// 4-wide node (128 bytes): child k's box is (lox[k], loy[k], loz[k]) -
+// (hix[k], hiy[k], hiz[k]); child[k] is a 4-wide node index, LEAF | local
+// primitive index (geometry slot in slot[k]) or EMPTY with a NaN box.
+struct __attribute__((aligned(128))) Node4
+    {
+    float4 lox, loy, loz, hix, hiy, hiz;
+    uint4 child;
+    uint4 slot;
+    };
+
 struct SceneDev
     {
     const Node* nodes;
+    const Node4* nodes4; // root: node 0 (as for nodes) unless root is EMPTY
     const GeomDev* geoms;
     uint32_t root;      // EMPTY when the scene has no primitive
     uint32_t n_geoms;

@@ -551,6 +551,8 @@ struct SceneObj
     DBuf<float> partial, cb, root_box, extent;
     float cull_margin = 0.0f; // CULL_K * 3 * largest primitive half-extent
     DBuf<Node> nodes;
+    DBuf<Node4> nodes4;
+    DBuf<uint32_t> idx4; // 4-wide node of each binary inner node (exclusive scan of even-depth flags)
     uint32_t n_prims = 0;
     bool bvh_valid = false;
     std::vector<uint32_t> topo; // (slot, prim count) of the enabled geometries in the last build
@@ -772,7 +774,16 @@ static void build_bvh(SceneObj& sc, bool refit_only)
     sc.root_box.ensure(6);
     if (N == 1)
         {
-        D.launch("lbvh_single", [&] { k_single_leaf<<<1, 1, 0, s>>>(sc.blo.p, sc.bhi.p, sc.nodes.p); });
+        static const uint32_t one[2] = {0u, 1u};
+        sc.idx4.ensure(2);
+        sc.nodes4.ensure(1);
+        HIPCHECK(hipMemcpyAsync(sc.idx4.p, one, sizeof(one), hipMemcpyHostToDevice, s));
+        D.launch("lbvh_single",
+                 [&]
+                 {
+                     k_single_leaf<<<1, 1, 0, s>>>(sc.blo.p, sc.bhi.p, sc.nodes.p);
+                     k_bvh4_fill<<<1, 64, 0, s>>>(sc.nodes.p, 1, sc.idx4.p, sc.nodes4.p);
+                 });
         sc.bvh_valid = true;
         sc.builds++;
         return;
@@ -805,6 +816,15 @@ static void build_bvh(SceneObj& sc, bool refit_only)
                      k_karras<<<blocks(N - 1, 256), 256, 0, s>>>(sc.keys.p, sc.vals.p, sc.blo.p, sc.bhi.p, (int)N,
                                                                  sc.nodes.p, sc.parent.p);
                  });
+        // 4-wide topology: inner nodes at even depth, numbered in index order
+        sc.idx4.ensure(N);
+        sc.nodes4.ensure(N - 1);
+        D.launch("lbvh_bvh4",
+                 [&]
+                 {
+                     k_depth_even<<<blocks(N, 256), 256, 0, s>>>(sc.parent.p, N - 1, sc.idx4.p);
+                     scan_exclusive(D, s, sc.idx4.p, N, sc.scan_tmp);
+                 });
         sc.builds++;
         }
     else
@@ -817,6 +837,8 @@ static void build_bvh(SceneObj& sc, bool refit_only)
                  k_refit<<<blocks(N, 256), 256, 0, s>>>(sc.vals.p, sc.blo.p, sc.bhi.p, (int)N, sc.nodes.p,
                                                         sc.parent.p, sc.flags.p, sc.root_box.p);
              });
+    D.launch("lbvh_bvh4",
+             [&] { k_bvh4_fill<<<blocks(N - 1, 256), 256, 0, s>>>(sc.nodes.p, N - 1, sc.idx4.p, sc.nodes4.p); });
     sc.bvh_valid = true;
     }
 
@@ -929,6 +951,7 @@ static SceneDev scene_dev(const SceneObj& sc)
     {
     SceneDev d;
     d.nodes = sc.nodes.p;
+    d.nodes4 = sc.nodes4.p;
     d.geoms = sc.dgeoms.p;
     d.root = sc.n_prims ? 0u : EMPTY;
     d.n_geoms = sc.n_slots;

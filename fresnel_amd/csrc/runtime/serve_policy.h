@@ -36,7 +36,8 @@ struct PolicyItem
 // primitive data) plus outputs plus the queues and records of one sample.
 inline uint64_t request_footprint(uint64_t n_prims, uint64_t prim_bytes, uint64_t W, uint64_t H, uint64_t L)
     {
-    const uint64_t scene = n_prims * (32 + 16 + 64 + 8 + 4 + prim_bytes);
+    // bounds, sort keys, binary node, 4-wide node and its index, parent, flags
+    const uint64_t scene = n_prims * (32 + 16 + 64 + 128 + 4 + 8 + 4 + prim_bytes);
     const uint64_t outputs = W * H * (16 + 4 + 16 + 4);
     const uint64_t per_sample = W * H * L * (2 * 64 + 16 + 16) + W * H * 64;
     return scene + outputs + per_sample;

@@ -43,10 +43,10 @@ def simulate(lib, fp, pps, samples, arrive, budget, target, iters=10000):
 
 
 def test_footprint_formula(lib):
-    # scene: 1M spheres x (32 + 16 + 64 + 8 + 4 + 16) bytes; outputs 40 B/pixel;
-    # per sample: rays/paths 160 B per path + 64 B per pixel
+    # scene: 1M spheres x (32 + 16 + 64 + 128 + 4 + 8 + 4 + 16) bytes; outputs
+    # 40 B/pixel; per sample: rays/paths 160 B per path + 64 B per pixel
     W, H, L = 1920, 1080, 1
-    expect = (1 << 20) * 140 + W * H * 40 + W * H * L * 160 + W * H * 64
+    expect = (1 << 20) * 272 + W * H * 40 + W * H * L * 160 + W * H * 64
     assert footprint(lib, 1 << 20, 16, W, H, L) == expect
     assert footprint(lib, 0, 16, 1, 1, 1) == 40 + 160 + 64
     # monotone in every argument
