@@ -453,9 +453,10 @@ __device__ __forceinline__ void child_box(const Node& n, uint32_t side, float b[
     }
 
 // 4-wide node idx4[i] from inner node i of n, for the nodes at even depth:
-// idx4 is the exclusive scan of k_depth_even's flags (n + 1 entries).
+// idx4 is the exclusive scan of k_depth_even's flags (n + 1 entries).  With
+// sphere_geoms (a spheres-only scene) leaf children hold their spheres.
 __global__ void k_bvh4_fill(const Node* __restrict__ nodes, uint32_t n, const uint32_t* __restrict__ idx4,
-                            Node4* __restrict__ out)
+                            Node4* __restrict__ out, const GeomDev* __restrict__ sphere_geoms)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || idx4[i + 1] == idx4[i])
@@ -485,6 +486,18 @@ __global__ void k_bvh4_fill(const Node* __restrict__ nodes, uint32_t n, const ui
             m++;
             }
         }
+    if (sphere_geoms)
+        for (uint32_t k = 0; k < m; k++)
+            if ((ref[k] & LEAF) && ref[k] != EMPTY)
+                {
+                const float4 sp = sphere_geoms[slot[k]].a[ref[k] & ~LEAF];
+                box[k][0] = sp.x;
+                box[k][1] = sp.y;
+                box[k][2] = sp.z;
+                box[k][3] = sp.w;
+                box[k][4] = 0.0f;
+                box[k][5] = 0.0f;
+                }
     const float nan = __builtin_nanf("");
     for (; m < 4; m++)
         {

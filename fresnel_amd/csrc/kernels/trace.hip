@@ -132,6 +132,7 @@ struct TraceState
     float margin;
     HitT best;
     uint32_t node;
+    bool sph; // 4-wide leaf children hold spheres
     };
 
 template<class Stack>
@@ -146,6 +147,7 @@ __device__ __forceinline__ void trace_begin(const SceneDev& sc, v3 o, v3 d, floa
     r.best.geom = EMPTY;
     r.best.prim = EMPTY;
     r.node = sc.root;
+    r.sph = sc.sphere_leaves != 0;
     st.sp = 0;
     }
 
@@ -207,13 +209,22 @@ __device__ __forceinline__ bool trace_visit(const Node4* __restrict__ nodes, con
     const uint32_t sl[4] = {np->slot.x, np->slot.y, np->slot.z, np->slot.w};
     if (COUNT)
         n_nodes++;
+    // spheres-only kernels always see sphere leaves (checked on the host)
+    const bool sph = (TYPES == T_SPHERES) || r.sph;
     float tmin[4], cb[4];
     bool h[4];
 #pragma unroll
     for (int k = 0; k < 4; k++)
         {
+        // a sphere leaf's box: centre -+ radius, as the builder bounds it
+        // (GeometrySphere.cc:60-73)
+        const bool sl_k = sph && (ch[k] & LEAF);
+        const float rr = hx[k];
+        const float x0 = sl_k ? lx[k] - rr : lx[k], x1 = sl_k ? lx[k] + rr : hx[k];
+        const float y0 = sl_k ? ly[k] - rr : ly[k], y1 = sl_k ? ly[k] + rr : hy[k];
+        const float z0 = sl_k ? lz[k] - rr : lz[k], z1 = sl_k ? lz[k] + rr : hz[k];
         float tmax;
-        slab(r.o, r.inv, lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], tmin[k], tmax);
+        slab(r.o, r.inv, x0, y0, z0, x1, y1, z1, tmin[k], tmax);
         cb[k] = cull_bound(tmin[k], tmax, r.margin);
         h[k] = (tmin[k] <= tmax) && (tmax >= r.tnear) && !(cb[k] > r.best.t);
         }
@@ -221,7 +232,24 @@ __device__ __forceinline__ bool trace_visit(const Node4* __restrict__ nodes, con
     for (int k = 0; k < 4; k++)
         if (h[k] && (ch[k] & LEAF))
             {
-            visit_leaf<COUNT, TYPES>(geoms, ch[k], sl[k], r, n_prims);
+            if (sph)
+                {
+                // the sphere is in the node: no memory access
+                const uint32_t prim = ch[k] & ~LEAF;
+                float t, de;
+                bool first;
+                if (COUNT)
+                    n_prims++;
+                if (sphere_t(r.o, r.d, mk3(lx[k], ly[k], lz[k]), hx[k], t, de, first) && (r.tnear < t)
+                    && better(t, sl[k], prim, r.best))
+                    {
+                    r.best.t = t;
+                    r.best.geom = sl[k];
+                    r.best.prim = prim;
+                    }
+                }
+            else
+                visit_leaf<COUNT, TYPES>(geoms, ch[k], sl[k], r, n_prims);
             h[k] = false;
             }
     // inner children still open, nearest entry first

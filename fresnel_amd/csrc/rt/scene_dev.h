@@ -80,7 +80,10 @@ struct __attribute__((aligned(64))) Node
 
 // 4-wide node (128 bytes): child k's box is (lox[k], loy[k], loz[k]) -
 // (hix[k], hiy[k], hiz[k]); child[k] is a 4-wide node index, LEAF | local
-// primitive index (geometry slot in slot[k]) or EMPTY with a NaN box.
+// primitive index (geometry slot in slot[k]) or EMPTY with a NaN box.  In a
+// spheres-only scene (SceneDev::sphere_leaves) a leaf child holds its sphere
+// instead, centre (lox, loy, loz)[k] and radius hix[k]; its box is
+// centre -+ radius, the builder's bounds.
 struct __attribute__((aligned(128))) Node4
     {
     float4 lox, loy, loz, hix, hiy, hiz;
@@ -94,6 +97,7 @@ struct SceneDev
     const Node4* nodes4; // root: node 0 (as for nodes) unless root is EMPTY
     const GeomDev* geoms;
     uint32_t root;      // EMPTY when the scene has no primitive
+    uint32_t sphere_leaves; // 4-wide leaf children hold spheres (spheres-only scene)
     uint32_t n_geoms;
     uint32_t n_prims;
     float cull_margin;  // CULL_K * 3 * largest primitive half-extent

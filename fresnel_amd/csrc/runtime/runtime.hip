@@ -552,6 +552,7 @@ struct SceneObj
     float cull_margin = 0.0f; // CULL_K * 3 * largest primitive half-extent
     DBuf<Node> nodes;
     DBuf<Node4> nodes4;
+    bool sphere_leaves = false; // nodes4 leaf children hold spheres
     DBuf<uint32_t> idx4; // 4-wide node of each binary inner node (exclusive scan of even-depth flags)
     uint32_t n_prims = 0;
     bool bvh_valid = false;
@@ -717,6 +718,8 @@ static bool radix_sort_pairs(DeviceObj& D, hipStream_t s, const char* name, uint
     return flipped;
     }
 
+static uint32_t scene_types(const SceneObj& sc);
+
 static void build_bvh(SceneObj& sc, bool refit_only)
     {
     DeviceObj& D = *sc.dev;
@@ -736,6 +739,7 @@ static void build_bvh(SceneObj& sc, bool refit_only)
         N += n;
         }
     sc.n_prims = N;
+    sc.sphere_leaves = scene_types(sc) == T_SPHERES;
     if (N == 0)
         {
         sc.bvh_valid = true;
@@ -782,7 +786,8 @@ static void build_bvh(SceneObj& sc, bool refit_only)
                  [&]
                  {
                      k_single_leaf<<<1, 1, 0, s>>>(sc.blo.p, sc.bhi.p, sc.nodes.p);
-                     k_bvh4_fill<<<1, 64, 0, s>>>(sc.nodes.p, 1, sc.idx4.p, sc.nodes4.p);
+                     k_bvh4_fill<<<1, 64, 0, s>>>(sc.nodes.p, 1, sc.idx4.p, sc.nodes4.p,
+                                                   sc.sphere_leaves ? sc.dgeoms.p : nullptr);
                  });
         sc.bvh_valid = true;
         sc.builds++;
@@ -838,7 +843,11 @@ static void build_bvh(SceneObj& sc, bool refit_only)
                                                         sc.parent.p, sc.flags.p, sc.root_box.p);
              });
     D.launch("lbvh_bvh4",
-             [&] { k_bvh4_fill<<<blocks(N - 1, 256), 256, 0, s>>>(sc.nodes.p, N - 1, sc.idx4.p, sc.nodes4.p); });
+             [&]
+             {
+                 k_bvh4_fill<<<blocks(N - 1, 256), 256, 0, s>>>(sc.nodes.p, N - 1, sc.idx4.p, sc.nodes4.p,
+                                                                 sc.sphere_leaves ? sc.dgeoms.p : nullptr);
+             });
     sc.bvh_valid = true;
     }
 
@@ -952,6 +961,7 @@ static SceneDev scene_dev(const SceneObj& sc)
     SceneDev d;
     d.nodes = sc.nodes.p;
     d.nodes4 = sc.nodes4.p;
+    d.sphere_leaves = sc.sphere_leaves ? 1u : 0u;
     d.geoms = sc.dgeoms.p;
     d.root = sc.n_prims ? 0u : EMPTY;
     d.n_geoms = sc.n_slots;
@@ -1072,8 +1082,6 @@ struct TracerObj
         HIPCHECK(hipStreamSynchronize(dev->stream));
         }
     };
-
-static uint32_t scene_types(const SceneObj& sc);
 
 // Bring a tracer's per-tile candidate lists up to date for scene `sc` seen
 // through `cam` (pinhole perspective or orthographic cameras only; a scene a
@@ -1352,6 +1360,9 @@ static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
     auto reset_segments = [&] { HIPCHECK(hipMemsetAsync(R.qseg.p, 0, (QSEG_FLAG + 1) * sizeof(uint32_t), s)); };
 
     const bool spheres = (types == T_SPHERES);
+    for (const auto& J : jobs)
+        REQUIRE(!spheres || J.scene.sphere_leaves || J.scene.root == EMPTY,
+                "internal error: spheres-only kernels on a hierarchy without sphere leaves");
     auto traverse = [&](const Queue& q, uint32_t n, bool primary)
     {
         HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
