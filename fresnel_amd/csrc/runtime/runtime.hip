@@ -150,11 +150,14 @@ struct DeviceObj
     {
     int hip_dev = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr; // second render stream: two batches in flight
+    hipStream_t lstream = nullptr; // stream of the launches being enqueued (timing events)
     int n_cu = 0;
     std::string name;
     int limit = -1;
     uint32_t trav_grid = 0;
-    RenderScratch rs;
+    RenderScratch rs, rs2; // render scratch of the two batches in flight
+    hipEvent_t acc_done[2] = {nullptr, nullptr}; // last accumulation enqueued on stream / stream2
     DBuf<unsigned long long> counters;
     uint32_t* h_pinned = nullptr; // 16 uint32 of pinned host memory for counts
     uint32_t* h_counts = nullptr; // pinned copy of a queue's segment counters and flag
@@ -213,10 +216,11 @@ struct DeviceObj
         kt.name = nm;
         kt.a = event();
         kt.b = event();
-        HIPCHECK(hipEventRecord(kt.a, stream));
+        hipStream_t ls = lstream ? lstream : stream;
+        HIPCHECK(hipEventRecord(kt.a, ls));
         f();
         HIPCHECK(hipGetLastError());
-        HIPCHECK(hipEventRecord(kt.b, stream));
+        HIPCHECK(hipEventRecord(kt.b, ls));
         times.push_back(kt);
         }
     void begin_stats()
@@ -233,6 +237,8 @@ struct DeviceObj
     void end_stats()
         {
         HIPCHECK(hipStreamSynchronize(stream));
+        if (stream2)
+            HIPCHECK(hipStreamSynchronize(stream2));
         last_stats.clear();
         for (auto& t : times)
             {
@@ -1259,11 +1265,11 @@ static uint32_t scene_types(const SceneObj& sc)
 // Counts of the QSEG segments a producer filled (counters QSEG_STRIDE words
 // apart, then the overflow flag): sets the consumer view of q (pre, adj) and
 // returns the number of entries.
-static uint32_t read_segments(DeviceObj& D, const uint32_t* dcount, Queue& q)
+static uint32_t read_segments(DeviceObj& D, hipStream_t s, const uint32_t* dcount, Queue& q)
     {
     const size_t words = QSEG_FLAG + 1;
-    HIPCHECK(hipMemcpyAsync(D.h_counts, dcount, words * sizeof(uint32_t), hipMemcpyDeviceToHost, D.stream));
-    HIPCHECK(hipStreamSynchronize(D.stream));
+    HIPCHECK(hipMemcpyAsync(D.h_counts, dcount, words * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
     REQUIRE(D.h_counts[QSEG_FLAG] == 0, "internal error: a ray queue segment overflowed");
     uint64_t total = 0;
     for (uint32_t k = 0; k < QSEG; k++)
@@ -1304,140 +1310,210 @@ static void set_order_divisors(JobDev& J)
     J.div_tiles = make_fastdiv(std::max(1u, J.W >> 3));
     }
 
-// Run one batch: every job advances by jobs[j].K camera samples.
-static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
+// One batch in flight: every job advances by jobs[j].K camera samples.  The
+// batch runs on one render scratch and stream; start() enqueues the primary
+// rays, each step() waits for the last producer's queue counts (on this
+// stream only) and enqueues the next bounce or the tail, and accumulate()
+// enqueues the Welford accumulation once every path has a record.
+struct BatchExec
     {
-    hipStream_t s = D.stream;
-    RenderScratch& R = D.rs;
-    uint64_t n_pri = 0, n_paths = 0, n_pix = 0;
+    DeviceObj& D;
+    RenderScratch& R;
+    hipStream_t s;
+    std::vector<JobDev> jobs;
+    uint32_t types;
+    bool spheres;
     std::vector<uint32_t> pix_base;
-    for (auto& J : jobs)
-        {
-        set_order_divisors(J);
-        J.ray_base = (uint32_t)n_pri;
-        J.rec_base = n_paths;
-        pix_base.push_back((uint32_t)n_pix);
-        n_pri += (uint64_t)J.npix * J.K;
-        n_paths += (uint64_t)J.npix * J.K * J.L;
-        n_pix += J.npix;
-        }
-    REQUIRE(n_pri < (1ull << 31) && n_paths < (1ull << 31), "batch too large");
-    // primary rays are generated from their index; the queues hold paths
-    uint32_t max_l = 1;
-    for (const auto& J : jobs)
-        max_l = std::max(max_l, J.L);
-    const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
-    const uint32_t seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * groups, D.trav_chunk);
-    const uint64_t cap = (uint64_t)QSEG * seg_cap;
-    R.hits.ensure(std::max(n_pri, n_paths));
-    for (int q = 0; q < 2; q++)
-        {
-        R.qo[q].ensure(cap);
-        R.qd[q].ensure(cap);
-        R.qatt[q].ensure(cap);
-        R.qmeta[q].ensure(cap);
-        }
-    R.records.ensure(n_paths);
-    R.counters32.ensure(4);
-    R.qseg.ensure(QSEG_FLAG + 1);
-    R.jobs.ensure(jobs.size());
-    R.pix_base.ensure(jobs.size());
-    HIPCHECK(hipMemcpyAsync(R.jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemcpyAsync(R.pix_base.p, pix_base.data(), pix_base.size() * 4, hipMemcpyHostToDevice, s));
-    const uint32_t gstride = D.trav_grid * TRAV_BLOCK; // max resident threads of any traversal variant
-    R.gstack.ensure((size_t)(STACK_MAX - 8) * gstride);
-    R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * D.tail_rays);
-    Queue Q[2];
-    for (int q = 0; q < 2; q++)
-        {
-        Q[q] = Queue {R.qo[q].p, R.qd[q].p, R.qatt[q].p, R.qmeta[q].p};
-        Q[q].count = R.qseg.p;
-        Q[q].seg_cap = seg_cap;
-        }
-    const uint32_t nj = (uint32_t)jobs.size();
-    uint32_t* work = R.counters32.p;
-    unsigned long long* ctr = D.counters.p;
-    auto reset_segments = [&] { HIPCHECK(hipMemsetAsync(R.qseg.p, 0, (QSEG_FLAG + 1) * sizeof(uint32_t), s)); };
-
-    const bool spheres = (types == T_SPHERES);
-    for (const auto& J : jobs)
-        REQUIRE(!spheres || J.scene.sphere_leaves || J.scene.root == EMPTY,
-                "internal error: spheres-only kernels on a hierarchy without sphere leaves");
-    auto traverse = [&](const Queue& q, uint32_t n, bool primary)
-    {
-        HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
-        TravArgs a {R.jobs.p, nj, q, n, R.hits.p, work, D.trav_chunk, R.gstack.p, gstride, ctr};
-        dispatch_traverse(D, primary ? "traverse_primary" : "traverse", s, a, primary, spheres);
-    };
-
-    if (D.fuse_primary == 2)
-        {
-        HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
-        reset_segments();
-        PrimaryArgs a {R.jobs.p, nj, (uint32_t)n_pri, work, groups, R.gstack.p, gstride, Q[1], R.records.p, ctr};
-        dispatch_primary(D, s, a, spheres);
-        }
-    else
-        {
-        traverse(Q[0], (uint32_t)n_pri, true);
-        reset_segments();
-        D.launch("shade_primary",
-                 [&]
-                 {
-                     auto* f = spheres ? k_shade_primary<SHADE_BLOCK, T_SPHERES> : k_shade_primary<SHADE_BLOCK, T_ALL>;
-                     f<<<blocks(n_pri, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri,
-                                                                          Q[1], R.records.p);
-                 });
-        }
-    uint32_t n = read_segments(D, R.qseg.p, Q[1]);
+    uint64_t n_pri = 0, n_paths = 0, n_pix = 0;
+    uint32_t nj = 0, gstride = 0, n = 0;
     int cur = 1;
-    while (n > 0)
+    bool paths_done = false;
+    Queue Q[2];
+
+    BatchExec(DeviceObj& d, RenderScratch& r, hipStream_t st, std::vector<JobDev> j, uint32_t t)
+        : D(d), R(r), s(st), jobs(std::move(j)), types(t), spheres(t == T_SPHERES)
         {
+        }
+
+    template<class F> void launch(const char* nm, F f)
+        {
+        D.lstream = s;
+        D.launch(nm, f);
+        D.lstream = nullptr;
+        }
+
+    void reset_segments()
+        {
+        HIPCHECK(hipMemsetAsync(R.qseg.p, 0, (QSEG_FLAG + 1) * sizeof(uint32_t), s));
+        }
+
+    void traverse(const Queue& q, uint32_t count, bool primary)
+        {
+        HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), s));
+        TravArgs a {R.jobs.p, nj, q, count, R.hits.p, R.counters32.p, D.trav_chunk, R.gstack.p, gstride,
+                    D.counters.p};
+        D.lstream = s;
+        dispatch_traverse(D, primary ? "traverse_primary" : "traverse", s, a, primary, spheres);
+        D.lstream = nullptr;
+        }
+
+    void start()
+        {
+        for (auto& J : jobs)
+            {
+            set_order_divisors(J);
+            J.ray_base = (uint32_t)n_pri;
+            J.rec_base = n_paths;
+            pix_base.push_back((uint32_t)n_pix);
+            n_pri += (uint64_t)J.npix * J.K;
+            n_paths += (uint64_t)J.npix * J.K * J.L;
+            n_pix += J.npix;
+            }
+        REQUIRE(n_pri < (1ull << 31) && n_paths < (1ull << 31), "batch too large");
+        for (const auto& J : jobs)
+            REQUIRE(!spheres || J.scene.sphere_leaves || J.scene.root == EMPTY,
+                    "internal error: spheres-only kernels on a hierarchy without sphere leaves");
+        // primary rays are generated from their index; the queues hold paths
+        uint32_t max_l = 1;
+        for (const auto& J : jobs)
+            max_l = std::max(max_l, J.L);
+        const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
+        const uint32_t seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * groups, D.trav_chunk);
+        const uint64_t cap = (uint64_t)QSEG * seg_cap;
+        R.hits.ensure(std::max(n_pri, n_paths));
+        for (int q = 0; q < 2; q++)
+            {
+            R.qo[q].ensure(cap);
+            R.qd[q].ensure(cap);
+            R.qatt[q].ensure(cap);
+            R.qmeta[q].ensure(cap);
+            }
+        R.records.ensure(n_paths);
+        R.counters32.ensure(4);
+        R.qseg.ensure(QSEG_FLAG + 1);
+        R.jobs.ensure(jobs.size());
+        R.pix_base.ensure(jobs.size());
+        HIPCHECK(hipMemcpyAsync(R.jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(R.pix_base.p, pix_base.data(), pix_base.size() * 4, hipMemcpyHostToDevice, s));
+        gstride = D.trav_grid * TRAV_BLOCK; // max resident threads of any traversal variant
+        R.gstack.ensure((size_t)(STACK_MAX - 8) * gstride);
+        R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * D.tail_rays);
+        for (int q = 0; q < 2; q++)
+            {
+            Q[q] = Queue {R.qo[q].p, R.qd[q].p, R.qatt[q].p, R.qmeta[q].p};
+            Q[q].count = R.qseg.p;
+            Q[q].seg_cap = seg_cap;
+            }
+        nj = (uint32_t)jobs.size();
+        if (D.fuse_primary == 2)
+            {
+            HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), s));
+            reset_segments();
+            PrimaryArgs a {R.jobs.p, nj, (uint32_t)n_pri, R.counters32.p, groups, R.gstack.p, gstride, Q[1],
+                           R.records.p, D.counters.p};
+            D.lstream = s;
+            dispatch_primary(D, s, a, spheres);
+            D.lstream = nullptr;
+            }
+        else
+            {
+            traverse(Q[0], (uint32_t)n_pri, true);
+            reset_segments();
+            launch("shade_primary",
+                   [&]
+                   {
+                       auto* f = spheres ? k_shade_primary<SHADE_BLOCK, T_SPHERES> : k_shade_primary<SHADE_BLOCK, T_ALL>;
+                       f<<<blocks(n_pri, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, R.hits.p, (uint32_t)n_pri,
+                                                                            Q[1], R.records.p);
+                   });
+            }
+        cur = 1;
+        }
+
+    // Wait for the queue the last launches filled and enqueue the next work;
+    // false once every path has reached its record.
+    bool step()
+        {
+        if (paths_done)
+            return false;
+        n = read_segments(D, s, R.qseg.p, Q[cur]);
+        if (n == 0)
+            {
+            paths_done = true;
+            return false;
+            }
         if (n <= D.tail_rays)
             {
-            D.launch("finish",
-                     [&]
-                     {
-                         const dim3 g(blocks(n, TRAV_BLOCK));
-                         auto* f = spheres ? (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_SPHERES>
-                                                         : k_finish<TRAV_BLOCK, TRAV_SS, false, T_SPHERES>)
-                                           : (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_ALL>
-                                                         : k_finish<TRAV_BLOCK, TRAV_SS, false, T_ALL>);
-                         f<<<g, TRAV_BLOCK, 0, s>>>(R.jobs.p, Q[cur], n, R.records.p, MAX_DEPTH, R.tail_stack.p,
-                                                    n, ctr);
-                     });
-            break;
+            const uint32_t cnt = n;
+            const Queue q = Q[cur];
+            launch("finish",
+                   [&]
+                   {
+                       const dim3 g(blocks(cnt, TRAV_BLOCK));
+                       auto* f = spheres ? (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_SPHERES>
+                                                       : k_finish<TRAV_BLOCK, TRAV_SS, false, T_SPHERES>)
+                                         : (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_ALL>
+                                                       : k_finish<TRAV_BLOCK, TRAV_SS, false, T_ALL>);
+                       f<<<g, TRAV_BLOCK, 0, s>>>(R.jobs.p, q, cnt, R.records.p, MAX_DEPTH, R.tail_stack.p, cnt,
+                                                  D.counters.p);
+                   });
+            paths_done = true;
+            return false;
             }
         if (D.fuse_bounce == 2)
             {
-            HIPCHECK(hipMemsetAsync(work, 0, sizeof(uint32_t), s));
+            HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), s));
             reset_segments();
-            BounceArgs a {R.jobs.p, Q[cur], n, work, D.trav_chunk, R.gstack.p, gstride, Q[1 - cur], R.records.p,
-                          ctr};
+            BounceArgs a {R.jobs.p, Q[cur], n, R.counters32.p, D.trav_chunk, R.gstack.p, gstride, Q[1 - cur],
+                          R.records.p, D.counters.p};
+            D.lstream = s;
             dispatch_bounce(D, s, a, spheres);
+            D.lstream = nullptr;
             }
         else
             {
             traverse(Q[cur], n, false);
             reset_segments();
-            D.launch("shade",
-                     [&]
-                     {
-                         auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
-                         f<<<blocks(n, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, n, Q[1 - cur],
-                                                                          R.records.p, MAX_DEPTH, ctr);
-                     });
+            const uint32_t cnt = n;
+            launch("shade",
+                   [&]
+                   {
+                       auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
+                       f<<<blocks(cnt, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, cnt,
+                                                                          Q[1 - cur], R.records.p, MAX_DEPTH,
+                                                                          D.counters.p);
+                   });
             }
-        n = read_segments(D, R.qseg.p, Q[1 - cur]);
         cur = 1 - cur;
+        return true;
         }
-    D.launch("accumulate",
-             [&]
-             {
-                 k_accumulate<<<blocks(n_pix, 256), 256, 0, s>>>(R.jobs.p, R.pix_base.p, nj, (uint32_t)n_pix,
-                                                                 R.records.p);
-             });
-    HIPCHECK(hipStreamSynchronize(s));
+
+    // Enqueue the accumulation after `after` (the previous batch's, if any).
+    void accumulate(hipEvent_t after, hipEvent_t done)
+        {
+        if (after)
+            HIPCHECK(hipStreamWaitEvent(s, after, 0));
+        launch("accumulate",
+               [&]
+               {
+                   k_accumulate<<<blocks(n_pix, 256), 256, 0, s>>>(R.jobs.p, R.pix_base.p, nj, (uint32_t)n_pix,
+                                                                   R.records.p);
+               });
+        if (done)
+            HIPCHECK(hipEventRecord(done, s));
+        }
+    };
+
+// Run one batch to completion on the device's first scratch and stream.
+static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
+    {
+    BatchExec e(D, D.rs, D.stream, jobs, types);
+    e.start();
+    while (e.step())
+        {
+        }
+    e.accumulate(nullptr, nullptr);
+    HIPCHECK(hipStreamSynchronize(D.stream));
     }
 
 // Run jobs whose scenes hold spheres only and the other jobs as two batches,
@@ -1489,22 +1565,77 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
     std::vector<uint32_t> types;
     for (auto* sc : scenes)
         types.push_back(scene_types(*sc));
-    uint32_t done = 0;
-    while (done < samples)
+    // Two batches in flight, one per render scratch and stream: while the host
+    // waits for one batch's queue counts the other batch's kernels run, and a
+    // batch's tail overlaps the next batch's primary rays.  Accumulations are
+    // enqueued in batch order, each after the previous one (event), so every
+    // pixel's samples are accumulated in order.
+    struct Slot
         {
-        const uint32_t k = std::min(K, samples - done);
-        std::vector<JobDev> jobs;
-        for (size_t j = 0; j < scenes.size(); j++)
+        std::unique_ptr<BatchExec> e;
+        int slot;
+        };
+    std::deque<Slot> active;
+    std::deque<std::pair<std::vector<JobDev>, uint32_t>> pending;
+    HIPCHECK(hipStreamSynchronize(D.stream)); // scene, tile lists and counters ready for both streams
+    hipEvent_t last_acc = nullptr;
+    uint32_t done = 0;
+    for (;;)
+        {
+        while (active.size() < 2)
             {
-            JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
-            J.K = k;
-            jobs.push_back(J);
+            if (pending.empty())
+                {
+                if (done >= samples)
+                    break;
+                const uint32_t k = std::min(K, samples - done);
+                std::vector<JobDev> sph, mixed;
+                uint32_t mixed_types = 0;
+                for (size_t j = 0; j < scenes.size(); j++)
+                    {
+                    JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
+                    J.K = k;
+                    J.n_base = tracers[j]->n_samples + done;
+                    if ((types[j] & ~T_SPHERES) == 0)
+                        sph.push_back(J);
+                    else
+                        {
+                        mixed.push_back(J);
+                        mixed_types |= types[j];
+                        }
+                    }
+                if (!sph.empty())
+                    pending.emplace_back(std::move(sph), T_SPHERES);
+                if (!mixed.empty())
+                    pending.emplace_back(std::move(mixed), mixed_types);
+                done += k;
+                continue;
+                }
+            const int slot = active.empty() ? 0 : 1 - active.front().slot;
+            Slot sl;
+            sl.slot = slot;
+            sl.e.reset(new BatchExec(D, slot ? D.rs2 : D.rs, slot ? D.stream2 : D.stream,
+                                     std::move(pending.front().first), pending.front().second));
+            pending.pop_front();
+            sl.e->start();
+            active.push_back(std::move(sl));
             }
-        run_grouped(D, jobs, types);
-        for (auto* tr : tracers)
-            tr->n_samples += k;
-        done += k;
+        if (active.empty())
+            break;
+        for (auto& a : active)
+            a.e->step();
+        while (!active.empty() && active.front().e->paths_done)
+            {
+            Slot& f = active.front();
+            f.e->accumulate(last_acc, D.acc_done[f.slot]);
+            last_acc = D.acc_done[f.slot];
+            active.pop_front();
+            }
         }
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream2));
+    for (auto* tr : tracers)
+        tr->n_samples += samples;
     D.end_stats();
     if (D.last_counters[CTR_OVERFLOW])
         throw Error(FH_ERR_STATE, "traversal stack overflow");
@@ -1750,6 +1881,9 @@ extern "C"
                 D.n_cu = p.multiProcessorCount;
                 D.name = std::string(p.name) + " (" + p.gcnArchName + ")";
                 HIPCHECK(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
+                HIPCHECK(hipStreamCreateWithFlags(&D.stream2, hipStreamNonBlocking));
+                for (auto& e : D.acc_done)
+                    HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 // overflow stacks are sized for the largest resident traversal grid
                 D.trav_grid = std::max(max_grid_ss<8>(D), max_grid_ss<16>(D));
                 if (const char* e = getenv("FRESNEL_AMD_BATCH_PATHS"))
@@ -1787,10 +1921,16 @@ extern "C"
                     (void)hipHostFree(dev->obj.h_pinned);
                 if (dev->obj.h_counts)
                     (void)hipHostFree(dev->obj.h_counts);
-                hipStream_t s = dev->obj.stream;
+                (void)hipStreamSynchronize(dev->obj.stream2);
+                for (auto e : dev->obj.acc_done)
+                    if (e)
+                        (void)hipEventDestroy(e);
+                hipStream_t s = dev->obj.stream, s2 = dev->obj.stream2;
                 delete dev;
                 if (s)
                     (void)hipStreamDestroy(s);
+                if (s2)
+                    (void)hipStreamDestroy(s2);
             });
         }
 

@@ -198,6 +198,42 @@ def test_batch_equals_plain(gpu_device):
         assert numpy.array_equal(numpy.array(t.linear_output[:]), plain[k]), f"scene {k} differs"
 
 
+def _spheres_and_cylinders(device):
+    s = scenes.four_cylinders(device)
+    fresnel.geometry.Sphere(s, position=[[0.5, 0.5, 0.5], [-1, 0.2, 0.3], [0.3, -1, -0.4]], radius=0.35)
+    return s
+
+
+def test_batches_in_flight_equal_one_batch(gpu_device):
+    """Many small batches (two in flight, spheres-only and mixed groups) give the one-batch images bit for bit."""
+    d = gpu_device._device
+    saved = d.tune('batch_paths', 0)
+    builders = [lambda dv: c1_scene(dv, n=3000, w_seed=5), _spheres_and_cylinders]
+    sizes = [(64, 48), (40, 56)]
+
+    def render():
+        sc = [b(gpu_device) for b in builders]
+        trs = [fresnel.tracer.Path(gpu_device, w=w, h=h) for (w, h) in sizes]
+        for t in trs:
+            t.reset()
+            t._tracer.setLightSamples(1)
+        fresnel._gpu.batch_render([x._scene for x in sc], [t._tracer for t in trs], 9)
+        alone = [_linear(gpu_device, b, w, h, 9, 1)[0] for b, (w, h) in zip(builders, sizes)]
+        return [numpy.array(t.linear_output[:]) for t in trs], alone
+
+    try:
+        d.tune('batch_paths', 1 << 26)
+        one_batch, one_alone = render()
+        d.tune('batch_paths', 2000)  # one camera sample per batch: 9 batches of two groups
+        many, many_alone = render()
+    finally:
+        d.tune('batch_paths', saved)
+    for k in range(len(builders)):
+        assert numpy.array_equal(many[k], one_batch[k]), f"scene {k} differs (batched)"
+        assert numpy.array_equal(many_alone[k], one_alone[k]), f"scene {k} differs (alone)"
+        assert numpy.array_equal(one_alone[k], one_batch[k]), f"scene {k}: batched differs from alone"
+
+
 def test_random_rays_vs_checker(gpu_device, cpu_device):
     """Closest hits of random rays (many starting inside spheres) are bit-exact."""
     rng = numpy.random.default_rng(3)
