@@ -21,6 +21,7 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
 * ``fuse_primary`` primary traversal and depth-0 shading in one kernel (2) or
                   two (1).
 * ``fuse_bounce`` the same for each bounce of the secondary rays.
+* ``inflight`` batches of one render in flight on the device's two streams.
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -43,6 +44,7 @@ CANDIDATES = {
     'primary_tiles': [1, 2],
     'fuse_primary': [1, 2],
     'fuse_bounce': [1, 2],
+    'inflight': [1, 2],
 }
 
 

@@ -191,7 +191,7 @@ __device__ __forceinline__ void order2(float& ka, uint32_t& na, float& ca, float
 // distances (cull_bound) exceeds the closest hit, so neither the hierarchy
 // nor the visiting order changes the result.
 template<class Stack, bool COUNT, uint32_t TYPES>
-__device__ __forceinline__ bool trace_visit(const Node4* __restrict__ nodes, const GeomDev* __restrict__ geoms,
+__device__ __forceinline__ bool trace_visit4(const Node4* __restrict__ nodes, const GeomDev* __restrict__ geoms,
                                             TraceState& r, Stack& st, uint32_t& n_nodes, uint32_t& n_prims,
                                             uint32_t& err)
     {
@@ -291,13 +291,117 @@ __device__ __forceinline__ bool trace_visit(const Node4* __restrict__ nodes, con
     return false;
     }
 
+// One node visit of the closest-hit search over the binary hierarchy (the
+// kernels for scenes with other primitive types, whose larger intersectors
+// leave no registers for 4-wide nodes): the primitives of leaf children
+// are tested, the nearer hit inner child is visited next and the farther one
+// pushed.  Returns false when the search is complete.  The closest hit is
+// the smallest (t, geometry, primitive) over the primitives whose bounds pass
+// the slab test with tmax >= tnear; a subtree is culled when the lower bound
+// of its primitives' distances (cull_bound) exceeds the closest hit.
+template<class Stack, bool COUNT, uint32_t TYPES>
+__device__ __forceinline__ bool trace_visit2(const Node* __restrict__ nodes, const GeomDev* __restrict__ geoms,
+                                            TraceState& r, Stack& st, uint32_t& n_nodes, uint32_t& n_prims,
+                                            uint32_t& err)
+    {
+    if (r.node == EMPTY)
+        return false;
+    // global address space: direct global loads instead of flat ones
+    const GlobalNode* np = (const GlobalNode*)(nodes + r.node);
+    const float4 A = make_float4(np->a_lo.x, np->a_lo.y, np->a_lo.z, np->a_lo.w);
+    const float4 B = make_float4(np->a_hi.x, np->a_hi.y, np->a_hi.z, np->a_hi.w);
+    const float4 C = make_float4(np->b_box.x, np->b_box.y, np->b_box.z, np->b_box.w);
+    const uint4 lk = make_uint4(np->link.x, np->link.y, np->link.z, np->link.w);
+    if (COUNT)
+        n_nodes++;
+    float t0min, t0max, t1min, t1max;
+    slab(r.o, r.inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
+    slab(r.o, r.inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
+    const float c0 = cull_bound(t0min, t0max, r.margin);
+    const float c1 = cull_bound(t1min, t1max, r.margin);
+    bool h0 = (t0min <= t0max) && (t0max >= r.tnear) && !(c0 > r.best.t);
+    bool h1 = (t1min <= t1max) && (t1max >= r.tnear) && !(c1 > r.best.t);
+    if (h0 && (lk.x & LEAF))
+        {
+        const uint32_t prim = lk.x & ~LEAF;
+        float t;
+        if (COUNT)
+            n_prims++;
+        if (leaf_test<TYPES>(geoms[lk.z], prim, r.o, r.d, r.tnear, t) && better(t, lk.z, prim, r.best))
+            {
+            r.best.t = t;
+            r.best.geom = lk.z;
+            r.best.prim = prim;
+            }
+        h0 = false;
+        }
+    if (h1 && (lk.y & LEAF))
+        {
+        const uint32_t prim = lk.y & ~LEAF;
+        float t;
+        if (COUNT)
+            n_prims++;
+        if (leaf_test<TYPES>(geoms[lk.w], prim, r.o, r.d, r.tnear, t) && better(t, lk.w, prim, r.best))
+            {
+            r.best.t = t;
+            r.best.geom = lk.w;
+            r.best.prim = prim;
+            }
+        h1 = false;
+        }
+    h0 = h0 && !(c0 > r.best.t);
+    h1 = h1 && !(c1 > r.best.t);
+    if (h0 && h1)
+        {
+        const bool swap = t1min < t0min;
+        st.push(swap ? lk.x : lk.y, swap ? c0 : c1, err);
+        r.node = swap ? lk.y : lk.x;
+        return true;
+        }
+    if (h0)
+        {
+        r.node = lk.x;
+        return true;
+        }
+    if (h1)
+        {
+        r.node = lk.y;
+        return true;
+        }
+    while (st.sp > 0)
+        {
+        uint32_t n;
+        float tm;
+        st.pop(n, tm);
+        if (!(tm > r.best.t))
+            {
+            r.node = n;
+            return true;
+            }
+        }
+    r.node = EMPTY;
+    return false;
+    }
+
+// One node visit of the closest-hit search: spheres-only kernels search the
+// 4-wide hierarchy, the others the binary one (same result either way).
+template<class Stack, bool COUNT, uint32_t TYPES>
+__device__ __forceinline__ bool trace_visit(const SceneDev& sc, TraceState& r, Stack& st, uint32_t& n_nodes,
+                                            uint32_t& n_prims, uint32_t& err)
+    {
+    if constexpr (TYPES == T_SPHERES)
+        return trace_visit4<Stack, COUNT, TYPES>(sc.nodes4, sc.geoms, r, st, n_nodes, n_prims, err);
+    else
+        return trace_visit2<Stack, COUNT, TYPES>(sc.nodes, sc.geoms, r, st, n_nodes, n_prims, err);
+    }
+
 template<class Stack, bool COUNT, uint32_t TYPES = T_ALL>
 __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear, HitT& best, Stack& st,
                                      uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
     {
     TraceState r;
     trace_begin(sc, o, d, tnear, r, st);
-    while (trace_visit<Stack, COUNT, TYPES>(sc.nodes4, sc.geoms, r, st, n_nodes, n_prims, err))
+    while (trace_visit<Stack, COUNT, TYPES>(sc, r, st, n_nodes, n_prims, err))
         {
         }
     best = r.best;
@@ -1284,7 +1388,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
     bool active = false;
     uint32_t i = 0;
     TraceState r;
-    const Node4* nodes = nullptr;
+    const Node* nodes2 = nullptr;
+    const Node4* nodes4 = nullptr;
     const GeomDev* geoms = nullptr;
     for (;;)
         {
@@ -1317,7 +1422,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
                     job = __float_as_uint(b.w);
                     }
                 const SceneDev& sc = jobs[job].scene;
-                nodes = sc.nodes4;
+                nodes2 = sc.nodes;
+                nodes4 = sc.nodes4;
                 geoms = sc.geoms;
                 trace_begin(sc, o, d, tnear, r, st);
                 active = true;
@@ -1333,7 +1439,15 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
             }
         if (__ballot(active) == 0)
             break;
-        if (active && !trace_visit<LdsStack<SS>, COUNT, TYPES>(nodes, geoms, r, st, n_nodes, n_prims, err))
+        bool more = false;
+        if (active)
+            {
+            if constexpr (TYPES == T_SPHERES)
+                more = trace_visit4<LdsStack<SS>, COUNT, TYPES>(nodes4, geoms, r, st, n_nodes, n_prims, err);
+            else
+                more = trace_visit2<LdsStack<SS>, COUNT, TYPES>(nodes2, geoms, r, st, n_nodes, n_prims, err);
+            }
+        if (active && !more)
             {
             hits[i] = make_uint4(__float_as_uint(r.best.t), r.best.geom, r.best.prim, 0);
             active = false;

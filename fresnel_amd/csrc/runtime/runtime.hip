@@ -171,6 +171,7 @@ struct DeviceObj
     uint32_t primary_tiles = 2;         // primary rays of pinhole/orthographic cameras: 2 tile lists, 1 BVH
     uint32_t fuse_primary = 2;          // 2: primary traversal and depth-0 shading in one kernel, 1: two
     uint32_t fuse_bounce = 1;           // 2: secondary traversal and shading in one kernel, 1: two
+    uint32_t inflight = 2;              // batches of one Path.sample in flight (1 or 2)
     uint32_t trav_refill = 65;          // lanes take new rays once this many of a wavefront are idle
                                         // (65: batch-synchronous kernel, 64 rays at a time)
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
@@ -1582,7 +1583,7 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
     uint32_t done = 0;
     for (;;)
         {
-        while (active.size() < 2)
+        while (active.size() < D.inflight)
             {
             if (pending.empty())
                 {
@@ -2918,6 +2919,15 @@ extern "C"
                         {
                         REQUIRE(value % 64 == 0 && value <= 65536, "trav_chunk must be a multiple of 64, at most 65536");
                         D.trav_chunk = (uint32_t)value;
+                        }
+                    }
+                else if (k == "inflight")
+                    {
+                    prev = D.inflight;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 2, "inflight must be 1 or 2");
+                        D.inflight = (uint32_t)value;
                         }
                     }
                 else if (k == "fuse_bounce")

@@ -265,7 +265,10 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * primary traversal and depth-0 shading in one kernel (2, default) or two (1);
  * "fuse_bounce": the same for each bounce of the secondary rays (default 1:
  * the fused kernel holds 88 registers, and the latency-bound secondary
- * traversal runs faster at the 8 wavefronts per SIMD of the separate kernel).
+ * traversal runs faster at the 8 wavefronts per SIMD of the separate kernel);
+ * "inflight": batches of one render kept in flight on the device's two
+ * render streams (2, default: one batch's kernels run while the host waits
+ * for the other's queue counts) or 1.
  * value <= 0 only reads the current value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
