@@ -1505,21 +1505,10 @@ struct BatchExec
         }
     };
 
-// Run one batch to completion on the device's first scratch and stream.
-static void run_batch(DeviceObj& D, std::vector<JobDev>& jobs, uint32_t types)
-    {
-    BatchExec e(D, D.rs, D.stream, jobs, types);
-    e.start();
-    while (e.step())
-        {
-        }
-    e.accumulate(nullptr, nullptr);
-    HIPCHECK(hipStreamSynchronize(D.stream));
-    }
-
-// Run jobs whose scenes hold spheres only and the other jobs as two batches,
-// so the spheres-only jobs use the spheres-only kernels (fewer registers).
-static void run_grouped(DeviceObj& D, const std::vector<JobDev>& jobs, const std::vector<uint32_t>& types)
+// Jobs whose scenes hold spheres only and the other jobs form two groups, so
+// the spheres-only jobs use the spheres-only kernels (fewer registers).
+static void split_groups(const std::vector<JobDev>& jobs, const std::vector<uint32_t>& types,
+                         std::deque<std::pair<std::vector<JobDev>, uint32_t>>& out)
     {
     std::vector<JobDev> sph, mixed;
     uint32_t mixed_types = 0;
@@ -1534,9 +1523,54 @@ static void run_grouped(DeviceObj& D, const std::vector<JobDev>& jobs, const std
             }
         }
     if (!sph.empty())
-        run_batch(D, sph, T_SPHERES);
+        out.emplace_back(std::move(sph), T_SPHERES);
     if (!mixed.empty())
-        run_batch(D, mixed, mixed_types);
+        out.emplace_back(std::move(mixed), mixed_types);
+    }
+
+// Run batches (job lists with their geometry-type sets) with up to
+// D.inflight of them in flight, one per render scratch and stream: while the
+// host waits for one batch's queue counts the other batch's kernels run, and
+// a batch's tail overlaps the next batch's primary rays.  Accumulations are
+// enqueued in batch order, each after the previous one (event), so every
+// pixel's samples are accumulated in order.  Returns with both streams idle.
+static void run_inflight(DeviceObj& D, std::deque<std::pair<std::vector<JobDev>, uint32_t>>& pending)
+    {
+    struct Slot
+        {
+        std::unique_ptr<BatchExec> e;
+        int slot;
+        };
+    std::deque<Slot> active;
+    HIPCHECK(hipStreamSynchronize(D.stream)); // scene, tile lists and counters ready for both streams
+    hipEvent_t last_acc = nullptr;
+    for (;;)
+        {
+        while (active.size() < D.inflight && !pending.empty())
+            {
+            const int slot = active.empty() ? 0 : 1 - active.front().slot;
+            Slot sl;
+            sl.slot = slot;
+            sl.e.reset(new BatchExec(D, slot ? D.rs2 : D.rs, slot ? D.stream2 : D.stream,
+                                     std::move(pending.front().first), pending.front().second));
+            pending.pop_front();
+            sl.e->start();
+            active.push_back(std::move(sl));
+            }
+        if (active.empty())
+            break;
+        for (auto& a : active)
+            a.e->step();
+        while (!active.empty() && active.front().e->paths_done)
+            {
+            Slot& f = active.front();
+            f.e->accumulate(last_acc, D.acc_done[f.slot]);
+            last_acc = D.acc_done[f.slot];
+            active.pop_front();
+            }
+        }
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream2));
     }
 
 // Render `samples` Path passes for each (scene, tracer) pair.
@@ -1566,75 +1600,22 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
     std::vector<uint32_t> types;
     for (auto* sc : scenes)
         types.push_back(scene_types(*sc));
-    // Two batches in flight, one per render scratch and stream: while the host
-    // waits for one batch's queue counts the other batch's kernels run, and a
-    // batch's tail overlaps the next batch's primary rays.  Accumulations are
-    // enqueued in batch order, each after the previous one (event), so every
-    // pixel's samples are accumulated in order.
-    struct Slot
+    std::deque<std::pair<std::vector<JobDev>, uint32_t>> groups;
+    for (uint32_t done = 0; done < samples;)
         {
-        std::unique_ptr<BatchExec> e;
-        int slot;
-        };
-    std::deque<Slot> active;
-    std::deque<std::pair<std::vector<JobDev>, uint32_t>> pending;
-    HIPCHECK(hipStreamSynchronize(D.stream)); // scene, tile lists and counters ready for both streams
-    hipEvent_t last_acc = nullptr;
-    uint32_t done = 0;
-    for (;;)
-        {
-        while (active.size() < D.inflight)
+        const uint32_t k = std::min(K, samples - done);
+        std::vector<JobDev> jobs;
+        for (size_t j = 0; j < scenes.size(); j++)
             {
-            if (pending.empty())
-                {
-                if (done >= samples)
-                    break;
-                const uint32_t k = std::min(K, samples - done);
-                std::vector<JobDev> sph, mixed;
-                uint32_t mixed_types = 0;
-                for (size_t j = 0; j < scenes.size(); j++)
-                    {
-                    JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
-                    J.K = k;
-                    J.n_base = tracers[j]->n_samples + done;
-                    if ((types[j] & ~T_SPHERES) == 0)
-                        sph.push_back(J);
-                    else
-                        {
-                        mixed.push_back(J);
-                        mixed_types |= types[j];
-                        }
-                    }
-                if (!sph.empty())
-                    pending.emplace_back(std::move(sph), T_SPHERES);
-                if (!mixed.empty())
-                    pending.emplace_back(std::move(mixed), mixed_types);
-                done += k;
-                continue;
-                }
-            const int slot = active.empty() ? 0 : 1 - active.front().slot;
-            Slot sl;
-            sl.slot = slot;
-            sl.e.reset(new BatchExec(D, slot ? D.rs2 : D.rs, slot ? D.stream2 : D.stream,
-                                     std::move(pending.front().first), pending.front().second));
-            pending.pop_front();
-            sl.e->start();
-            active.push_back(std::move(sl));
+            JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
+            J.K = k;
+            J.n_base = tracers[j]->n_samples + done;
+            jobs.push_back(J);
             }
-        if (active.empty())
-            break;
-        for (auto& a : active)
-            a.e->step();
-        while (!active.empty() && active.front().e->paths_done)
-            {
-            Slot& f = active.front();
-            f.e->accumulate(last_acc, D.acc_done[f.slot]);
-            last_acc = D.acc_done[f.slot];
-            active.pop_front();
-            }
+        split_groups(jobs, types, groups);
+        done += k;
         }
-    HIPCHECK(hipStreamSynchronize(D.stream));
-    HIPCHECK(hipStreamSynchronize(D.stream2));
+    run_inflight(D, groups);
     for (auto* tr : tracers)
         tr->n_samples += samples;
     D.end_stats();
@@ -1757,7 +1738,9 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         std::vector<uint32_t> types;
         for (uint64_t id : S.active)
             types.push_back(scene_types(*S.reqs[id].scene));
-        run_grouped(D, jobs, types);
+        std::deque<std::pair<std::vector<JobDev>, uint32_t>> groups;
+        split_groups(jobs, types, groups);
+        run_inflight(D, groups);
         D.end_stats();
         if (D.last_counters[CTR_OVERFLOW])
             throw Error(FH_ERR_STATE, "traversal stack overflow");
