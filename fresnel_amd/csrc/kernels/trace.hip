@@ -1545,11 +1545,25 @@ __global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __
     const JobDev& J = jobs[lo];
     const uint32_t pixel = i - pix_base[lo];
     float4 mean = J.linear[pixel];
-    for (uint32_t k = 0; k < J.K; k++)
+    const uint32_t K = J.K;
+    // the first records of four camera samples are loaded together, then
+    // folded into the mean in sample order
+    constexpr uint32_t U = 4;
+    for (uint32_t k0 = 0; k0 < K; k0 += U)
+    {
+    uint4 rk[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++)
+        rk[u] = (k0 + u < K) ? records[rec_index(J, k0 + u, pixel, 0)] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (uint32_t u = 0; u < U; u++)
+        {
+        const uint32_t k = k0 + u;
+        if (k < K)
         {
         v3 result = mk3(0, 0, 0);
         float a = 1.0f;
-        const uint4 r0 = records[rec_index(J, k, pixel, 0)];
+        const uint4 r0 = rk[u];
         if ((r0.w & 3u) == REC_BG)
             {
             result = J.bg * float(J.L);
@@ -1557,9 +1571,8 @@ __global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __
             }
         else
             {
-            for (uint32_t ls = 0; ls < J.L; ls++)
-                {
-                const uint4 r = ls ? records[rec_index(J, k, pixel, ls)] : r0;
+            auto fold = [&](const uint4 r)
+            {
                 const v3 val = mk3(__uint_as_float(r.x), __uint_as_float(r.y), __uint_as_float(r.z));
                 const uint32_t kind = r.w & 3u;
                 if (kind == REC_ADD)
@@ -1571,7 +1584,10 @@ __global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __
                         if (mask & (1u << li))
                             result = result + (val * J.lights.color[li]) * J.lights.inv_sin[li];
                     }
-                }
+            };
+            fold(r0);
+            for (uint32_t ls = 1; ls < J.L; ls++)
+                fold(records[rec_index(J, k, pixel, ls)]);
             }
         const v3 s = div_rcp(result, float(J.L));
         const float q = 1.0f / float(J.n_base + k + 1);
@@ -1580,6 +1596,8 @@ __global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __
         mean.z = mean.z + (s.z - mean.z) * q;
         mean.w = mean.w + (a - mean.w) * q;
         }
+        }
+    }
     J.linear[pixel] = mean;
     if (!J.highlight || (mean.x <= 1.0f && mean.y <= 1.0f && mean.z <= 1.0f))
         J.srgb[pixel] = to_srgb(mean);
