@@ -157,19 +157,6 @@ __device__ __forceinline__ void tangent_frame(v3 n, v3& t_x, v3& t_y)
     t_y = cross(n, t_x);
     }
 
-// Material::importanceSampleGGX, Material.h:230-256
-__device__ inline v3 sample_ggx(const Mat& m, float xi_x, float xi_y, v3 v, v3 n)
-    {
-    const float alpha = m.roughness * m.roughness;
-    // phi = 2 pi xi_x
-    const float cos_theta = sh_sqrt(sh_div(1.0f - xi_y, 1.0f + (alpha * alpha - 1.0f) * xi_y));
-    const float sin_theta = sh_sqrt(1.0f - cos_theta * cos_theta);
-    const v3 h_t = mk3(sin_theta * sh_cos_rev(xi_x), sin_theta * sh_sin_rev(xi_x), cos_theta);
-    v3 t_x, t_y;
-    tangent_frame(n, t_x, t_y);
-    const v3 h = t_x * h_t.x + t_y * h_t.y + n * h_t.z;
-    return 2.0f * dot(v, h) * h - v;
-    }
 // Material::pdfGGX, Material.h:258-280
 __device__ inline float pdf_ggx(const Mat& m, v3 l, v3 v, v3 n)
     {
@@ -183,16 +170,6 @@ __device__ inline float pdf_ggx(const Mat& m, v3 l, v3 v, v3 n)
     const float D = sh_div(alpha_sq, PI_F * denom_rt * denom_rt);
     const float pdf_l = sh_div(D * ndoth, 4.0f * ldoth);
     return (pdf_l > 0.0f) ? pdf_l : 0.0f;
-    }
-// Material::importanceSampleDiffuse, Material.h:282-304
-__device__ inline v3 sample_diffuse(float xi_x, float xi_y, v3 n)
-    {
-    const float r = sh_sqrt(xi_x);
-    // theta = 2 pi xi_y
-    const v3 v_t = mk3(r * sh_cos_rev(xi_y), r * sh_sin_rev(xi_y), sh_sqrt(1.0f - xi_x));
-    v3 t_x, t_y;
-    tangent_frame(n, t_x, t_y);
-    return t_x * v_t.x + t_y * v_t.y + n * v_t.z;
     }
 // Material::pdfDiffuse, Material.h:306-313
 __device__ __forceinline__ float pdf_diffuse(v3 l, v3 n)
@@ -303,21 +280,28 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
     const bool transmit = (choice_trans <= m.spec_trans);
     if (transmit)
         l = -v;
-    else if (choice_mis <= 0.5f)
-        {
-        l = sample_diffuse(xi_x, xi_y, n);
-        const float pd = pdf_diffuse(l, n);
-        const float pg = pdf_ggx(m, l, v, n);
-        const float w_d = sh_div(pd, pd + pg);
-        factor = sh_div(w_d, 0.5f * pd);
-        }
     else
         {
-        l = sample_ggx(m, xi_x, xi_y, v, n);
+        // Material::importanceSampleDiffuse (Material.h:282-304) or
+        // importanceSampleGGX (Material.h:230-256) in one code path, so a
+        // wavefront whose lanes pick both lobes runs the frame, the angle and
+        // the two pdfs once; each lane performs its own lobe's operations.
+        const bool diffuse = choice_mis <= 0.5f;
+        const float alpha = m.roughness * m.roughness;
+        const float cos_theta = sh_sqrt(sh_div(1.0f - xi_y, 1.0f + (alpha * alpha - 1.0f) * xi_y));
+        const float sin_theta = sh_sqrt(1.0f - cos_theta * cos_theta);
+        const float radial = diffuse ? sh_sqrt(xi_x) : sin_theta;
+        const float angle = diffuse ? xi_y : xi_x;
+        const float zc = diffuse ? sh_sqrt(1.0f - xi_x) : cos_theta;
+        const v3 s_t = mk3(radial * sh_cos_rev(angle), radial * sh_sin_rev(angle), zc);
+        v3 t_x, t_y;
+        tangent_frame(n, t_x, t_y);
+        const v3 w = t_x * s_t.x + t_y * s_t.y + n * s_t.z;
+        l = diffuse ? w : (2.0f * dot(v, w) * w - v);
         const float pd = pdf_diffuse(l, n);
         const float pg = pdf_ggx(m, l, v, n);
-        const float w_g = sh_div(pg, pd + pg);
-        factor = sh_div(w_g, 0.5f * pg);
+        const float p_lobe = diffuse ? pd : pg;
+        factor = sh_div(sh_div(p_lobe, pd + pg), 0.5f * p_lobe);
         }
     if (transmit)
         {
