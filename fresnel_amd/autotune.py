@@ -22,6 +22,7 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
                   two (1).
 * ``fuse_bounce`` the same for each bounce of the secondary rays.
 * ``inflight`` batches of one render in flight on the device's two streams.
+* ``xcd_bands`` queue segments as image bands claimed per XCD (2) or not (1).
 
 `tune` runs a coordinate descent over the candidates on a caller-provided
 scene, timing short renders, and stores the winner in a JSON cache keyed by
@@ -45,6 +46,7 @@ CANDIDATES = {
     'fuse_primary': [1, 2],
     'fuse_bounce': [1, 2],
     'inflight': [1, 2],
+    'xcd_bands': [1, 2],
 }
 
 

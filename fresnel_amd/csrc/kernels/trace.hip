@@ -624,10 +624,55 @@ __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uin
 // Persistent traversal: each wavefront claims `chunk` rays (a multiple of 64)
 // per atomic on the work counter and claims its next chunk before tracing the
 // current one, so the atomic's round trip overlaps the traversal.
+// Work claims of the persistent traversal.  Without bands the wavefronts
+// claim `chunk` rays at a time from [0, n).  With bands the queue's 16
+// segments hold the rays of 16 image bands (k_primary) and the dense range is
+// cut into 8 ranges of two segments, one per XCD: a workgroup starts on the
+// range of the XCD it runs on (workgroups are dispatched to the 8 XCDs in
+// turn) and moves to the next range once its own is exhausted, so each XCD's
+// L2 mostly serves the hierarchy around two bands.  Each range's counter only
+// grows, so every wavefront stops after at most 8 exhausted ranges.
+struct Claims
+    {
+    uint32_t* work; // one counter per range, QSEG_STRIDE words apart
+    uint32_t chunk, n, r, ranges, tried;
+    __device__ __forceinline__ uint32_t lo(const Queue& q, uint32_t k) const
+        {
+        return ranges == 1 ? 0u : q.pre[2 * k];
+        }
+    __device__ __forceinline__ uint32_t hi(const Queue& q, uint32_t k) const
+        {
+        return (ranges == 1 || k == ranges - 1) ? n : q.pre[2 * k + 2];
+        }
+    // next claimed [base, end) of the calling wavefront, or base = EMPTY
+    __device__ __forceinline__ void next(const Queue& q, uint32_t lane, uint32_t& base, uint32_t& end)
+        {
+        while (tried < ranges)
+            {
+            uint32_t c = 0;
+            if (lane == 0)
+                c = atomicAdd(work + r * QSEG_STRIDE, chunk);
+            c = __shfl(c, 0, 64);
+            const uint32_t a = lo(q, r), b = hi(q, r);
+            if (c < b - a)
+                {
+                base = a + c;
+                end = min(b, base + chunk);
+                return;
+                }
+            r = (r + 1 == ranges) ? 0u : r + 1;
+            tried++;
+            }
+        base = EMPTY;
+        end = EMPTY;
+        }
+    };
+
 template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
                                                     uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
-                                                    uint2* gstack, uint32_t gstride, unsigned long long* counters)
+                                                    uint32_t bands, uint2* gstack, uint32_t gstride,
+                                                    unsigned long long* counters)
     {
     __shared__ uint2 lds[SS * BLOCK];
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
@@ -639,16 +684,19 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
     st.sp = 0;
     const uint32_t lane = lane_id();
     uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
-    uint32_t claim = 0;
-    if (lane == 0)
-        claim = atomicAdd(work, chunk);
-    uint32_t base = __shfl(claim, 0, 64);
-    while (base < n)
+    Claims cl;
+    cl.work = work;
+    cl.chunk = chunk;
+    cl.n = n;
+    cl.ranges = (bands && !PRIMARY) ? 8u : 1u;
+    cl.r = (cl.ranges == 8u) ? (blockIdx.x & 7u) : 0u;
+    cl.tried = 0;
+    uint32_t base, end;
+    cl.next(q, lane, base, end);
+    while (base != EMPTY)
         {
-        uint32_t next = 0;
-        if (lane == 0)
-            next = atomicAdd(work, chunk);
-        const uint32_t end = min(n, base + chunk);
+        uint32_t nbase, nend; // the next chunk, claimed before tracing this one
+        cl.next(q, lane, nbase, nend);
         for (uint32_t sub = base; sub < end; sub += 64)
         {
         const uint32_t i = sub + lane;
@@ -681,7 +729,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
                 n_rays++;
             }
         }
-        base = __shfl(next, 0, 64);
+        base = nbase;
+        end = nend;
         }
     if (COUNT)
         {
@@ -1208,8 +1257,9 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
 // generated twice.  Same arithmetic as k_traverse_packet + k_shade_primary.
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t n,
-                                                   uint32_t* work, uint32_t groups, uint2* gstack, uint32_t gstride,
-                                                   Queue out, uint4* records, unsigned long long* counters)
+                                                   uint32_t* work, uint32_t groups, uint32_t bands, uint2* gstack,
+                                                   uint32_t gstride, Queue out, uint4* records,
+                                                   unsigned long long* counters)
     {
     __shared__ uint2 lds[SS * BLOCK];
     __shared__ uint32_t sh[BLOCK / 64 + 1];
@@ -1225,17 +1275,26 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t chunk = BLOCK * groups;
     uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0, n_entries = 0;
+    // Chunk u of n_units goes to queue segment u % QSEG, or with bands to
+    // segment u / G (16 contiguous bands of G chunks, i.e. of the image); the
+    // claims then visit the bands in turn, claim c -> chunk (c % QSEG) G + c /
+    // QSEG, so the chunks in flight still append to all 16 segments.
+    const uint32_t n_units = (n + chunk - 1) / chunk;
+    const uint32_t G = (n_units + QSEG - 1) / QSEG;
+    const uint32_t n_claims = bands ? QSEG * G : n_units;
     if (threadIdx.x == 0)
-        s_claim = atomicAdd(work, chunk);
+        s_claim = atomicAdd(work, 1u);
     __syncthreads();
-    uint32_t base = s_claim;
+    uint32_t c = s_claim;
     __syncthreads();
-    while (base < n)
+    while (c < n_claims)
         {
         if (threadIdx.x == 0)
-            s_claim = atomicAdd(work, chunk); // the next chunk, claimed ahead
-        const uint32_t seg = (base / chunk) % QSEG; // the chunk's output segment
-        for (uint32_t g = 0; g < groups; g++)
+            s_claim = atomicAdd(work, 1u); // the next chunk, claimed ahead
+        const uint32_t u = bands ? (c % QSEG) * G + c / QSEG : c;
+        const uint32_t seg = bands ? u / G : u % QSEG; // the chunk's output segment
+        const uint32_t base = u * chunk;
+        for (uint32_t g = 0; g < groups && u < n_units; g++)
             {
             const uint32_t i = base + (wave * groups + g) * 64 + lane;
             const bool valid = i < n;
@@ -1255,7 +1314,7 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
                                               records, sh);
             }
         __syncthreads();
-        base = s_claim;
+        c = s_claim;
         __syncthreads();
         }
     if (COUNT)

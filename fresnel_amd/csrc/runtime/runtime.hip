@@ -172,6 +172,7 @@ struct DeviceObj
     uint32_t fuse_primary = 2;          // 2: primary traversal and depth-0 shading in one kernel, 1: two
     uint32_t fuse_bounce = 1;           // 2: secondary traversal and shading in one kernel, 1: two
     uint32_t inflight = 2;              // batches of one Path.sample in flight (1 or 2)
+    uint32_t xcd_bands = 2;             // 2: queue segments hold image bands, traversal claims per XCD; 1: off
     uint32_t trav_refill = 65;          // lanes take new rays once this many of a wavefront are idle
                                         // (65: batch-synchronous kernel, 64 rays at a time)
     int stack_lds = 8;                  // traversal stack entries kept in LDS (8 or 16)
@@ -314,8 +315,8 @@ static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const T
     D.launch(nm,
              [&]
              {
-                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.chunk, a.gstack,
-                                                 a.gstride, a.counters);
+                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.chunk,
+                                                 D.xcd_bands == 2 ? 1u : 0u, a.gstack, a.gstride, a.counters);
              });
     }
 
@@ -364,8 +365,9 @@ static void launch_primary(DeviceObj& D, hipStream_t s, const PrimaryArgs& a)
     D.launch("primary",
              [&]
              {
-                 kern<<<grid, PRIMARY_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.n, a.work, a.groups, a.gstack, a.gstride,
-                                                     a.out, a.records, a.counters);
+                 kern<<<grid, PRIMARY_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.n, a.work, a.groups,
+                                                     D.xcd_bands == 2 ? 1u : 0u, a.gstack, a.gstride, a.out,
+                                                     a.records, a.counters);
              });
     }
 
@@ -1350,7 +1352,7 @@ struct BatchExec
 
     void traverse(const Queue& q, uint32_t count, bool primary)
         {
-        HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), s));
+        HIPCHECK(hipMemsetAsync(R.counters32.p, 0, 8 * QSEG_STRIDE * sizeof(uint32_t), s));
         TravArgs a {R.jobs.p, nj, q, count, R.hits.p, R.counters32.p, D.trav_chunk, R.gstack.p, gstride,
                     D.counters.p};
         D.lstream = s;
@@ -1390,7 +1392,7 @@ struct BatchExec
             R.qmeta[q].ensure(cap);
             }
         R.records.ensure(n_paths);
-        R.counters32.ensure(4);
+        R.counters32.ensure(8 * QSEG_STRIDE);
         R.qseg.ensure(QSEG_FLAG + 1);
         R.jobs.ensure(jobs.size());
         R.pix_base.ensure(jobs.size());
@@ -2902,6 +2904,15 @@ extern "C"
                         {
                         REQUIRE(value % 64 == 0 && value <= 65536, "trav_chunk must be a multiple of 64, at most 65536");
                         D.trav_chunk = (uint32_t)value;
+                        }
+                    }
+                else if (k == "xcd_bands")
+                    {
+                    prev = D.xcd_bands;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 2, "xcd_bands must be 1 (off) or 2 (on)");
+                        D.xcd_bands = (uint32_t)value;
                         }
                     }
                 else if (k == "inflight")

@@ -268,7 +268,10 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * traversal runs faster at the 8 wavefronts per SIMD of the separate kernel);
  * "inflight": batches of one render kept in flight on the device's two
  * render streams (2, default: one batch's kernels run while the host waits
- * for the other's queue counts) or 1.
+ * for the other's queue counts) or 1; "xcd_bands": the primary kernel fills
+ * the 16 queue segments with 16 image bands and the secondary traversal's
+ * workgroups claim the rays of their own XCD's two bands first (2, default)
+ * or the segments hold interleaved chunks and claims are global (1).
  * value <= 0 only reads the current value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 

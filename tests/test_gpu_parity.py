@@ -273,14 +273,16 @@ def test_tuning_knobs_do_not_change_image(builder, gpu_device):
     """
     d = gpu_device._device
     keys = ('stack_lds', 'batch_paths', 'tail_rays', 'trav_chunk', 'trav_refill', 'trav_packet', 'primary_tiles',
-            'primary_chunk', 'fuse_primary', 'fuse_bounce', 'inflight')
+            'primary_chunk', 'fuse_primary', 'fuse_bounce', 'inflight', 'xcd_bands')
     saved = {k: d.tune(k, 0) for k in keys}
     settings = [dict(stack_lds=16, batch_paths=1 << 25, tail_rays=32768, trav_chunk=256, trav_refill=65,
-                     trav_packet=1, primary_tiles=1, fuse_primary=1, fuse_bounce=1),
+                     trav_packet=1, primary_tiles=1, fuse_primary=1, fuse_bounce=1, xcd_bands=1),
                 dict(stack_lds=8, batch_paths=4096, tail_rays=1, trav_chunk=64, trav_refill=1, trav_packet=2,
                      primary_tiles=2, primary_chunk=64),
-                dict(stack_lds=8, batch_paths=1 << 20, tail_rays=1 << 20, trav_chunk=4096, trav_refill=32,
-                     trav_packet=2, primary_tiles=1, fuse_primary=2, primary_chunk=256),
+                dict(stack_lds=8, batch_paths=1 << 20, tail_rays=1 << 20, trav_chunk=4096, trav_refill=65,
+                     trav_packet=2, primary_tiles=1, fuse_primary=2, primary_chunk=256, xcd_bands=2),
+                dict(stack_lds=8, batch_paths=2000, tail_rays=64, trav_chunk=64, trav_refill=65,
+                     trav_packet=2, primary_tiles=2, fuse_primary=2, primary_chunk=256, xcd_bands=1),
                 dict(stack_lds=16, batch_paths=10000, tail_rays=100, trav_chunk=192, trav_refill=64,
                      trav_packet=2, primary_tiles=2, fuse_bounce=1, inflight=1),
                 dict(stack_lds=16, batch_paths=1 << 26, tail_rays=100, trav_chunk=64, trav_refill=17,
