@@ -556,9 +556,12 @@ __device__ __forceinline__ uint4 make_rec(v3 c, uint32_t code)
     return make_uint4(__float_as_uint(c.x), __float_as_uint(c.y), __float_as_uint(c.z), code);
     }
 
-__device__ __forceinline__ uint64_t rec_index(const JobDev& J, uint32_t k, uint32_t pixel, uint32_t ls)
+// Termination record of light sample ls of camera sample k at pixel ordinal
+// j (the primary order below): chunk-major, [j / 64][k][j % 64][ls], so the
+// 64 paths of a wavefront's tile are stored and read back contiguously.
+__device__ __forceinline__ uint64_t rec_index(const JobDev& J, uint32_t k, uint32_t j, uint32_t ls)
     {
-    return J.rec_base + ((uint64_t)k * J.npix + pixel) * J.L + ls;
+    return J.rec_base + (((uint64_t)(j >> 6) * J.K + k) * 64u + (j & 63u)) * J.L + ls;
     }
 
 
@@ -573,14 +576,9 @@ __device__ __forceinline__ uint64_t rec_index(const JobDev& J, uint32_t k, uint3
 // wavefront therefore traces one sample of one compact tile, and the rays in
 // flight at any time cover a K times smaller part of the image than in
 // sample-major order.  local in [0, npix K) -> (sample k, pixel px, py).
-__device__ __forceinline__ void primary_order(const JobDev& J, uint32_t local, uint32_t& k, uint32_t& px,
-                                              uint32_t& py)
+// Pixel of ordinal j of the primary order (below).
+__device__ __forceinline__ void ordinal_pixel(const JobDev& J, uint32_t j, uint32_t& px, uint32_t& py)
     {
-    const uint32_t chunk = fdiv(local, J.div_chunk); // local / (64 K)
-    const uint32_t rem = local - chunk * 64u * J.K;
-    const uint32_t size = min(64u, J.npix - chunk * 64u);
-    k = (size == 64u) ? (rem >> 6) : rem / size;
-    const uint32_t j = chunk * 64u + (rem - k * size);
     const uint32_t W8 = J.W & ~7u, H8 = J.H & ~7u;
     if (j < W8 * H8)
         {
@@ -604,20 +602,50 @@ __device__ __forceinline__ void primary_order(const JobDev& J, uint32_t local, u
     px = r - by * J.W;
     }
 
+// Ordinal of pixel (px, py): the inverse of ordinal_pixel.
+__device__ __forceinline__ uint32_t pixel_ordinal(const JobDev& J, uint32_t px, uint32_t py)
+    {
+    const uint32_t W8 = J.W & ~7u, H8 = J.H & ~7u;
+    if (px < W8 && py < H8)
+        return (((py >> 3) * (W8 >> 3) + (px >> 3)) << 6) | ((py & 7u) << 3) | (px & 7u);
+    const uint32_t rw = J.W - W8;
+    if (py < H8)
+        return W8 * H8 + py * rw + (px - W8);
+    return W8 * H8 + rw * H8 + (py - H8) * J.W + px;
+    }
+
+// Ordinal of the pixel with raster index `pixel`.
+__device__ __forceinline__ uint32_t raster_ordinal(const JobDev& J, uint32_t pixel)
+    {
+    const uint32_t py = fdiv(pixel, J.div_w);
+    return pixel_ordinal(J, pixel - py * J.W, py);
+    }
+
+__device__ __forceinline__ void primary_order(const JobDev& J, uint32_t local, uint32_t& k, uint32_t& px,
+                                              uint32_t& py, uint32_t& j)
+    {
+    const uint32_t chunk = fdiv(local, J.div_chunk); // local / (64 K)
+    const uint32_t rem = local - chunk * 64u * J.K;
+    const uint32_t size = min(64u, J.npix - chunk * 64u);
+    k = (size == 64u) ? (rem >> 6) : rem / size;
+    j = chunk * 64u + (rem - k * size);
+    ordinal_pixel(J, j, px, py);
+    }
+
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
-                                            uint32_t& k, uint32_t& px, uint32_t& py, v3& o, v3& d)
+                                            uint32_t& k, uint32_t& px, uint32_t& py, uint32_t& jord, v3& o, v3& d)
     {
     job = find_job(jobs, n_jobs, i);
     const JobDev& J = jobs[job];
-    primary_order(J, i - J.ray_base, k, px, py);
+    primary_order(J, i - J.ray_base, k, px, py, jord);
     camera_ray(J.cam, J.W, J.H, J.xs_off, J.inv_h, J.seed, px, py, J.n_base + k + 1, true, o, d);
     }
 
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
-                                            uint32_t& k, uint32_t& pixel, v3& o, v3& d)
+                                            uint32_t& k, uint32_t& pixel, uint32_t& jord, v3& o, v3& d)
     {
     uint32_t px, py;
-    primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
+    primary_ray(jobs, n_jobs, i, job, k, px, py, jord, o, d);
     pixel = py * jobs[job].W + px;
     }
 
@@ -708,8 +736,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ j
             uint32_t job;
             if (PRIMARY)
                 {
-                uint32_t k, pixel;
-                primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+                uint32_t k, pixel, jord;
+                primary_ray(jobs, n_jobs, i, job, k, pixel, jord, o, d);
                 tnear = 1e-3f;
                 }
             else
@@ -775,7 +803,8 @@ __device__ __forceinline__ void hit_attributes(const SceneDev& sc, const HitT& h
 // thread of the workgroup must call it.
 template<int BS, uint32_t TYPES>
 __device__ __forceinline__ void shade_primary_block(const JobDev* __restrict__ jobs, bool valid, uint32_t job,
-                                                    uint32_t k, uint32_t pixel, v3 o, v3 d, const HitT& h,
+                                                    uint32_t k, uint32_t pixel, uint32_t jord, v3 o, v3 d,
+                                                    const HitT& h,
                                                     const Queue& out, uint32_t seg, uint4* records, uint32_t* sh)
     {
     const float tnear = 1e-3f;
@@ -787,7 +816,7 @@ __device__ __forceinline__ void shade_primary_block(const JobDev* __restrict__ j
         if (h.geom == EMPTY)
             {
             // primary miss: background for the whole camera sample (TracerPathMethods.h:44-49)
-            records[rec_index(J, k, pixel, 0)] = make_rec(mk3(0, 0, 0), REC_BG);
+            records[rec_index(J, k, jord, 0)] = make_rec(mk3(0, 0, 0), REC_BG);
             L = 0;
             }
         }
@@ -810,7 +839,7 @@ __device__ __forceinline__ void shade_primary_block(const JobDev* __restrict__ j
             bool addv;
             cont = path_hit(*m, at, o, d, h.t, pixel, J.seed, 0, (n_sample - 1) * J.L + ls, att, no, nd, add, addv);
             if (!cont)
-                records[rec_index(J, k, pixel, ls)] = make_rec(addv ? add : mk3(0, 0, 0), addv ? REC_ADD : REC_NONE);
+                records[rec_index(J, k, jord, ls)] = make_rec(addv ? add : mk3(0, 0, 0), addv ? REC_ADD : REC_NONE);
             }
         const uint32_t slot = block_append<BS>(cont, out, seg, sh);
         if (cont && slot != EMPTY)
@@ -831,20 +860,20 @@ __global__ __launch_bounds__(BS) void k_shade_primary(const JobDev* __restrict__
     __shared__ uint32_t sh[BS / 64 + 1];
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
     const bool valid = i < n;
-    uint32_t job = 0, pixel = 0, k = 0;
+    uint32_t job = 0, pixel = 0, k = 0, jord = 0;
     HitT h;
     h.geom = EMPTY;
     v3 o = mk3(0, 0, 0), d = mk3(0, 0, 1);
     if (valid)
         {
         // the primary ray is regenerated rather than stored (bit-identical)
-        primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+        primary_ray(jobs, n_jobs, i, job, k, pixel, jord, o, d);
         const uint4 hh = hits[i];
         h.t = __uint_as_float(hh.x);
         h.geom = hh.y;
         h.prim = hh.z;
         }
-    shade_primary_block<BS, TYPES>(jobs, valid, job, k, pixel, o, d, h, out, blockIdx.x % QSEG, records, sh);
+    shade_primary_block<BS, TYPES>(jobs, valid, job, k, pixel, jord, o, d, h, out, blockIdx.x % QSEG, records, sh);
     }
 
 // Shade one hit or miss of a path at depth >= 1.  Returns true when the path
@@ -910,7 +939,7 @@ __global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, Q
             atomicAdd(counters + CTR_CAPPED, 1ull);
             }
         if (!cont)
-            records[rec_index(J, k, mt.y, ls)] = rec;
+            records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] = rec;
         }
     const uint32_t slot = block_append<BS>(cont, out, blockIdx.x % QSEG, sh);
     if (cont && slot != EMPTY)
@@ -1224,8 +1253,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
             uint32_t job = 0, px = 0, py = 0;
             if (valid)
                 {
-                uint32_t k;
-                primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
+                uint32_t k, jord;
+                primary_ray(jobs, n_jobs, i, job, k, px, py, jord, o, d);
                 }
             HitT best;
             primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, n_entries,
@@ -1299,9 +1328,9 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
             const uint32_t i = base + (wave * groups + g) * 64 + lane;
             const bool valid = i < n;
             v3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
-            uint32_t job = 0, k = 0, px = 0, py = 0;
+            uint32_t job = 0, k = 0, px = 0, py = 0, jord = 0;
             if (valid)
-                primary_ray(jobs, n_jobs, i, job, k, px, py, o, d);
+                primary_ray(jobs, n_jobs, i, job, k, px, py, jord, o, d);
             HitT best;
             best.t = __builtin_huge_valf();
             best.geom = EMPTY;
@@ -1310,8 +1339,8 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
                                                 err);
             if (COUNT && valid)
                 n_rays++;
-            shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, o, d, best, out, seg,
-                                              records, sh);
+            shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, jord, o, d, best, out,
+                                              seg, records, sh);
             }
         __syncthreads();
         c = s_claim;
@@ -1391,7 +1420,7 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ job
                     atomicAdd(counters + CTR_CAPPED, 1ull);
                     }
                 if (!cont)
-                    records[rec_index(J, k, mt.y, ls)] = rec;
+                    records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] = rec;
                 }
             const uint32_t slot = wave_append(cont, out, seg);
             if (cont && slot != EMPTY)
@@ -1466,8 +1495,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
                 uint32_t job;
                 if (PRIMARY)
                     {
-                    uint32_t k, pixel;
-                    primary_ray(jobs, n_jobs, i, job, k, pixel, o, d);
+                    uint32_t k, pixel, jord;
+                    primary_ray(jobs, n_jobs, i, job, k, pixel, jord, o, d);
                     tnear = 1e-3f;
                     }
                 else
@@ -1572,7 +1601,7 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
         d = nd;
         depth++;
         }
-    records[rec_index(J, k, mt.y, ls)] = rec;
+    records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] = rec;
     if (COUNT)
         {
         atomicAdd(counters + CTR_TAIL + 0, (unsigned long long)n_rays);
@@ -1602,7 +1631,12 @@ __global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __
             hi = mid;
         }
     const JobDev& J = jobs[lo];
-    const uint32_t pixel = i - pix_base[lo];
+    // thread i handles pixel ordinal j (records are chunk-major, so a
+    // wavefront reads 64 contiguous records per sample)
+    const uint32_t jord = i - pix_base[lo];
+    uint32_t px, py;
+    ordinal_pixel(J, jord, px, py);
+    const uint32_t pixel = py * J.W + px;
     float4 mean = J.linear[pixel];
     const uint32_t K = J.K;
     // the first records of four camera samples are loaded together, then
@@ -1613,7 +1647,7 @@ __global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __
     uint4 rk[U];
 #pragma unroll
     for (uint32_t u = 0; u < U; u++)
-        rk[u] = (k0 + u < K) ? records[rec_index(J, k0 + u, pixel, 0)] : make_uint4(0, 0, 0, 0);
+        rk[u] = (k0 + u < K) ? records[rec_index(J, k0 + u, jord, 0)] : make_uint4(0, 0, 0, 0);
 #pragma unroll
     for (uint32_t u = 0; u < U; u++)
         {
@@ -1646,7 +1680,7 @@ __global__ void k_accumulate(const JobDev* __restrict__ jobs, const uint32_t* __
             };
             fold(r0);
             for (uint32_t ls = 1; ls < J.L; ls++)
-                fold(records[rec_index(J, k, pixel, ls)]);
+                fold(records[rec_index(J, k, jord, ls)]);
             }
         const v3 s = div_rcp(result, float(J.L));
         const float q = 1.0f / float(J.n_base + k + 1);

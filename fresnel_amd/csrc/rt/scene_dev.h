@@ -148,6 +148,7 @@ struct JobDev
     uint32_t ray_base;    // first primary ray of this job in the batch
     FastDiv div_chunk;    // division by 64 K (primary rays per 64-pixel chunk)
     FastDiv div_tiles;    // division by (W / 8) (full 8x8 tiles per row)
+    FastDiv div_w;        // division by W
     uint64_t rec_base;    // first termination record of this job
     double inv_h;         // 1 / H
     float4* linear;       // running mean (h, w) RGBA

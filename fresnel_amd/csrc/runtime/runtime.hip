@@ -1311,6 +1311,7 @@ static void set_order_divisors(JobDev& J)
     {
     J.div_chunk = make_fastdiv(64u * J.K);
     J.div_tiles = make_fastdiv(std::max(1u, J.W >> 3));
+    J.div_w = make_fastdiv(std::max(1u, J.W));
     }
 
 // One batch in flight: every job advances by jobs[j].K camera samples.  The
@@ -1327,7 +1328,7 @@ struct BatchExec
     uint32_t types;
     bool spheres;
     std::vector<uint32_t> pix_base;
-    uint64_t n_pri = 0, n_paths = 0, n_pix = 0;
+    uint64_t n_pri = 0, n_paths = 0, n_pix = 0, n_rec = 0;
     uint32_t nj = 0, gstride = 0, n = 0;
     int cur = 1;
     bool paths_done = false;
@@ -1366,10 +1367,11 @@ struct BatchExec
             {
             set_order_divisors(J);
             J.ray_base = (uint32_t)n_pri;
-            J.rec_base = n_paths;
+            J.rec_base = n_rec;
             pix_base.push_back((uint32_t)n_pix);
             n_pri += (uint64_t)J.npix * J.K;
             n_paths += (uint64_t)J.npix * J.K * J.L;
+            n_rec += (uint64_t)((J.npix + 63) / 64) * 64 * J.K * J.L; // whole 64-pixel chunks
             n_pix += J.npix;
             }
         REQUIRE(n_pri < (1ull << 31) && n_paths < (1ull << 31), "batch too large");
@@ -1391,7 +1393,7 @@ struct BatchExec
             R.qatt[q].ensure(cap);
             R.qmeta[q].ensure(cap);
             }
-        R.records.ensure(n_paths);
+        R.records.ensure(n_rec);
         R.counters32.ensure(8 * QSEG_STRIDE);
         R.qseg.ensure(QSEG_FLAG + 1);
         R.jobs.ensure(jobs.size());
