@@ -507,14 +507,15 @@ __global__ void k_bvh4_fill(const Node* __restrict__ nodes, uint32_t n, const ui
         slot[m] = 0;
         }
     Node4 r;
-    r.lox = make_float4(box[0][0], box[1][0], box[2][0], box[3][0]);
-    r.loy = make_float4(box[0][1], box[1][1], box[2][1], box[3][1]);
-    r.loz = make_float4(box[0][2], box[1][2], box[2][2], box[3][2]);
-    r.hix = make_float4(box[0][3], box[1][3], box[2][3], box[3][3]);
-    r.hiy = make_float4(box[0][4], box[1][4], box[2][4], box[3][4]);
-    r.hiz = make_float4(box[0][5], box[1][5], box[2][5], box[3][5]);
-    r.child = make_uint4(ref[0], ref[1], ref[2], ref[3]);
-    r.slot = make_uint4(slot[0], slot[1], slot[2], slot[3]);
+    for (int q = 0; q < 2; q++)
+        {
+        const float* b0 = box[2 * q];
+        const float* b1 = box[2 * q + 1];
+        r.pair[q].a_lo = make_float4(b0[0], b0[1], b0[2], b0[3]);
+        r.pair[q].a_hi = make_float4(b0[4], b0[5], b1[0], b1[1]);
+        r.pair[q].b_box = make_float4(b1[2], b1[3], b1[4], b1[5]);
+        r.pair[q].link = make_uint4(ref[2 * q], ref[2 * q + 1], slot[2 * q], slot[2 * q + 1]);
+        }
     out[idx4[i]] = r;
     }
 

@@ -122,7 +122,6 @@ __device__ __forceinline__ bool leaf_test(const GeomDev& g, uint32_t prim, v3 o,
     }
 
 typedef const Node __attribute__((address_space(1))) GlobalNode;
-typedef const Node4 __attribute__((address_space(1))) GlobalNode4;
 
 // Closest-hit traversal state of one ray.
 struct TraceState
@@ -197,61 +196,70 @@ __device__ __forceinline__ bool trace_visit4(const Node4* __restrict__ nodes, co
     {
     if (r.node == EMPTY)
         return false;
-    // global address space: direct global loads instead of flat ones
-    const GlobalNode4* np = (const GlobalNode4*)(nodes + r.node);
-    const float lx[4] = {np->lox.x, np->lox.y, np->lox.z, np->lox.w};
-    const float ly[4] = {np->loy.x, np->loy.y, np->loy.z, np->loy.w};
-    const float lz[4] = {np->loz.x, np->loz.y, np->loz.z, np->loz.w};
-    const float hx[4] = {np->hix.x, np->hix.y, np->hix.z, np->hix.w};
-    const float hy[4] = {np->hiy.x, np->hiy.y, np->hiy.z, np->hiy.w};
-    const float hz[4] = {np->hiz.x, np->hiz.y, np->hiz.z, np->hiz.w};
-    const uint32_t ch[4] = {np->child.x, np->child.y, np->child.z, np->child.w};
-    const uint32_t sl[4] = {np->slot.x, np->slot.y, np->slot.z, np->slot.w};
     if (COUNT)
         n_nodes++;
     // spheres-only kernels always see sphere leaves (checked on the host)
     const bool sph = (TYPES == T_SPHERES) || r.sph;
     float tmin[4], cb[4];
     bool h[4];
+    uint32_t ch[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int q = 0; q < 2; q++)
         {
-        // a sphere leaf's box: centre -+ radius, as the builder bounds it
-        // (GeometrySphere.cc:60-73)
-        const bool sl_k = sph && (ch[k] & LEAF);
-        const float rr = hx[k];
-        const float x0 = sl_k ? lx[k] - rr : lx[k], x1 = sl_k ? lx[k] + rr : hx[k];
-        const float y0 = sl_k ? ly[k] - rr : ly[k], y1 = sl_k ? ly[k] + rr : hy[k];
-        const float z0 = sl_k ? lz[k] - rr : lz[k], z1 = sl_k ? lz[k] + rr : hz[k];
-        float tmax;
-        slab(r.o, r.inv, x0, y0, z0, x1, y1, z1, tmin[k], tmax);
-        cb[k] = cull_bound(tmin[k], tmax, r.margin);
-        h[k] = (tmin[k] <= tmax) && (tmax >= r.tnear) && !(cb[k] > r.best.t);
-        }
+        // one child pair: global address space, direct global loads
+        const GlobalNode* np = (const GlobalNode*)(&nodes[r.node].pair[q]);
+        const float4 A = make_float4(np->a_lo.x, np->a_lo.y, np->a_lo.z, np->a_lo.w);
+        const float4 B = make_float4(np->a_hi.x, np->a_hi.y, np->a_hi.z, np->a_hi.w);
+        const float4 C = make_float4(np->b_box.x, np->b_box.y, np->b_box.z, np->b_box.w);
+        const uint4 lk = make_uint4(np->link.x, np->link.y, np->link.z, np->link.w);
+        const float bx[2][6] = {{A.x, A.y, A.z, A.w, B.x, B.y}, {B.z, B.w, C.x, C.y, C.z, C.w}};
+        const uint32_t ref[2] = {lk.x, lk.y}, slot[2] = {lk.z, lk.w};
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-        if (h[k] && (ch[k] & LEAF))
+        for (int c = 0; c < 2; c++)
             {
-            if (sph)
-                {
-                // the sphere is in the node: no memory access
-                const uint32_t prim = ch[k] & ~LEAF;
-                float t, de;
-                bool first;
-                if (COUNT)
-                    n_prims++;
-                if (sphere_t(r.o, r.d, mk3(lx[k], ly[k], lz[k]), hx[k], t, de, first) && (r.tnear < t)
-                    && better(t, sl[k], prim, r.best))
-                    {
-                    r.best.t = t;
-                    r.best.geom = sl[k];
-                    r.best.prim = prim;
-                    }
-                }
-            else
-                visit_leaf<COUNT, TYPES>(geoms, ch[k], sl[k], r, n_prims);
-            h[k] = false;
+            const int k = 2 * q + c;
+            // a sphere leaf's box: centre -+ radius, as the builder bounds it
+            // (GeometrySphere.cc:60-73)
+            const bool sl_k = sph && (ref[c] & LEAF);
+            const float rr = bx[c][3];
+            const float x0 = sl_k ? bx[c][0] - rr : bx[c][0], x1 = sl_k ? bx[c][0] + rr : bx[c][3];
+            const float y0 = sl_k ? bx[c][1] - rr : bx[c][1], y1 = sl_k ? bx[c][1] + rr : bx[c][4];
+            const float z0 = sl_k ? bx[c][2] - rr : bx[c][2], z1 = sl_k ? bx[c][2] + rr : bx[c][5];
+            float tmax;
+            slab(r.o, r.inv, x0, y0, z0, x1, y1, z1, tmin[k], tmax);
+            cb[k] = cull_bound(tmin[k], tmax, r.margin);
+            h[k] = (tmin[k] <= tmax) && (tmax >= r.tnear) && !(cb[k] > r.best.t);
+            ch[k] = ref[c];
             }
+        // leaf children of this pair
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+            {
+            const int k = 2 * q + c;
+            if (h[k] && (ref[c] & LEAF))
+                {
+                if (sph)
+                    {
+                    // the sphere is in the node: no memory access
+                    const uint32_t prim = ref[c] & ~LEAF;
+                    float t, de;
+                    bool first;
+                    if (COUNT)
+                        n_prims++;
+                    if (sphere_t(r.o, r.d, mk3(bx[c][0], bx[c][1], bx[c][2]), bx[c][3], t, de, first)
+                        && (r.tnear < t) && better(t, slot[c], prim, r.best))
+                        {
+                        r.best.t = t;
+                        r.best.geom = slot[c];
+                        r.best.prim = prim;
+                        }
+                    }
+                else
+                    visit_leaf<COUNT, TYPES>(geoms, ref[c], slot[c], r, n_prims);
+                h[k] = false;
+                }
+            }
+        }
     // inner children still open, nearest entry first
     float key[4];
     uint32_t nd[4];
@@ -696,8 +704,10 @@ struct Claims
         }
     };
 
+// The spheres-only variant is held to 71 registers (7 wavefronts per SIMD, no
+// spills): its search is latency-bound, so wavefronts in flight matter.
 template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
-__global__ __launch_bounds__(BLOCK) void k_traverse(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TYPES == T_SPHERES ? 7 : 1))) void k_traverse(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
                                                     uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
                                                     uint32_t bands, uint2* gstack, uint32_t gstride,
                                                     unsigned long long* counters)

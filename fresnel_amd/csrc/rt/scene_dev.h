@@ -78,17 +78,16 @@ struct __attribute__((aligned(64))) Node
     uint4 link;   // c0, c1, g0, g1
     };
 
-// 4-wide node (128 bytes): child k's box is (lox[k], loy[k], loz[k]) -
-// (hix[k], hiy[k], hiz[k]); child[k] is a 4-wide node index, LEAF | local
-// primitive index (geometry slot in slot[k]) or EMPTY with a NaN box.  In a
-// spheres-only scene (SceneDev::sphere_leaves) a leaf child holds its sphere
-// instead, centre (lox, loy, loz)[k] and radius hix[k]; its box is
-// centre -+ radius, the builder's bounds.
+// 4-wide node (128 bytes): two child pairs, each laid out as a binary node
+// (children 2p and 2p + 1 of pair p: boxes, references, geometry slots), so a
+// search step loads and tests one pair at a time.  A child reference is a
+// 4-wide node index, LEAF | local primitive index (geometry slot alongside)
+// or EMPTY with a NaN box.  In a spheres-only scene (SceneDev::sphere_leaves)
+// a leaf child holds its sphere instead of its box, centre in lo.xyz and
+// radius in hi.x; its box is centre -+ radius, the builder's bounds.
 struct __attribute__((aligned(128))) Node4
     {
-    float4 lox, loy, loz, hix, hiy, hiz;
-    uint4 child;
-    uint4 slot;
+    Node pair[2];
     };
 
 struct SceneDev
