@@ -1669,9 +1669,26 @@ struct RequestObj
     SceneObj* scene = nullptr;
     TracerObj* tracer = nullptr;
     uint32_t samples = 0, done = 0, L = 1;
+    uint32_t scheduled = 0; // samples planned into batches (done <= scheduled <= samples)
     double t_arrival = 0, t_admit = -1, t_done = -1;
     uint64_t footprint = 0;
     int state = 0; // 0 queued, 1 active, 2 done, 3 rejected
+    };
+
+// A batch of the server in flight: its requests and the samples it adds.
+struct ServerBatch
+    {
+    std::unique_ptr<BatchExec> e;
+    int slot = -1; // render scratch and stream of the server, -1 while pending
+    std::vector<JobDev> jobs;
+    uint32_t types = 0;
+    std::vector<std::pair<uint64_t, uint32_t>> adds; // (request, samples)
+    bool accumulated = false;
+    uint64_t serial = 0;
+    int kind() const
+        {
+        return types == T_SPHERES ? 0 : 1;
+        }
     };
 
 struct ServerObj
@@ -1684,6 +1701,19 @@ struct ServerObj
     uint64_t next_id = 1;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     uint64_t iterations = 0;
+    // two batches in flight on the server's own render scratches and streams,
+    // batches in plan order (front: oldest); accumulations chained by events
+    RenderScratch rs[2];
+    hipStream_t st[2] = {nullptr, nullptr};
+    hipEvent_t acc[2] = {nullptr, nullptr};
+    std::deque<ServerBatch> batches;
+    // last accumulation enqueued per batch kind (spheres-only / mixed): a
+    // request's batches are all of one kind, so only batches of one kind
+    // share tracers and need ordered accumulation
+    hipEvent_t last_acc[2] = {nullptr, nullptr};
+    uint64_t last_acc_batch[2] = {0, 0};
+    uint64_t next_serial = 1;
+    bool draining = false; // destruction: finish started batches, plan and start no more
     double clock(double now) const
         {
         if (now >= 0)
@@ -1692,7 +1722,99 @@ struct ServerObj
         }
     };
 
-// One server iteration: admit, advance every active request, retire.
+// Plan the next server step: every active request with unplanned samples
+// gets its share of target_paths (policy_allocate); the step's spheres-only
+// and mixed jobs become one or two batches.
+static void server_plan(ServerObj& S)
+    {
+    DeviceObj& D = *S.dev;
+    std::vector<PolicyItem> items;
+    std::vector<uint64_t> ids;
+    for (uint64_t id : S.active)
+        {
+        const RequestObj& r = S.reqs[id];
+        if (r.scheduled < r.samples)
+            {
+            items.push_back(PolicyItem {id, r.footprint, (uint64_t)r.tracer->W * r.tracer->H * r.L,
+                                        r.samples - r.scheduled});
+            ids.push_back(id);
+            }
+        }
+    if (items.empty())
+        return;
+    const std::vector<uint32_t> k = policy_allocate(items, S.target_paths);
+    std::vector<JobDev> sph, mixed;
+    std::vector<std::pair<uint64_t, uint32_t>> add_sph, add_mixed;
+    uint32_t mixed_types = 0;
+    for (size_t i = 0; i < ids.size(); i++)
+        {
+        if (k[i] == 0)
+            continue;
+        RequestObj& r = S.reqs[ids[i]];
+        update_tiles(D, *r.scene, *r.tracer, make_cam(r.scene->camera));
+        JobDev J = make_job(*r.scene, *r.tracer, r.scene->camera);
+        J.K = k[i];
+        J.n_base = r.tracer->n_samples; // samples planned before this batch
+        r.tracer->n_samples += k[i];
+        r.scheduled += k[i];
+        const uint32_t t = scene_types(*r.scene);
+        if ((t & ~T_SPHERES) == 0)
+            {
+            sph.push_back(J);
+            add_sph.push_back({ids[i], k[i]});
+            }
+        else
+            {
+            mixed.push_back(J);
+            add_mixed.push_back({ids[i], k[i]});
+            mixed_types |= t;
+            }
+        }
+    if (!sph.empty())
+        {
+        ServerBatch b;
+        b.jobs = std::move(sph);
+        b.types = T_SPHERES;
+        b.adds = std::move(add_sph);
+        b.serial = S.next_serial++;
+        S.batches.push_back(std::move(b));
+        }
+    if (!mixed.empty())
+        {
+        ServerBatch b;
+        b.jobs = std::move(mixed);
+        b.types = mixed_types;
+        b.adds = std::move(add_mixed);
+        b.serial = S.next_serial++;
+        S.batches.push_back(std::move(b));
+        }
+    S.iterations++;
+    }
+
+// Retire the requests whose planned samples have all been accumulated.
+static void server_retire(ServerObj& S, double now)
+    {
+    std::vector<uint64_t> still;
+    for (uint64_t id : S.active)
+        {
+        RequestObj& r = S.reqs[id];
+        if (r.done >= r.samples)
+            {
+            r.state = 2;
+            r.t_done = S.clock(now);
+            S.in_use -= r.footprint;
+            r.tracer->L = 1; // as Path.sample leaves it (fresnel/tracer.py:259-261)
+            }
+        else
+            still.push_back(id);
+        }
+    S.active.swap(still);
+    }
+
+// One server iteration: admit; plan a step when no planned batch is waiting;
+// start planned batches on free slots; advance every batch in flight by one
+// host round trip; accumulate finished batches in plan order and retire the
+// requests they complete.  Work stays in flight between calls.
 static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& n_queued)
     {
     DeviceObj& D = *S.dev;
@@ -1704,7 +1826,6 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         r.state = 3;
         r.t_done = S.clock(now);
         }
-    {
     std::lock_guard<std::mutex> lock(D.mu);
     for (auto& it : admitted)
         {
@@ -1714,61 +1835,111 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         sync_scene(*r.scene); // BVH build and uploads happen at admission
         S.active.push_back(it.id);
         }
-    }
-    if (!S.active.empty())
+    if (!S.st[0])
+        for (int k = 0; k < 2; k++)
+            {
+            HIPCHECK(hipStreamCreateWithFlags(&S.st[k], hipStreamNonBlocking));
+            HIPCHECK(hipEventCreateWithFlags(&S.acc[k], hipEventDisableTiming));
+            }
+    if (S.draining)
+        while (!S.batches.empty() && S.batches.back().slot < 0)
+            S.batches.pop_back();
+    bool waiting = false;
+    for (auto& b : S.batches)
+        waiting = waiting || b.slot < 0;
+    if (!waiting && !S.draining)
+        server_plan(S);
+    // start waiting batches on free slots (scene, tile and output work on the
+    // device stream completes first)
+    bool used[2] = {false, false};
+    for (auto& b : S.batches)
+        if (b.slot >= 0)
+            used[b.slot] = true;
+    bool synced = false;
+    for (auto& b : S.batches)
         {
-        std::vector<PolicyItem> items;
-        for (uint64_t id : S.active)
+        if (b.slot >= 0)
+            continue;
+        const int slot = !used[0] ? 0 : (!used[1] ? 1 : -1);
+        if (slot < 0)
+            break;
+        if (!synced)
             {
-            const RequestObj& r = S.reqs[id];
-            items.push_back(PolicyItem {id, r.footprint, (uint64_t)r.tracer->W * r.tracer->H * r.L, r.samples - r.done});
+            HIPCHECK(hipStreamSynchronize(D.stream));
+            synced = true;
             }
-        std::vector<uint32_t> k = policy_allocate(items, S.target_paths);
-        std::lock_guard<std::mutex> lock(D.mu);
-        D.begin_stats();
-        for (uint64_t id : S.active)
-            {
-            RequestObj& r = S.reqs[id];
-            update_tiles(D, *r.scene, *r.tracer, make_cam(r.scene->camera));
-            }
-        std::vector<JobDev> jobs;
-        for (size_t i = 0; i < S.active.size(); i++)
-            {
-            RequestObj& r = S.reqs[S.active[i]];
-            JobDev J = make_job(*r.scene, *r.tracer, r.scene->camera);
-            J.K = k[i];
-            jobs.push_back(J);
-            }
-        std::vector<uint32_t> types;
-        for (uint64_t id : S.active)
-            types.push_back(scene_types(*S.reqs[id].scene));
-        std::deque<std::pair<std::vector<JobDev>, uint32_t>> groups;
-        split_groups(jobs, types, groups);
-        run_inflight(D, groups);
-        D.end_stats();
-        if (D.last_counters[CTR_OVERFLOW])
-            throw Error(FH_ERR_STATE, "traversal stack overflow");
-        S.iterations++;
-        std::vector<uint64_t> still;
-        for (size_t i = 0; i < S.active.size(); i++)
-            {
-            RequestObj& r = S.reqs[S.active[i]];
-            r.tracer->n_samples += k[i];
-            r.done += k[i];
-            if (r.done >= r.samples)
-                {
-                r.state = 2;
-                r.t_done = S.clock(now);
-                S.in_use -= r.footprint;
-                r.tracer->L = 1; // as Path.sample leaves it (fresnel/tracer.py:259-261)
-                }
-            else
-                still.push_back(S.active[i]);
-            }
-        S.active.swap(still);
+        b.slot = slot;
+        used[slot] = true;
+        b.e.reset(new BatchExec(D, S.rs[slot], S.st[slot], b.jobs, b.types));
+        b.e->start();
         }
+    for (auto& b : S.batches)
+        if (b.slot >= 0 && !b.accumulated)
+            b.e->step();
+    // accumulate each finished batch after the earlier batches of its kind
+    bool pending_kind[2] = {false, false};
+    for (auto& b : S.batches)
+        {
+        const int kd = b.kind();
+        if (b.slot >= 0 && !b.accumulated && b.e->paths_done && !pending_kind[kd])
+            {
+            b.e->accumulate(S.last_acc[kd], S.acc[b.slot]);
+            S.last_acc[kd] = S.acc[b.slot];
+            S.last_acc_batch[kd] = b.serial;
+            b.accumulated = true;
+            }
+        if (!b.accumulated)
+            pending_kind[kd] = true;
+        }
+    // retire batches whose accumulation has finished; wait for one only when
+    // no batch is left to advance
+    bool any_running = false;
+    for (auto& b : S.batches)
+        any_running = any_running || (b.slot >= 0 && !b.accumulated);
+    for (size_t i = 0; i < S.batches.size();)
+        {
+        ServerBatch& b = S.batches[i];
+        if (!b.accumulated)
+            {
+            i++;
+            continue;
+            }
+        if (hipEventQuery(S.acc[b.slot]) == hipErrorNotReady)
+            {
+            if (any_running)
+                {
+                i++;
+                continue;
+                }
+            HIPCHECK(hipEventSynchronize(S.acc[b.slot]));
+            }
+        uint64_t overflow = 0;
+        HIPCHECK(hipMemcpyAsync(&overflow, D.counters.p + CTR_OVERFLOW, sizeof(overflow), hipMemcpyDeviceToHost,
+                                S.st[b.slot]));
+        HIPCHECK(hipStreamSynchronize(S.st[b.slot]));
+        if (overflow)
+            throw Error(FH_ERR_STATE, "traversal stack overflow");
+        for (auto& a : b.adds)
+            S.reqs[a.first].done += a.second;
+        const int kd = b.kind();
+        if (S.last_acc_batch[kd] == b.serial)
+            S.last_acc[kd] = nullptr; // finished: later batches of its kind need not wait
+        S.batches.erase(S.batches.begin() + (std::ptrdiff_t)i);
+        }
+    server_retire(S, now);
     n_active = (uint32_t)S.active.size();
     n_queued = (uint32_t)S.queue.size();
+    }
+
+// Finish every batch in flight (server destruction).
+static void server_drain(ServerObj& S)
+    {
+    S.draining = true;
+    while (!S.batches.empty())
+        {
+        uint32_t a = 0, q = 0;
+        server_step(S, -1.0, a, q);
+        }
     }
     } // namespace fh
 
@@ -2695,7 +2866,33 @@ extern "C"
         }
     int fh_server_destroy(fh_server srv)
         {
-        return guard([&] { delete srv; });
+        return guard(
+            [&]
+            {
+                if (!srv)
+                    return;
+                ServerObj& S = srv->obj;
+                try
+                    {
+                    server_drain(S);
+                    }
+                catch (...)
+                    {
+                    }
+                for (int k = 0; k < 2; k++)
+                    {
+                    if (S.st[k])
+                        (void)hipStreamSynchronize(S.st[k]);
+                    if (S.acc[k])
+                        (void)hipEventDestroy(S.acc[k]);
+                    }
+                hipStream_t st0 = S.st[0], st1 = S.st[1];
+                delete srv;
+                if (st0)
+                    (void)hipStreamDestroy(st0);
+                if (st1)
+                    (void)hipStreamDestroy(st1);
+            });
         }
     int fh_server_submit(fh_server srv, fh_scene scene, fh_tracer tr, uint32_t samples, uint32_t light_samples,
                          double now, uint64_t* id)

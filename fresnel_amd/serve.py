@@ -6,10 +6,11 @@ left in ``tracer.output`` / ``tracer.linear_output``, is bit-identical to
 ``tracer.sample(scene, samples, light_samples=light_samples)``.
 
 Each `Server.step` admits queued requests first-come first-served while their
-HBM footprints fit the budget, renders one wavefront batch in which every
-active request advances by some camera samples (an equal share of
-``target_paths`` light-sample paths, at least one sample), and retires the
-requests that are complete.
+HBM footprints fit the budget, plans a wavefront batch in which every active
+request advances by some camera samples (an equal share of ``target_paths``
+light-sample paths, at least one sample) when none is waiting, keeps up to two
+batches in flight on the device (one's kernels run while the host serves the
+other), and retires the requests whose samples have all been accumulated.
 """
 
 QUEUED, ACTIVE, DONE, REJECTED = 0, 1, 2, 3

@@ -217,8 +217,12 @@ int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t s
  * counterpart).  A server keeps admitted requests' scenes (BVH, primitives)
  * and accumulation buffers resident and advances all active requests together:
  * each fh_server_step admits queued requests first-come first-served while
- * their HBM footprints fit the budget, then renders one batch in which every
- * active request takes some camera samples, then retires finished requests.
+ * their HBM footprints fit the budget, plans a step (every active request
+ * takes some camera samples) when none is waiting, keeps up to two planned
+ * batches in flight on the server's own two render scratches and streams,
+ * advances each by one host round trip, and retires requests whose samples
+ * have all been accumulated.  Work stays in flight between calls;
+ * fh_server_destroy finishes the batches already started.
  * A request's image is bit-identical to Path.sample(scene, samples,
  * light_samples) on its own: submission resets the tracer like sample()'s
  * reset, and scheduling never changes per-path arithmetic.
