@@ -86,6 +86,7 @@ struct Array
     {
     std::function<void()> on_map;
     std::function<py::buffer_info()> info;
+    std::function<void(uint64_t, uint64_t)> on_rows;
     std::shared_ptr<void> owner;
     void map()
         {
@@ -93,6 +94,13 @@ struct Array
             on_map();
         }
     void unmap() { }
+    // rows [first, first + count) of a geometry buffer were written
+    void update_rows(uint64_t first, uint64_t count)
+        {
+        if (!on_rows)
+            throw std::runtime_error("not a geometry buffer");
+        on_rows(first, count);
+        }
     };
 
 struct Geometry
@@ -169,6 +177,8 @@ static std::shared_ptr<Array> geometry_buffer(std::shared_ptr<Geometry> g, int w
         }
     a->info = [data, shp, strides]()
     { return py::buffer_info(data, sizeof(float), py::format_descriptor<float>::format(), (py::ssize_t)shp.size(), shp, strides); };
+    Geometry* gp = g.get();
+    a->on_rows = [gp, which](uint64_t first, uint64_t count) { check(fh_geometry_update_rows(gp->h, which, first, count)); };
     return a;
     }
 
@@ -552,7 +562,8 @@ PYBIND11_MODULE(_gpu, m)
     py::class_<Array, std::shared_ptr<Array>>(m, "Array", py::buffer_protocol())
         .def_buffer([](Array& a) { return a.info(); })
         .def("map", &Array::map)
-        .def("unmap", &Array::unmap);
+        .def("unmap", &Array::unmap)
+        .def("update_rows", &Array::update_rows);
 
     py::class_<Geometry, std::shared_ptr<Geometry>>(m, "Geometry")
         .def("getMaterial", &Geometry::getMaterial)
@@ -605,6 +616,13 @@ PYBIND11_MODULE(_gpu, m)
         .def("resize", &Tracer::resize)
         .def("getSRGBOutputBuffer", [](std::shared_ptr<Tracer> t) { return output_buffer(t, FH_OUT_SRGB); })
         .def("getLinearOutputBuffer", [](std::shared_ptr<Tracer> t) { return output_buffer(t, FH_OUT_LINEAR); })
+        .def("histogram",
+             [](std::shared_ptr<Tracer> t)
+             {
+                 py::array_t<uint32_t> out({(py::ssize_t)512, (py::ssize_t)4});
+                 check(fh_tracer_histogram(t->h, out.mutable_data()));
+                 return out;
+             })
         .def("enableHighlightWarning", &Tracer::enableHighlightWarning)
         .def("disableHighlightWarning", &Tracer::disableHighlightWarning)
         .def("getSeed", &Tracer::getSeed)

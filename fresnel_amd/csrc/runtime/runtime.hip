@@ -21,6 +21,7 @@
 #include "fresnel_hip.h"
 #include "kernels/bvh.hip"
 #include "kernels/trace.hip"
+#include "kernels/image.hip"
 #include "runtime/serve_policy.h"
 
 namespace fh
@@ -509,7 +510,12 @@ struct GeometryObj
     // device copies
     DBuf<float4> a, b, aux0, aux1;
     DBuf<float> dcolor, daux_color;
-    bool dirty = true;
+    bool dirty = true; // upload everything on the next sync
+    // rows changed since the last upload, [lo, hi) along the buffers' first
+    // axis: those that feed the hierarchy (positions, radii, points,
+    // orientations, angles) and colour rows (shading only)
+    uint64_t shape_lo = UINT64_MAX, shape_hi = 0;
+    uint64_t color_lo = UINT64_MAX, color_hi = 0;
     bool aux_uploaded = false;
     };
 
@@ -594,53 +600,96 @@ static void upload(DBuf<float>& d, const std::vector<float>& h, hipStream_t s)
         HIPCHECK(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s));
     }
 
-// Pack one geometry's host buffers into the device layout of GeomDev.
-static void upload_geometry(GeometryObj& g, hipStream_t s)
+// Pack rows [lo, hi) of one geometry's host buffers into the device layout of
+// GeomDev (a[i], b[i] per primitive, or per instance for a mesh).
+static void pack_rows(const GeometryObj& g, uint64_t lo, uint64_t hi, std::vector<float4>& a,
+                      std::vector<float4>& b)
     {
-    std::vector<float4> a, b;
-    const uint32_t N = g.N;
-    switch (g.type)
+    const size_t n = (size_t)(hi - lo);
+    a.resize(n);
+    if (g.type != G_SPHERE)
+        b.resize(n);
+    for (size_t k = 0; k < n; k++)
         {
-    case G_SPHERE:
-        a.resize(N);
-        for (uint32_t i = 0; i < N; i++)
-            a[i] = make_float4(g.position[3 * i], g.position[3 * i + 1], g.position[3 * i + 2], g.radius_buf[i]);
-        break;
-    case G_CYLINDER:
-        a.resize(N);
-        b.resize(N);
-        for (uint32_t i = 0; i < N; i++)
+        const size_t i = (size_t)lo + k;
+        switch (g.type)
+            {
+        case G_SPHERE:
+            a[k] = make_float4(g.position[3 * i], g.position[3 * i + 1], g.position[3 * i + 2], g.radius_buf[i]);
+            break;
+        case G_CYLINDER:
             {
             const float* p = &g.points[6 * i];
-            a[i] = make_float4(p[0], p[1], p[2], g.radius_buf[i]);
-            b[i] = make_float4(p[3], p[4], p[5], 0.0f);
+            a[k] = make_float4(p[0], p[1], p[2], g.radius_buf[i]);
+            b[k] = make_float4(p[3], p[4], p[5], 0.0f);
+            break;
             }
-        break;
-    case G_POLYHEDRON:
-    case G_MESH:
-        a.resize(N);
-        b.resize(N);
-        for (uint32_t i = 0; i < N; i++)
+        case G_POLYHEDRON:
+        case G_MESH:
             {
-            a[i] = make_float4(g.position[3 * i], g.position[3 * i + 1], g.position[3 * i + 2], 0.0f);
+            a[k] = make_float4(g.position[3 * i], g.position[3 * i + 1], g.position[3 * i + 2], 0.0f);
             const float* q = &g.orientation[4 * i];
-            b[i] = make_float4(q[0], q[1], q[2], q[3]);
+            b[k] = make_float4(q[0], q[1], q[2], q[3]);
+            break;
             }
-        break;
-    case G_POLYGON:
-        a.resize(N);
-        b.resize(N);
-        for (uint32_t i = 0; i < N; i++)
+        case G_POLYGON:
             {
-            a[i] = make_float4(g.position[2 * i], g.position[2 * i + 1], 0.0f, 0.0f);
+            a[k] = make_float4(g.position[2 * i], g.position[2 * i + 1], 0.0f, 0.0f);
             // quat::fromAxisAngle((0,0,1), angle), fresnel/common/VectorMath.h:920-925,
             // evaluated on the host with the C library's cosf/sinf
             const float th = g.angle[i];
             const float c = cosf(th / 2.0f), sn = sinf(th / 2.0f);
-            b[i] = make_float4(c, sn * 0.0f, sn * 0.0f, sn * 1.0f);
+            b[k] = make_float4(c, sn * 0.0f, sn * 0.0f, sn * 1.0f);
+            break;
             }
-        break;
+            }
         }
+    }
+
+// Copy n elements of h into d starting at element `at` (d already sized).
+template <class T>
+static void upload_at(DBuf<T>& d, size_t at, const T* h, size_t n, hipStream_t s)
+    {
+    if (n)
+        HIPCHECK(hipMemcpyAsync(d.p + at, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+
+// Floats of the colour buffer per row of its first axis.
+static size_t color_row_floats(const GeometryObj& g)
+    {
+    return g.type == G_CYLINDER ? 6 : 3;
+    }
+
+// Bring a geometry's device buffers up to date: everything after creation or
+// update(), else only the rows marked by update_rows().  Returns whether
+// anything the hierarchy depends on (positions, radii, orientations) changed.
+static bool upload_geometry(GeometryObj& g, hipStream_t s)
+    {
+    std::vector<float4> a, b;
+    if (!g.dirty)
+        {
+        const bool shape = g.shape_hi > g.shape_lo;
+        if (shape)
+            {
+            pack_rows(g, g.shape_lo, g.shape_hi, a, b);
+            upload_at(g.a, (size_t)g.shape_lo, a.data(), a.size(), s);
+            if (!b.empty())
+                upload_at(g.b, (size_t)g.shape_lo, b.data(), b.size(), s);
+            }
+        if (g.color_hi > g.color_lo)
+            {
+            const size_t f = color_row_floats(g);
+            upload_at(g.dcolor, (size_t)g.color_lo * f, g.color.data() + g.color_lo * f,
+                      (size_t)(g.color_hi - g.color_lo) * f, s);
+            }
+        g.shape_lo = g.color_lo = UINT64_MAX;
+        g.shape_hi = g.color_hi = 0;
+        // the packed rows must stay alive until the copies have read them
+        if (shape)
+            HIPCHECK(hipStreamSynchronize(s));
+        return shape;
+        }
+    pack_rows(g, 0, g.N, a, b);
     upload(g.a, a, s);
     upload(g.b, b, s);
     upload(g.dcolor, g.color, s);
@@ -672,8 +721,13 @@ static void upload_geometry(GeometryObj& g, hipStream_t s)
         upload(g.aux0, x0, s);
         upload(g.aux1, x1, s);
         g.aux_uploaded = true;
+        HIPCHECK(hipStreamSynchronize(s));
         }
+    HIPCHECK(hipStreamSynchronize(s));
     g.dirty = false;
+    g.shape_lo = g.color_lo = UINT64_MAX;
+    g.shape_hi = g.color_hi = 0;
+    return true;
     }
 
 static uint32_t prim_count(const GeometryObj& g)
@@ -911,10 +965,10 @@ static void sync_scene(SceneObj& sc)
         GeometryObj& g = *kv.second;
         if (!g.valid)
             continue;
-        if (g.dirty)
+        if (g.dirty || g.shape_hi > g.shape_lo || g.color_hi > g.color_lo)
             {
-            upload_geometry(g, D.stream);
-            if (g.enabled)
+            // colour-only changes leave the hierarchy as it is
+            if (upload_geometry(g, D.stream) && g.enabled)
                 any_dirty = true;
             }
         if (g.enabled)
@@ -1067,6 +1121,7 @@ struct TracerObj
     fh_rgb hw_color {1, 0, 1};
     DBuf<float4> linear;
     DBuf<uchar4> srgb;
+    DBuf<uint32_t> hist; // histogram counts (512 x 4)
     std::vector<float> h_linear;
     std::vector<uint8_t> h_srgb;
 
@@ -2476,6 +2531,28 @@ extern "C"
                 geom->obj.dirty = true;
             });
         }
+    int fh_geometry_update_rows(fh_geometry geom, int which, uint64_t first, uint64_t count)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(geom);
+                GeometryObj& g = geom->obj;
+                float* data;
+                uint32_t nd;
+                uint64_t shape[3];
+                const int rc = fh_geometry_buffer(geom, which, &data, &nd, shape);
+                if (rc != FH_OK)
+                    throw Error(rc, fh_last_error());
+                REQUIRE(first <= shape[0] && count <= shape[0] - first, "rows out of range");
+                if (count == 0)
+                    return;
+                uint64_t& lo = which == FH_BUF_COLOR ? g.color_lo : g.shape_lo;
+                uint64_t& hi = which == FH_BUF_COLOR ? g.color_hi : g.shape_hi;
+                lo = std::min(lo, first);
+                hi = std::max(hi, first + count);
+            });
+        }
 #define GEOM_VALID(g)                                                                                \
     do                                                                                               \
         {                                                                                            \
@@ -2731,6 +2808,25 @@ extern "C"
                     }
                 else
                     throw Error(FH_ERR_INVALID, "unknown output buffer");
+            });
+        }
+    int fh_tracer_histogram(fh_tracer tr, uint32_t* counts)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(counts);
+                TracerObj& T = tr->obj;
+                DeviceObj& D = *T.dev;
+                const uint32_t n = T.W * T.H;
+                T.hist.ensure(4 * HIST_BINS);
+                hipStream_t s = D.stream;
+                HIPCHECK(hipMemsetAsync(T.hist.p, 0, 4 * HIST_BINS * sizeof(uint32_t), s));
+                k_histogram<<<std::min(blocks(n, 256), 1024u), 256, 0, s>>>(T.linear.p, n, T.hist.p);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipMemcpyAsync(counts, T.hist.p, 4 * HIST_BINS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
             });
         }
     int fh_tracer_enable_highlight_warning(fh_tracer tr, fh_rgb color)

@@ -42,6 +42,10 @@ class Tracer(object):
             lightness (0.21 R + 0.72 G + 0.07 B), R, G, B, each taken to the
             power 1/2.2 and binned over [0, 1].
         """
+        if hasattr(self._tracer, 'histogram'):
+            # computed on the device from the resident linear output
+            counts = numpy.asarray(self._tracer.histogram()).astype(numpy.int64)
+            return counts, numpy.linspace(0, 1, 513, dtype=numpy.float32)[1:]
         a = numpy.array(self.linear_output[:])
         img = a[a[:, :, 3] > 0]
         r, g, b = img[:, 0], img[:, 1], img[:, 2]

@@ -33,13 +33,24 @@ class Array(object):
         self.buf.unmap()
 
     def __setitem__(self, slice, data):
-        """Assign ``data`` to ``self[slice]``."""
+        """Assign ``data`` to ``self[slice]``.
+
+        For a geometry buffer only the rows the assignment touches are marked
+        for upload when the backend supports it (``update_rows``); otherwise the
+        whole geometry is, as in the reference (fresnel/util.py:41-48).
+        """
         self.buf.map()
         a = numpy.asarray(self.buf)
         a[slice] = data
         self.buf.unmap()
-        if self.geom is not None:
+        if self.geom is None:
+            return
+        mark = getattr(self.buf, 'update_rows', None)
+        rows = _touched_rows(slice, self.shape[0]) if mark is not None else None
+        if rows is None:
             self.geom._geometry.update()
+        elif rows[1] > rows[0]:
+            mark(rows[0], rows[1] - rows[0])
 
     def __getitem__(self, slice):
         """Return a copy of ``self[slice]``."""
@@ -48,6 +59,39 @@ class Array(object):
         data = numpy.array(a[slice], copy=True)
         self.buf.unmap()
         return data
+
+
+def _touched_rows(index, n):
+    """Rows [lo, hi) along the first axis that ``a[index] = ...`` can write.
+
+    Integers, slices and integer or boolean arrays in the first position are
+    resolved exactly (as a bounding range); anything else (an Ellipsis, a
+    newaxis, a field name) counts as every row.
+    """
+    first = index[0] if isinstance(index, tuple) and len(index) > 0 else index
+    if isinstance(index, tuple) and len(index) == 0:
+        return 0, n
+    if isinstance(first, (int, numpy.integer)) and not isinstance(first, bool):
+        i = int(first)
+        i = i + n if i < 0 else i
+        return (i, i + 1) if 0 <= i < n else (0, n)
+    if isinstance(first, slice):
+        start, stop, step = first.indices(n)
+        rows = range(start, stop, step)
+        if len(rows) == 0:
+            return 0, 0
+        return min(rows[0], rows[-1]), max(rows[0], rows[-1]) + 1
+    if isinstance(first, (list, numpy.ndarray)):
+        idx = numpy.asarray(first)
+        if idx.dtype == bool and idx.shape == (n,):
+            idx = numpy.nonzero(idx)[0]
+        elif idx.dtype.kind not in 'iu':
+            return 0, n
+        if idx.size == 0:
+            return 0, 0
+        idx = numpy.where(idx < 0, idx + n, idx)
+        return int(idx.min()), int(idx.max()) + 1
+    return 0, n
 
 
 class ImageArray(Array):

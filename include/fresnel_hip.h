@@ -150,6 +150,13 @@ int fh_geometry_buffer(fh_geometry geom, int which, float** data, uint32_t* ndim
 /* Geometry.update(): buffers changed, rebuild on the next render
  * (fresnel/cpu/Geometry.h:91-94, fresnel/gpu/Geometry.h:85-88). */
 int fh_geometry_update(fh_geometry geom);
+/* Partial form of update(): rows [first, first + count) along the first axis
+ * of buffer `which` changed (the reference's Array __setitem__ followed by
+ * update(), fresnel/util.py:41-48, which re-commits the whole geometry).  The
+ * next render uploads only the marked rows; colour rows leave the hierarchy
+ * untouched, other rows refit or rebuild it as fh_scene_set_refit selects.
+ * FH_ERR_INVALID for an unknown buffer or rows beyond the buffer. */
+int fh_geometry_update_rows(fh_geometry geom, int which, uint64_t first, uint64_t count);
 /* get/setMaterial, get/setOutlineMaterial, get/setOutlineWidth, enable,
  * disable, remove: fresnel/gpu/Geometry.cc:125-139 */
 int fh_geometry_set_material(fh_geometry geom, const fh_material* m);
@@ -191,6 +198,12 @@ int fh_tracer_render_samples(fh_tracer tr, fh_scene scene, uint32_t samples);
 #define FH_OUT_SRGB 0
 #define FH_OUT_LINEAR 1
 int fh_tracer_map_output(fh_tracer tr, int which, void** data);
+/* Tracer.histogram() (fresnel/tracer.py:79-108) on the device-resident
+ * linear output: counts[bin * 4 + column], 512 bins over [0, 1] of
+ * lightness**(1/2.2), R**(1/2.2), G**(1/2.2), B**(1/2.2) of the pixels with
+ * alpha > 0, lightness = 0.21 R + 0.72 G + 0.07 B.  Only the 8 KiB of counts
+ * cross PCIe; the bin positions are k / 512, k = 1..512. */
+int fh_tracer_histogram(fh_tracer tr, uint32_t counts[2048]);
 /* enable/disableHighlightWarning, get/setSeed: cpu/Tracer.h:57-87 */
 int fh_tracer_enable_highlight_warning(fh_tracer tr, fh_rgb color);
 int fh_tracer_disable_highlight_warning(fh_tracer tr);
