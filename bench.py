@@ -11,11 +11,14 @@ Default workload (the large particle scene, one GPU):
 A step renders one frame: Path.sample(scene, samples=256) on an HBM-resident
 scene (the BVH is built during warmup).  Mpaths/s = W*H*samples*light_samples
 per second.  The JSON line also carries:
-  roofline      traversal kernel: algorithmic bytes per launch (64 B per BVH
-                node visited + 16 B per sphere tested + 48 B of ray in / hit
-                out per ray, from the instrumented traversal counters) over the
-                average launch duration (HIP events on the render stream),
-                against 8 TB/s HBM.
+  roofline      the kernel with the most time per frame (the fused
+                primary-ray kernel or the secondary-ray traversal):
+                algorithmic bytes per launch (64 B per BVH node visited +
+                prim_bytes per primitive tested + the ray's own bytes, from
+                the instrumented traversal counters) over the average launch
+                duration (HIP events on the stream it runs on), against 8 TB/s
+                HBM; with its VALU busy and active-lane fractions from the
+                committed rocprofv3 counters (profiles/traffic_<workload>.json).
   cpu_baseline  the CPU checker (a scalar restatement of the reference's CPU
                 path, oracle/) on the same scene, full-resolution passes on
                 host threads for about 10 s.
@@ -310,11 +313,14 @@ def run_plain(args):
     ctr = d.countersLast()
     d.countersEnable(False)
     pb = wl['prim_bytes']
-    traffic = {}
+    traffic, valu_busy, valu_lanes = {}, {}, {}
     tf = os.path.join(ROOT, 'profiles', f'traffic_{args.workload}.json')
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get('hbm_bytes_per_launch', {})
+            prof = json.load(f)
+        traffic = prof.get('hbm_bytes_per_launch', {})
+        valu_busy = prof.get('valu_busy_pct', {})
+        valu_lanes = prof.get('valu_active_lanes_pct', {})
 
     def kernel_roofline(name, rays, nodes, prims, ray_bytes, what):
         """Algorithmic bytes per launch (64 B per BVH node visited, prim_bytes per
@@ -328,7 +334,13 @@ def run_plain(args):
                 "kernel": what, "alg_bytes_per_launch": int(alg / launches), "launches_per_step": int(launches),
                 "avg_launch_ms": round(ms / launches, 3), "rays_per_launch": int(rays / launches),
                 "nodes_per_ray": round(nodes / max(rays, 1), 2), "prims_per_ray": round(prims / max(rays, 1), 2),
-                "ms_per_step": round(ms, 3)}
+                "ms_per_step": round(ms, 3),
+                # the vector ALU side of the same kernel, from the committed
+                # rocprofv3 counters (VALUBusy, VALUUtilization): a kernel far
+                # below the HBM peak is bound by its FP32/integer instruction
+                # issue instead when its VALU is busy most of the time
+                "valu_busy_frac": (round(valu_busy[name] / 100, 3) if name in valu_busy else None),
+                "valu_active_lanes_frac": (round(valu_lanes[name] / 100, 3) if name in valu_lanes else None)}
 
     # primary rays are generated in the kernel; the fused primary kernel
     # (k_primary) writes at least a 16 B record or queue entry per ray, the
