@@ -454,7 +454,11 @@ __device__ __forceinline__ void child_box(const Node& n, uint32_t side, float b[
 
 // 4-wide node idx4[i] from inner node i of n, for the nodes at even depth:
 // idx4 is the exclusive scan of k_depth_even's flags (n + 1 entries).  With
-// sphere_geoms (a spheres-only scene) leaf children hold their spheres.
+// sphere_geoms (a spheres-only scene) every child is stored as a centre and
+// three half extents, so the search forms each child's box as c -+ h with
+// the same operations: a leaf child holds its sphere (h = radius on every
+// axis, so c -+ h is exactly the builder's bounds), an inner child a box
+// that contains its exact box after rounding.
 __global__ void k_bvh4_fill(const Node* __restrict__ nodes, uint32_t n, const uint32_t* __restrict__ idx4,
                             Node4* __restrict__ out, const GeomDev* __restrict__ sphere_geoms)
     {
@@ -488,16 +492,33 @@ __global__ void k_bvh4_fill(const Node* __restrict__ nodes, uint32_t n, const ui
         }
     if (sphere_geoms)
         for (uint32_t k = 0; k < m; k++)
-            if ((ref[k] & LEAF) && ref[k] != EMPTY)
+            {
+            if (ref[k] == EMPTY)
+                continue; // NaN box, NaN centre and extents
+            if (ref[k] & LEAF)
                 {
+                // the sphere itself: centre and radius on every axis
                 const float4 sp = sphere_geoms[slot[k]].a[ref[k] & ~LEAF];
                 box[k][0] = sp.x;
                 box[k][1] = sp.y;
                 box[k][2] = sp.z;
                 box[k][3] = sp.w;
-                box[k][4] = 0.0f;
-                box[k][5] = 0.0f;
+                box[k][4] = sp.w;
+                box[k][5] = sp.w;
+                continue;
                 }
+            // an inner box as centre c and half extents h with c - h <= lo
+            // and c + h >= hi after float rounding: h is one ulp above the
+            // rounded larger distance to c, hence above the exact distance
+            for (int x = 0; x < 3; x++)
+                {
+                const float lo = box[k][x], hi = box[k][3 + x];
+                const float c = 0.5f * lo + 0.5f * hi;
+                const float h = fmaxf(c - lo, hi - c);
+                box[k][x] = c;
+                box[k][3 + x] = __uint_as_float(__float_as_uint(h) + 1u); // h >= 0: next float up
+                }
+            }
     const float nan = __builtin_nanf("");
     for (; m < 4; m++)
         {

@@ -218,13 +218,19 @@ __device__ __forceinline__ bool trace_visit4(const Node4* __restrict__ nodes, co
         for (int c = 0; c < 2; c++)
             {
             const int k = 2 * q + c;
-            // a sphere leaf's box: centre -+ radius, as the builder bounds it
-            // (GeometrySphere.cc:60-73)
-            const bool sl_k = sph && (ref[c] & LEAF);
-            const float rr = bx[c][3];
-            const float x0 = sl_k ? bx[c][0] - rr : bx[c][0], x1 = sl_k ? bx[c][0] + rr : bx[c][3];
-            const float y0 = sl_k ? bx[c][1] - rr : bx[c][1], y1 = sl_k ? bx[c][1] + rr : bx[c][4];
-            const float z0 = sl_k ? bx[c][2] - rr : bx[c][2], z1 = sl_k ? bx[c][2] + rr : bx[c][5];
+            // with sphere leaves every child is a centre and half extents
+            // (k_bvh4_fill): a sphere leaf's box is centre -+ radius, as the
+            // builder bounds it (GeometrySphere.cc:60-73)
+            float x0 = bx[c][0], y0 = bx[c][1], z0 = bx[c][2], x1 = bx[c][3], y1 = bx[c][4], z1 = bx[c][5];
+            if (sph)
+                {
+                x0 = bx[c][0] - bx[c][3];
+                x1 = bx[c][0] + bx[c][3];
+                y0 = bx[c][1] - bx[c][4];
+                y1 = bx[c][1] + bx[c][4];
+                z0 = bx[c][2] - bx[c][5];
+                z1 = bx[c][2] + bx[c][5];
+                }
             float tmax;
             slab(r.o, r.inv, x0, y0, z0, x1, y1, z1, tmin[k], tmax);
             cb[k] = cull_bound(tmin[k], tmax, r.margin);

@@ -83,8 +83,10 @@ struct __attribute__((aligned(64))) Node
 // search step loads and tests one pair at a time.  A child reference is a
 // 4-wide node index, LEAF | local primitive index (geometry slot alongside)
 // or EMPTY with a NaN box.  In a spheres-only scene (SceneDev::sphere_leaves)
-// a leaf child holds its sphere instead of its box, centre in lo.xyz and
-// radius in hi.x; its box is centre -+ radius, the builder's bounds.
+// every child holds a centre c in lo.xyz and half extents h in hi.xyz, and its
+// box is c -+ h: a leaf child its sphere (h = radius on every axis, so the box
+// is exactly the builder's bounds), an inner child a box that contains the
+// exact one after rounding.
 struct __attribute__((aligned(128))) Node4
     {
     Node pair[2];
