@@ -49,7 +49,9 @@ __device__ __forceinline__ float cull_bound(float tmin, float tmax, float margin
 
 __device__ __forceinline__ bool better(float t, uint32_t g, uint32_t p, const HitT& b)
     {
-    return (t < b.t) || (t == b.t && (g < b.geom || (g == b.geom && p < b.prim)));
+    // every comparison evaluated (no short circuit): lane masks, no branches
+    const bool lt = t < b.t, eq = t == b.t, g_lt = g < b.geom, g_eq = g == b.geom, p_lt = p < b.prim;
+    return lt | (eq & (g_lt | (g_eq & p_lt)));
     }
 
 // Stack with the top SS entries in LDS (interleaved by thread) and the rest
