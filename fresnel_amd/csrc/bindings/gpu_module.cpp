@@ -82,7 +82,7 @@ struct Scene
 
 // Host view of a backend buffer.  For geometry buffers the host copy is
 // authoritative; for tracer outputs map() copies the device image first.
-struct Array
+struct __attribute__((visibility("hidden"))) Array
     {
     std::function<void()> on_map;
     std::function<py::buffer_info()> info;
