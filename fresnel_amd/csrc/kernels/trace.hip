@@ -648,13 +648,19 @@ __device__ __forceinline__ void primary_order(const JobDev& J, uint32_t local, u
     ordinal_pixel(J, j, px, py);
     }
 
+// Primary ray i of job J (the job whose ray range holds i).
+__device__ __forceinline__ void primary_ray_of(const JobDev& J, uint32_t i, uint32_t& k, uint32_t& px, uint32_t& py,
+                                               uint32_t& jord, v3& o, v3& d)
+    {
+    primary_order(J, i - J.ray_base, k, px, py, jord);
+    camera_ray(J.cam, J.W, J.H, J.xs_off, J.inv_h, J.seed, px, py, J.n_base + k + 1, true, o, d);
+    }
+
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
                                             uint32_t& k, uint32_t& px, uint32_t& py, uint32_t& jord, v3& o, v3& d)
     {
     job = find_job(jobs, n_jobs, i);
-    const JobDev& J = jobs[job];
-    primary_order(J, i - J.ray_base, k, px, py, jord);
-    camera_ray(J.cam, J.W, J.H, J.xs_off, J.inv_h, J.seed, px, py, J.n_base + k + 1, true, o, d);
+    primary_ray_of(jobs[job], i, k, px, py, jord, o, d);
     }
 
 __device__ __forceinline__ void primary_ray(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t i, uint32_t& job,
@@ -1341,25 +1347,56 @@ __global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jo
         const uint32_t u = bands ? (c % QSEG) * G + c / QSEG : c;
         const uint32_t seg = bands ? u / G : u % QSEG; // the chunk's output segment
         const uint32_t base = u * chunk;
-        for (uint32_t g = 0; g < groups && u < n_units; g++)
+        // one job for the whole chunk (every chunk of a single-job batch, all
+        // but the chunks that straddle two jobs otherwise): its index is
+        // workgroup-uniform, so the job record arrives by scalar loads; the
+        // branch is taken by the whole workgroup, barriers included
+        const uint32_t last = min(n, base + chunk) - 1u;
+        const uint32_t job_first = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_job(jobs, n_jobs, base));
+        const uint32_t job_last = (uint32_t)__builtin_amdgcn_readfirstlane((int)find_job(jobs, n_jobs, last));
+        if (u < n_units && job_first == job_last)
             {
-            const uint32_t i = base + (wave * groups + g) * 64 + lane;
-            const bool valid = i < n;
-            v3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
-            uint32_t job = 0, k = 0, px = 0, py = 0, jord = 0;
-            if (valid)
-                primary_ray(jobs, n_jobs, i, job, k, px, py, jord, o, d);
-            HitT best;
-            best.t = __builtin_huge_valf();
-            best.geom = EMPTY;
-            best.prim = EMPTY;
-            primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims, n_entries,
-                                                err);
-            if (COUNT && valid)
-                n_rays++;
-            shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, jord, o, d, best, out,
-                                              seg, records, sh);
+            const JobDev& J = jobs[job_first];
+            for (uint32_t g = 0; g < groups; g++)
+                {
+                const uint32_t i = base + (wave * groups + g) * 64 + lane;
+                const bool valid = i < n;
+                v3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
+                uint32_t k = 0, px = 0, py = 0, jord = 0;
+                if (valid)
+                    primary_ray_of(J, i, k, px, py, jord, o, d);
+                HitT best;
+                best.t = __builtin_huge_valf();
+                best.geom = EMPTY;
+                best.prim = EMPTY;
+                primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job_first, px, py, o, d, st, best, n_nodes, n_prims,
+                                                    n_entries, err);
+                if (COUNT && valid)
+                    n_rays++;
+                shade_primary_block<BLOCK, TYPES>(jobs, valid, job_first, k, py * J.W + px, jord, o, d, best, out,
+                                                  seg, records, sh);
+                }
             }
+        else
+            for (uint32_t g = 0; g < groups && u < n_units; g++)
+                {
+                const uint32_t i = base + (wave * groups + g) * 64 + lane;
+                const bool valid = i < n;
+                v3 o = mk3(0.0f, 0.0f, 0.0f), d = mk3(0.0f, 0.0f, 1.0f);
+                uint32_t job = 0, k = 0, px = 0, py = 0, jord = 0;
+                if (valid)
+                    primary_ray(jobs, n_jobs, i, job, k, px, py, jord, o, d);
+                HitT best;
+                best.t = __builtin_huge_valf();
+                best.geom = EMPTY;
+                best.prim = EMPTY;
+                primary_group_hit<SS, COUNT, TYPES>(jobs, valid, job, px, py, o, d, st, best, n_nodes, n_prims,
+                                                    n_entries, err);
+                if (COUNT && valid)
+                    n_rays++;
+                shade_primary_block<BLOCK, TYPES>(jobs, valid, job, k, py * jobs[job].W + px, jord, o, d, best, out,
+                                                  seg, records, sh);
+                }
         __syncthreads();
         c = s_claim;
         __syncthreads();
