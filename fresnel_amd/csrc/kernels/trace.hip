@@ -926,8 +926,43 @@ __device__ inline bool shade_bounce(const JobDev& J, const HitT& h, v3 o, v3 d, 
     return cont;
     }
 
+// Shade queued ray i (queue slot sl) of job J: returns whether its path
+// continues with (no, nd, att); a terminated path's record is written.
+template<uint32_t TYPES>
+__device__ __forceinline__ bool shade_queued(const JobDev& J, const Queue& in, const uint4* __restrict__ hits,
+                                             uint32_t i, uint32_t sl, const uint4& mt, uint4* records,
+                                             uint32_t max_depth, unsigned long long* counters, v3& att, v3& no,
+                                             v3& nd)
+    {
+    const uint4 hh = hits[i];
+    HitT h;
+    h.t = __uint_as_float(hh.x);
+    h.geom = hh.y;
+    h.prim = hh.z;
+    const float4 a = in.o[sl];
+    const float4 b = in.d[sl];
+    const float4 at4 = in.att[sl];
+    att = mk3(at4.x, at4.y, at4.z);
+    const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
+    uint4 rec;
+    bool cont = shade_bounce<TYPES>(J, h, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, mt.y, k, ls, mt.w, att, no, nd,
+                                    rec);
+    if (cont && mt.w + 1 >= max_depth)
+        {
+        // depth cap (the reference has none): stop without contribution
+        cont = false;
+        rec = make_rec(mk3(0, 0, 0), REC_NONE);
+        atomicAdd(counters + CTR_CAPPED, 1ull);
+        }
+    if (!cont)
+        records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] = rec;
+    return cont;
+    }
+
+// A batch of one job reads its job record through scalar loads (jobs[0]);
+// otherwise each ray's record is found by its job index.
 template<int BS, uint32_t TYPES>
-__global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, Queue in,
+__global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue in,
                                               const uint4* __restrict__ hits, uint32_t n, Queue out,
                                               uint4* records, uint32_t max_depth, unsigned long long* counters)
     {
@@ -941,29 +976,10 @@ __global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, Q
     if (i < n)
         {
         mt = in.meta[sl];
-        const JobDev& J = jobs[mt.x];
-        const uint4 hh = hits[i];
-        HitT h;
-        h.t = __uint_as_float(hh.x);
-        h.geom = hh.y;
-        h.prim = hh.z;
-        const float4 a = in.o[sl];
-        const float4 b = in.d[sl];
-        const float4 at4 = in.att[sl];
-        att = mk3(at4.x, at4.y, at4.z);
-        const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
-        uint4 rec;
-        cont = shade_bounce<TYPES>(J, h, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, mt.y, k, ls, mt.w, att, no, nd,
-                                   rec);
-        if (cont && mt.w + 1 >= max_depth)
-            {
-            // depth cap (the reference has none): stop without contribution
-            cont = false;
-            rec = make_rec(mk3(0, 0, 0), REC_NONE);
-            atomicAdd(counters + CTR_CAPPED, 1ull);
-            }
-        if (!cont)
-            records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] = rec;
+        if (n_jobs == 1)
+            cont = shade_queued<TYPES>(jobs[0], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
+        else
+            cont = shade_queued<TYPES>(jobs[mt.x], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
         }
     const uint32_t slot = block_append<BS>(cont, out, blockIdx.x % QSEG, sh);
     if (cont && slot != EMPTY)
@@ -1610,32 +1626,13 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
     }
 
 // Tail: each thread finishes its path (trace + shade until termination).
-template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
-__global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ jobs, Queue in, uint32_t n,
-                                                  uint4* records, uint32_t max_depth, uint2* gstack,
-                                                  uint32_t gstride, unsigned long long* counters)
+template<int SS, bool COUNT, uint32_t TYPES>
+__device__ __forceinline__ void finish_path(const JobDev& J, const uint4& mt, v3 o, v3 d, v3 att, LdsStack<SS>& st,
+                                            uint4* records, uint32_t max_depth, unsigned long long* counters,
+                                            uint32_t& n_nodes, uint32_t& n_prims, uint32_t& n_rays, uint32_t& err)
     {
-    __shared__ uint2 lds[SS * BLOCK];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n)
-        return;
-    LdsStack<SS> st;
-    st.lds = lds + threadIdx.x;
-    st.lstride = BLOCK;
-    st.glob = gstack + i;
-    st.gstride = gstride;
-    st.sp = 0;
-    const uint32_t first = blockIdx.x * BLOCK;
-    const uint32_t sl = q_slot(in, i, first, min(n, first + BLOCK) - 1u);
-    const uint4 mt = in.meta[sl];
-    const JobDev& J = jobs[mt.x];
-    const float4 a = in.o[sl];
-    const float4 b = in.d[sl];
-    const float4 at4 = in.att[sl];
-    v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
     const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
     uint32_t depth = mt.w;
-    uint32_t n_nodes = 0, n_prims = 0, n_rays = 0, err = 0;
     uint4 rec;
     for (;;)
         {
@@ -1657,6 +1654,38 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
         depth++;
         }
     records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] = rec;
+    }
+
+// A batch of one job reads its job record through scalar loads, as in k_shade.
+template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
+__global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue in,
+                                                  uint32_t n, uint4* records, uint32_t max_depth, uint2* gstack,
+                                                  uint32_t gstride, unsigned long long* counters)
+    {
+    __shared__ uint2 lds[SS * BLOCK];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n)
+        return;
+    LdsStack<SS> st;
+    st.lds = lds + threadIdx.x;
+    st.lstride = BLOCK;
+    st.glob = gstack + i;
+    st.gstride = gstride;
+    st.sp = 0;
+    const uint32_t first = blockIdx.x * BLOCK;
+    const uint32_t sl = q_slot(in, i, first, min(n, first + BLOCK) - 1u);
+    const uint4 mt = in.meta[sl];
+    const float4 a = in.o[sl];
+    const float4 b = in.d[sl];
+    const float4 at4 = in.att[sl];
+    const v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
+    uint32_t n_nodes = 0, n_prims = 0, n_rays = 0, err = 0;
+    if (n_jobs == 1)
+        finish_path<SS, COUNT, TYPES>(jobs[0], mt, o, d, att, st, records, max_depth, counters, n_nodes, n_prims,
+                                      n_rays, err);
+    else
+        finish_path<SS, COUNT, TYPES>(jobs[mt.x], mt, o, d, att, st, records, max_depth, counters, n_nodes, n_prims,
+                                      n_rays, err);
     if (COUNT)
         {
         atomicAdd(counters + CTR_TAIL + 0, (unsigned long long)n_rays);

@@ -1514,8 +1514,8 @@ struct BatchExec
                                                        : k_finish<TRAV_BLOCK, TRAV_SS, false, T_SPHERES>)
                                          : (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_ALL>
                                                        : k_finish<TRAV_BLOCK, TRAV_SS, false, T_ALL>);
-                       f<<<g, TRAV_BLOCK, 0, s>>>(R.jobs.p, q, cnt, R.records.p, MAX_DEPTH, R.tail_stack.p, cnt,
-                                                  D.counters.p);
+                       f<<<g, TRAV_BLOCK, 0, s>>>(R.jobs.p, nj, q, cnt, R.records.p, MAX_DEPTH, R.tail_stack.p,
+                                                  cnt, D.counters.p);
                    });
             paths_done = true;
             return false;
@@ -1539,7 +1539,7 @@ struct BatchExec
                    [&]
                    {
                        auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
-                       f<<<blocks(cnt, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, Q[cur], R.hits.p, cnt,
+                       f<<<blocks(cnt, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, Q[cur], R.hits.p, cnt,
                                                                           Q[1 - cur], R.records.p, MAX_DEPTH,
                                                                           D.counters.p);
                    });
