@@ -959,8 +959,10 @@ __device__ __forceinline__ bool shade_queued(const JobDev& J, const Queue& in, c
     return cont;
     }
 
-// A batch of one job reads its job record through scalar loads (jobs[0]);
-// otherwise each ray's record is found by its job index.
+// A wavefront whose rays all belong to one job (every wavefront of a
+// single-job batch, most of a multi-job one, whose queues are filled chunk by
+// chunk) reads the job record through scalar loads; otherwise each ray reads
+// its own job's record.
 template<int BS, uint32_t TYPES>
 __global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue in,
                                               const uint4* __restrict__ hits, uint32_t n, Queue out,
@@ -976,8 +978,9 @@ __global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, u
     if (i < n)
         {
         mt = in.meta[sl];
-        if (n_jobs == 1)
-            cont = shade_queued<TYPES>(jobs[0], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
+        const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
+        if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
+            cont = shade_queued<TYPES>(jobs[j0], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
         else
             cont = shade_queued<TYPES>(jobs[mt.x], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
         }
@@ -1656,7 +1659,8 @@ __device__ __forceinline__ void finish_path(const JobDev& J, const uint4& mt, v3
     records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] = rec;
     }
 
-// A batch of one job reads its job record through scalar loads, as in k_shade.
+// A wavefront whose paths all belong to one job reads the job record through
+// scalar loads, as in k_shade.
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue in,
                                                   uint32_t n, uint4* records, uint32_t max_depth, uint2* gstack,
@@ -1680,8 +1684,9 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ job
     const float4 at4 = in.att[sl];
     const v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
     uint32_t n_nodes = 0, n_prims = 0, n_rays = 0, err = 0;
-    if (n_jobs == 1)
-        finish_path<SS, COUNT, TYPES>(jobs[0], mt, o, d, att, st, records, max_depth, counters, n_nodes, n_prims,
+    const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
+    if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
+        finish_path<SS, COUNT, TYPES>(jobs[j0], mt, o, d, att, st, records, max_depth, counters, n_nodes, n_prims,
                                       n_rays, err);
     else
         finish_path<SS, COUNT, TYPES>(jobs[mt.x], mt, o, d, att, st, records, max_depth, counters, n_nodes, n_prims,
