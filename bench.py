@@ -633,7 +633,7 @@ def main():
     ap.add_argument('--frames', type=int, default=64, help='c4: trajectory frames')
     ap.add_argument('--requests', type=int, default=24, help='c5: requests per stream')
     ap.add_argument('--rates', default='5,10,20,50,100,200', help='c5: arrival rates (req/s)')
-    ap.add_argument('--target-paths', type=int, default=64 << 20, help='c5: paths per server step')
+    ap.add_argument('--target-paths', type=int, default=128 << 20, help='c5: paths per server step')
     ap.add_argument('--no-loop', action='store_true', help='c5: skip the render()-per-sample plain form')
     args = ap.parse_args()
     if args.gpus != 1:

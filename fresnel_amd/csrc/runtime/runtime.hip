@@ -1749,7 +1749,7 @@ struct ServerBatch
 struct ServerObj
     {
     DeviceObj* dev = nullptr;
-    uint64_t budget = 0, in_use = 0, target_paths = 64ull << 20;
+    uint64_t budget = 0, in_use = 0, target_paths = 128ull << 20;
     std::deque<PolicyItem> queue;
     std::vector<uint64_t> active; // admission order
     std::map<uint64_t, RequestObj> reqs;

@@ -48,10 +48,12 @@ class Server:
     Args:
         device: a GPU device.
         hbm_budget (int): bytes of HBM the active requests may hold.
-        target_paths (int): light-sample paths traced per step.
+        target_paths (int): light-sample paths traced per step.  The server's
+            two render scratches take about 2 x 160 B x target_paths of HBM
+            beyond the budget (43 GB at the default 128 Mi).
     """
 
-    def __init__(self, device, hbm_budget=240 << 30, target_paths=64 << 20):
+    def __init__(self, device, hbm_budget=240 << 30, target_paths=128 << 20):
         if device.mode != 'gpu':
             raise RuntimeError("the render server needs a GPU device")
         self.device = device
