@@ -164,7 +164,7 @@ struct DeviceObj
     uint32_t* h_counts = nullptr; // pinned copy of a queue's segment counters and flag
     bool stats_on = false;
     bool count_on = false;
-    uint64_t batch_paths = 64ull << 20; // light-sample paths per batch (sizes the queues)
+    uint64_t batch_paths = 256ull << 20; // light-sample paths per batch (sizes the queues)
     uint32_t tail_rays = 131072;        // finish paths in one kernel below this many rays
     uint32_t trav_chunk = 256;          // rays a secondary-ray traversal wavefront claims per atomic
     uint32_t primary_chunk = 1024;      // the same for primary rays
