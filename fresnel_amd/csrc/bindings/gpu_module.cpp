@@ -87,6 +87,7 @@ struct __attribute__((visibility("hidden"))) Array
     std::function<void()> on_map;
     std::function<py::buffer_info()> info;
     std::function<void(uint64_t, uint64_t)> on_rows;
+    std::function<py::tuple()> layout; // (shape, numpy dtype string) without a copy
     std::shared_ptr<void> owner;
     void map()
         {
@@ -177,6 +178,7 @@ static std::shared_ptr<Array> geometry_buffer(std::shared_ptr<Geometry> g, int w
         }
     a->info = [data, shp, strides]()
     { return py::buffer_info(data, sizeof(float), py::format_descriptor<float>::format(), (py::ssize_t)shp.size(), shp, strides); };
+    a->layout = [shp]() { return py::make_tuple(py::tuple(py::cast(shp)), "float32"); };
     Geometry* gp = g.get();
     a->on_rows = [gp, which](uint64_t first, uint64_t count) { check(fh_geometry_update_rows(gp->h, which, first, count)); };
     return a;
@@ -317,10 +319,12 @@ static std::shared_ptr<Array> output_buffer(std::shared_ptr<Tracer> t, int which
         void* p;
         check(fh_tracer_map_output(tp->h, which, &p));
     };
+    // map() copies the device image into the host staging buffer; the buffer
+    // protocol exposes that buffer as the last map() left it
     a->info = [tp, which]()
     {
         void* p;
-        check(fh_tracer_map_output(tp->h, which, &p));
+        check(fh_tracer_output_host(tp->h, which, &p));
         uint32_t w, hh;
         check(fh_tracer_get_size(tp->h, &w, &hh));
         if (which == FH_OUT_SRGB)
@@ -328,6 +332,12 @@ static std::shared_ptr<Array> output_buffer(std::shared_ptr<Tracer> t, int which
                                    {(py::ssize_t)w * 4, (py::ssize_t)4, (py::ssize_t)1});
         return py::buffer_info(p, 4, py::format_descriptor<float>::format(), 3, {(py::ssize_t)hh, (py::ssize_t)w, (py::ssize_t)4},
                                {(py::ssize_t)w * 16, (py::ssize_t)16, (py::ssize_t)4});
+    };
+    a->layout = [tp, which]()
+    {
+        uint32_t w, hh;
+        check(fh_tracer_get_size(tp->h, &w, &hh));
+        return py::make_tuple(py::make_tuple(hh, w, 4), which == FH_OUT_SRGB ? "uint8" : "float32");
     };
     return a;
     }
@@ -563,7 +573,8 @@ PYBIND11_MODULE(_gpu, m)
         .def_buffer([](Array& a) { return a.info(); })
         .def("map", &Array::map)
         .def("unmap", &Array::unmap)
-        .def("update_rows", &Array::update_rows);
+        .def("update_rows", &Array::update_rows)
+        .def_property_readonly("layout", [](Array& a) { return a.layout(); });
 
     py::class_<Geometry, std::shared_ptr<Geometry>>(m, "Geometry")
         .def("getMaterial", &Geometry::getMaterial)

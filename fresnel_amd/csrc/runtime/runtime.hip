@@ -2810,6 +2810,22 @@ extern "C"
                     throw Error(FH_ERR_INVALID, "unknown output buffer");
             });
         }
+    int fh_tracer_output_host(fh_tracer tr, int which, void** data)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(data);
+                TracerObj& T = tr->obj;
+                if (which == FH_OUT_SRGB)
+                    *data = T.h_srgb.data();
+                else if (which == FH_OUT_LINEAR)
+                    *data = T.h_linear.data();
+                else
+                    throw Error(FH_ERR_INVALID, "unknown output buffer");
+            });
+        }
     int fh_tracer_histogram(fh_tracer tr, uint32_t* counts)
         {
         return guard(

@@ -26,6 +26,14 @@ class Array(object):
     def __init__(self, buf, geom):
         self.buf = buf
         self.geom = geom
+        layout = getattr(buf, 'layout', None)
+        if layout is not None:
+            # shape and type without mapping: a tracer output stays on the
+            # device until it is read
+            shape, dtype = layout
+            self.shape = tuple(int(x) for x in shape)
+            self.dtype = numpy.dtype(dtype)
+            return
         self.buf.map()
         a = numpy.asarray(self.buf)
         self.shape = a.shape

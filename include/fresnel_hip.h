@@ -198,6 +198,10 @@ int fh_tracer_render_samples(fh_tracer tr, fh_scene scene, uint32_t samples);
 #define FH_OUT_SRGB 0
 #define FH_OUT_LINEAR 1
 int fh_tracer_map_output(fh_tracer tr, int which, void** data);
+/* The same host staging buffer without copying (no reference counterpart):
+ * what the last fh_tracer_map_output left there, for the buffer protocol of an
+ * array that was mapped before it is read. */
+int fh_tracer_output_host(fh_tracer tr, int which, void** data);
 /* Tracer.histogram() (fresnel/tracer.py:79-108) on the device-resident
  * linear output: counts[bin * 4 + column], 512 bins over [0, 1] of
  * lightness**(1/2.2), R**(1/2.2), G**(1/2.2), B**(1/2.2) of the pixels with
