@@ -505,6 +505,16 @@ PYBIND11_MODULE(_gpu, m)
                  check(fh_device_tune(d.h, key.c_str(), value, &prev));
                  return prev;
              })
+        .def("debugSincos",
+             [](Device& d, py::array_t<float, py::array::c_style | py::array::forcecast> x)
+             {
+                 auto i = x.request();
+                 const py::ssize_t n = i.size;
+                 py::array_t<float> s(n), c(n);
+                 check(fh_debug_sincos(d.h, (const float*)i.ptr, (uint32_t)n, (float*)s.request().ptr,
+                                       (float*)c.request().ptr));
+                 return py::make_tuple(s, c);
+             })
         .def("debugPhilox",
              [](Device& d, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> in)
              {

@@ -391,9 +391,17 @@ __global__ void k_refit(const uint32_t* __restrict__ vals, const float4* __restr
         // sibling's box loads are issued only after its increment returned 1.
         // A device-scope fence here would write back and invalidate the whole
         // L2 once per thread and level.
+        // The compiler sees only relaxed atomics to different addresses and an
+        // intrinsic it models as touching no memory, so the signal fences
+        // (compiler barriers, no instruction) keep the box stores before the
+        // wait and the increment, and the sibling's box loads after its
+        // increment, in program order; the wait then orders them in hardware.
         store_box(nodes + node, par & 1u, box);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         const uint32_t prev = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         if (prev == 0)
             return; // the sibling finishes this node
         float b0[6], b1[6];

@@ -1327,11 +1327,13 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
 // groups of 64 rays (primary_group_hit) and the workgroup shades them
 // together (shade_primary_block), so no hit is stored and no primary ray is
 // generated twice.  Same arithmetic as k_traverse_packet + k_shade_primary.
+// Held to 128 registers (4 wavefronts per SIMD; a few values spill): with the
+// double-precision sine and cosine of the depth-0 lobe sample it would take
+// 147 (3 wavefronts) and run 1.7x longer.
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
-__global__ __launch_bounds__(BLOCK) void k_primary(const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t n,
-                                                   uint32_t* work, uint32_t groups, uint32_t bands, uint2* gstack,
-                                                   uint32_t gstride, Queue out, uint4* records,
-                                                   unsigned long long* counters)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_primary(
+    const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t n, uint32_t* work, uint32_t groups, uint32_t bands,
+    uint2* gstack, uint32_t gstride, Queue out, uint4* records, unsigned long long* counters)
     {
     __shared__ uint2 lds[SS * BLOCK];
     __shared__ uint32_t sh[BLOCK / 64 + 1];
@@ -1878,6 +1880,18 @@ __global__ void k_debug_philox(const uint32_t* in, uint32_t n, uint32_t* out)
     const u4 r = philox(c[0], c[1], c[2], c[3], c[4], c[5]);
     for (int q = 0; q < 4; q++)
         out[4 * i + q] = r.v[q];
+    }
+
+// The shading code's sinf/cosf (cr_sincosf) of n arguments.
+__global__ void k_debug_sincos(const float* x, uint32_t n, float* s, float* c)
+    {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float a, b;
+    cr_sincosf(x[i], a, b);
+    s[i] = a;
+    c[i] = b;
     }
 
 // Primary ray and closest hit per pixel for camera sample `sample`.

@@ -3,41 +3,54 @@
 
 #include "scene_dev.h"
 
+#define FH_SC_QUAL __device__ __forceinline__
+#define FH_SC_TABLE static __constant__ const
+#include "sincos_poly.h"
+#include "sincos_hard.h"
+
 namespace fh
 {
 // Shading arithmetic (BRDF, lobe sampling and pdfs, MIS weights, Russian
-// roulette, transmission) runs on the hardware reciprocal, reciprocal square
-// root, square root and sine/cosine (v_rcp/v_rsq/v_sqrt/v_sin/v_cos_f32, about
-// one ulp; sine and cosine take revolutions).  These values never feed a
-// bit-exact comparison: camera rays and intersections keep IEEE division and
-// square root, and images are compared with the CPU checker within an RMSE
-// bound, as the device's cosf/sinf/powf already differ from the host's.
-__device__ __forceinline__ float sh_rcp(float x)
+// roulette, transmission) performs the reference's float operations in the
+// reference's order: correctly rounded division and square root (the device
+// code is compiled with -fhip-fp32-correctly-rounded-divide-sqrt and without
+// contraction), `v / s` of a vector as a multiply by 1 / s and `v /= s` as
+// three divisions (fresnel/common/VectorMath.h:508-542), `fast::rsqrt(x)` as
+// 1 / sqrtf(x) (VectorMath.h:46-53).  cosf and sinf are the correctly rounded
+// values, as the CPU checker computes them: C libraries' cosf/sinf are not
+// specified to the last bit (glibc's differs from the correctly rounded value
+// for about 1.3 % of angles in [0, 2 pi]), and the correctly rounded value is
+// what they return for all but near-midpoint arguments.  So a path's every
+// bounce is bit-identical to the checker's.
+
+// Correctly rounded sinf(x), cosf(x) for 0 <= x <= 8 (the angles 2 pi u of
+// lobe and aperture samples, u in (0, 1]): the double evaluation of
+// sincos_poly rounded to float, or, for the 101 floats in that range whose
+// rounding it cannot decide, the tabulated correctly rounded values.
+__device__ __forceinline__ void cr_sincosf(float x, float& s, float& c)
     {
-    return __builtin_amdgcn_rcpf(x);
+    double sd, cd;
+    const bool hard = sincos_poly(x, sd, cd);
+    s = (float)sd;
+    c = (float)cd;
+    if (hard)
+        {
+        int lo = 0, hi = SINCOS_HARD_N - 1;
+        while (lo < hi)
+            {
+            const int mid = (lo + hi) >> 1;
+            if (SINCOS_HARD[mid][0] < x)
+                lo = mid + 1;
+            else
+                hi = mid;
+            }
+        if (SINCOS_HARD[lo][0] == x)
+            {
+            s = SINCOS_HARD[lo][1];
+            c = SINCOS_HARD[lo][2];
+            }
+        }
     }
-__device__ __forceinline__ float sh_div(float a, float b)
-    {
-    return a * __builtin_amdgcn_rcpf(b);
-    }
-__device__ __forceinline__ float sh_sqrt(float x)
-    {
-    return __builtin_amdgcn_sqrtf(x);
-    }
-__device__ __forceinline__ v3 sh_normalize(v3 a)
-    {
-    return a * __builtin_amdgcn_rsqf(dot(a, a));
-    }
-// cos(2 pi x), sin(2 pi x)
-__device__ __forceinline__ float sh_cos_rev(float x)
-    {
-    return __builtin_amdgcn_cosf(x);
-    }
-__device__ __forceinline__ float sh_sin_rev(float x)
-    {
-    return __builtin_amdgcn_sinf(x);
-    }
-constexpr float INV_PI_F = 0.318309886f;
 
 __device__ __forceinline__ v3 mat_color(const Mat& m)
     {
@@ -62,7 +75,7 @@ __device__ __forceinline__ v3 disney_diffuse(const Mat& m, v3 base_color, float 
     const float FL = schlick(ndotl);
     const float FV = schlick(ndotv);
     const float RR = 2 * m.roughness * ldoth * ldoth;
-    return base_color * INV_PI_F
+    return div_rcp(base_color, PI_F)
            * ((1.0f - 0.5f * FL) * (1.0f - 0.5f * FV) + RR * (FL + FV + FL * FV * (RR - 1.0f)));
     }
 
@@ -73,7 +86,7 @@ __device__ inline v3 brdf(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
     if (ndotv <= 0)
         return mk3(0, 0, 0);
     v3 h = l + v;
-    h = sh_normalize(h);
+    h = div_true(h, sqrtf(dot(h, h)));
     const float ndotl = dot(n, l);
     const float ldoth = dot(l, h);
     const float ndoth = dot(n, h);
@@ -82,14 +95,14 @@ __device__ inline v3 brdf(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
     const float alpha = m.roughness * m.roughness;
     const float alpha_sq = alpha * alpha;
     const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
-    const float D = sh_div(alpha_sq, PI_F * denom_rt * denom_rt);
+    const float D = alpha_sq / (PI_F * denom_rt * denom_rt);
     const float f0 = 0.08f * m.specular;
     const v3 F0 = lerp(m.metal, mk3(f0, f0, f0), base_color);
     const v3 F = F0 + (mk3(1.0f, 1.0f, 1.0f) - F0) * schlick(ldoth);
     const float ndotv_sq = ndotv * ndotv;
     const float ndotl_sq = ndotl * ndotl;
-    const float V1 = sh_rcp(ndotv + sh_sqrt(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
-    const float V2 = sh_rcp(ndotl + sh_sqrt(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
+    const float V1 = 1.0f / (ndotv + sqrtf(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
+    const float V2 = 1.0f / (ndotl + sqrtf(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
     const float V = V1 * V2;
     const v3 f_s = D * F * V;
     return f_d * (1.0f - m.metal) + f_s;
@@ -102,7 +115,7 @@ __device__ inline v3 brdf_diffuse(const Mat& m, v3 l, v3 v, v3 n, v3 sc)
     if (ndotv <= 0 || m.metal == 1.0f)
         return mk3(0, 0, 0);
     v3 h = l + v;
-    h = sh_normalize(h);
+    h = div_true(h, sqrtf(dot(h, h)));
     const float ndotl = dot(n, l);
     const float ldoth = dot(l, h);
     return disney_diffuse(m, get_color(m, sc), ndotl, ndotv, ldoth) * (1.0f - m.metal);
@@ -117,7 +130,7 @@ __device__ inline v3 brdf_specular(const Mat& m, v3 l, v3 v, v3 n, v3 sc, float 
     if (ndotv <= 0)
         return mk3(0, 0, 0);
     v3 h = l + v;
-    h = sh_normalize(h);
+    h = div_true(h, sqrtf(dot(h, h)));
     const float ndotl = dot(n, l);
     const float ldoth = dot(l, h);
     const float ndoth = dot(n, h);
@@ -125,7 +138,7 @@ __device__ inline v3 brdf_specular(const Mat& m, v3 l, v3 v, v3 n, v3 sc, float 
     const float alpha = m.roughness * m.roughness;
     const float alpha_sq = alpha * alpha;
     const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
-    float D = sh_div(alpha_sq, PI_F * denom_rt * denom_rt);
+    float D = alpha_sq / (PI_F * denom_rt * denom_rt);
     if (light_half_angle > 0.99f * PI_F / 2.0f)
         light_half_angle = 0.99f * PI_F / 2.0f;
     if (light_half_angle > 0.0f)
@@ -133,15 +146,15 @@ __device__ inline v3 brdf_specular(const Mat& m, v3 l, v3 v, v3 n, v3 sc, float 
         float alpha_prime = alpha + 0.5f * tan_clamped;
         if (alpha_prime > 1.0f)
             alpha_prime = 1.0f;
-        D = sh_div(D * alpha_sq, alpha_prime * alpha_prime);
+        D = D * alpha_sq / (alpha_prime * alpha_prime);
         }
     const float f0 = 0.08f * m.specular;
     const v3 F0 = lerp(m.metal, mk3(f0, f0, f0), base_color);
     const v3 F = F0 + (mk3(1.0f, 1.0f, 1.0f) - F0) * schlick(ldoth);
     const float ndotv_sq = ndotv * ndotv;
     const float ndotl_sq = ndotl * ndotl;
-    const float V1 = sh_rcp(ndotv + sh_sqrt(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
-    const float V2 = sh_rcp(ndotl + sh_sqrt(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
+    const float V1 = 1.0f / (ndotv + sqrtf(alpha_sq + ndotv_sq - alpha_sq * ndotv_sq));
+    const float V2 = 1.0f / (ndotl + sqrtf(alpha_sq + ndotl_sq - alpha_sq * ndotl_sq));
     return D * F * (V1 * V2);
     }
 
@@ -153,7 +166,7 @@ __device__ __forceinline__ void tangent_frame(v3 n, v3& t_x, v3& t_y)
     if (fabsf(n.z) >= 0.999f)
         up = mk3(1.0f, 0, 0);
     t_x = cross(up, n);
-    t_x = sh_normalize(t_x);
+    t_x = div_rcp(t_x, sqrtf(dot(t_x, t_x)));
     t_y = cross(n, t_x);
     }
 
@@ -161,21 +174,21 @@ __device__ __forceinline__ void tangent_frame(v3 n, v3& t_x, v3& t_y)
 __device__ inline float pdf_ggx(const Mat& m, v3 l, v3 v, v3 n)
     {
     v3 h = l + v;
-    h = sh_normalize(h);
+    h = div_true(h, sqrtf(dot(h, h)));
     const float ldoth = dot(l, h);
     const float ndoth = dot(n, h);
     const float alpha = m.roughness * m.roughness;
     const float alpha_sq = alpha * alpha;
     const float denom_rt = (1.0f + (alpha_sq - 1.0f) * ndoth * ndoth);
-    const float D = sh_div(alpha_sq, PI_F * denom_rt * denom_rt);
-    const float pdf_l = sh_div(D * ndoth, 4.0f * ldoth);
+    const float D = alpha_sq / (PI_F * denom_rt * denom_rt);
+    const float pdf_l = D * ndoth / (4.0f * ldoth);
     return (pdf_l > 0.0f) ? pdf_l : 0.0f;
     }
 // Material::pdfDiffuse, Material.h:306-313
 __device__ __forceinline__ float pdf_diffuse(v3 l, v3 n)
     {
     const float ndotl = dot(n, l);
-    return (ndotl > 0.0f) ? ndotl * INV_PI_F : 0.0f;
+    return (ndotl > 0.0f) ? ndotl / PI_F : 0.0f;
     }
 
 // ---------------------------------------------------------------- camera
@@ -243,7 +256,9 @@ __device__ inline void camera_ray(const CamDev& c, uint32_t W, uint32_t H, float
         const u4 r = philox(0, 0, sample, RNG_APERTURE, key0, seed);
         const float theta = u01(r.v[0]) * 2.0f * PI_F;
         const float rr = u01(r.v[1]) * c.a * 0.5f;
-        const v3 offset = rr * cosf(theta) * c.u + rr * sinf(theta) * c.v;
+        float st, ct;
+        cr_sincosf(theta, st, ct);
+        const v3 offset = rr * ct * c.u + rr * st * c.v;
         o = c.p + offset;
         }
     d = C - o;
@@ -261,7 +276,7 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
     {
     add_valid = false;
     const v3 Ng = at.Ng;
-    const v3 n = sh_normalize(Ng);
+    const v3 n = Ng * (1.0f / sqrtf(dot(Ng, Ng)));
     const v3 v = -rd;
     if (m.solid > 0.5f)
         {
@@ -288,12 +303,14 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
         // the two pdfs once; each lane performs its own lobe's operations.
         const bool diffuse = choice_mis <= 0.5f;
         const float alpha = m.roughness * m.roughness;
-        const float cos_theta = sh_sqrt(sh_div(1.0f - xi_y, 1.0f + (alpha * alpha - 1.0f) * xi_y));
-        const float sin_theta = sh_sqrt(1.0f - cos_theta * cos_theta);
-        const float radial = diffuse ? sh_sqrt(xi_x) : sin_theta;
+        const float cos_theta = sqrtf((1.0f - xi_y) / (1.0f + (alpha * alpha - 1.0f) * xi_y));
+        const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+        const float radial = diffuse ? sqrtf(xi_x) : sin_theta;
         const float angle = diffuse ? xi_y : xi_x;
-        const float zc = diffuse ? sh_sqrt(1.0f - xi_x) : cos_theta;
-        const v3 s_t = mk3(radial * sh_cos_rev(angle), radial * sh_sin_rev(angle), zc);
+        const float zc = diffuse ? sqrtf(1.0f - xi_x) : cos_theta;
+        float sn, cs;
+        cr_sincosf(2.0f * PI_F * angle, sn, cs);
+        const v3 s_t = mk3(radial * cs, radial * sn, zc);
         v3 t_x, t_y;
         tangent_frame(n, t_x, t_y);
         const v3 w = t_x * s_t.x + t_y * s_t.y + n * s_t.z;
@@ -301,12 +318,12 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
         const float pd = pdf_diffuse(l, n);
         const float pg = pdf_ggx(m, l, v, n);
         const float p_lobe = diffuse ? pd : pg;
-        factor = sh_div(sh_div(p_lobe, pd + pg), 0.5f * p_lobe);
+        factor = (p_lobe / (pd + pg)) / (0.5f * p_lobe);
         }
     if (transmit)
         {
         const v3 tc = get_color(m, at.color);
-        att = att * mk3(sh_sqrt(tc.x), sh_sqrt(tc.y), sh_sqrt(tc.z));
+        att = att * mk3(sqrtf(tc.x), sqrtf(tc.y), sqrtf(tc.z));
         }
     else
         {
@@ -327,7 +344,7 @@ __device__ inline bool path_hit(const Mat& m, const Attr& at, v3 ro, v3 rd, floa
     if (u01(rr.v[0]) > p_continue)
         done = true;
     else
-        att = att * sh_rcp(p_continue);
+        att = div_true(att, p_continue);
     return !done;
     }
 

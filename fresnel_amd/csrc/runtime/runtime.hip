@@ -6,6 +6,7 @@
 // and the sampling state (seed, sample count).  Every C entry point catches
 // exceptions and returns a status (include/fresnel_hip.h).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdlib>
 #include <cmath>
@@ -549,9 +550,14 @@ static Mat to_mat(const fh_material& m)
     }
 
 // ====================================================================== scene
+// Serial numbers of scenes: a scene created after another one was destroyed
+// can reuse its address, so caches keyed on a scene use its serial.
+static std::atomic<uint64_t> g_scene_serial {0};
+
 struct SceneObj
     {
     DeviceObj* dev = nullptr;
+    uint64_t serial = ++g_scene_serial; // unique per scene for the life of the process
     fh_user_camera camera {};
     fh_rgb bg {0, 0, 0};
     float bg_alpha = 0;
@@ -1096,7 +1102,7 @@ enum TracerType
 struct TileLists
     {
     bool valid = false;        // lists usable for the key below
-    const void* scene = nullptr;
+    uint64_t scene = 0;        // serial of the scene they were built for (0: none)
     uint64_t builds = ~0ull, refits = ~0ull;
     CamDev cam {};
     uint32_t W = 0, H = 0;
@@ -1157,13 +1163,13 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
     if (D.primary_tiles != 2 || !pinhole || sc.n_prims == 0 || tr.W < TILE || tr.H < TILE)
         {
         T.valid = false;
-        T.scene = nullptr;
+        T.scene = 0;
         return;
         }
-    if (T.scene == &sc && T.builds == sc.builds && T.refits == sc.refits && T.W == tr.W && T.H == tr.H
+    if (T.scene == sc.serial && T.builds == sc.builds && T.refits == sc.refits && T.W == tr.W && T.H == tr.H
         && std::memcmp(&T.cam, &cam, sizeof(CamDev)) == 0)
         return; // cached, usable or known unusable
-    T.scene = &sc;
+    T.scene = sc.serial;
     T.builds = sc.builds;
     T.refits = sc.refits;
     T.cam = cam;
@@ -1295,7 +1301,8 @@ static JobDev make_job(SceneObj& sc, TracerObj& tr, const fh_user_camera& cam_us
     J.linear = tr.linear.p;
     J.srgb = tr.srgb.p;
     const TileLists& T = tr.tiles;
-    if (T.valid && T.scene == &sc && T.W == tr.W && T.H == tr.H && std::memcmp(&T.cam, &J.cam, sizeof(CamDev)) == 0)
+    if (T.valid && T.scene == sc.serial && T.builds == sc.builds && T.refits == sc.refits && T.W == tr.W
+        && T.H == tr.H && std::memcmp(&T.cam, &J.cam, sizeof(CamDev)) == 0)
         {
         J.use_tiles = 1;
         J.tiles_x = T.tiles_x;
@@ -3351,6 +3358,28 @@ extern "C"
                 HIPCHECK(hipGetLastError());
                 HIPCHECK(hipStreamSynchronize(D.stream));
                 HIPCHECK(hipMemcpy(out, dout.p, 16 * (size_t)n, hipMemcpyDeviceToHost));
+            });
+        }
+    int fh_debug_sincos(fh_device dev, const float* x, uint32_t n, float* s, float* c)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                NONNULL(x);
+                NONNULL(s);
+                NONNULL(c);
+                DeviceObj& D = dev->obj;
+                DBuf<float> dx, ds, dc;
+                dx.ensure(n);
+                ds.ensure(n);
+                dc.ensure(n);
+                HIPCHECK(hipMemcpy(dx.p, x, 4 * (size_t)n, hipMemcpyHostToDevice));
+                k_debug_sincos<<<blocks(n, 256), 256, 0, D.stream>>>(dx.p, n, ds.p, dc.p);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipStreamSynchronize(D.stream));
+                HIPCHECK(hipMemcpy(s, ds.p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+                HIPCHECK(hipMemcpy(c, dc.p, 4 * (size_t)n, hipMemcpyDeviceToHost));
             });
         }
     int fh_debug_primary(fh_tracer tr, fh_scene scene, uint32_t sample, float* rays, uint32_t* geom_id, uint32_t* prim,

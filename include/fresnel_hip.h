@@ -307,6 +307,9 @@ int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previ
 int fh_debug_copy(fh_device dev, uint64_t bytes, float* ms);
 /* Philox4x32-10 on the device: in[6k..6k+5] = ctr0..3, key0, key1 -> out[4k..] */
 int fh_debug_philox(fh_device dev, const uint32_t* in, uint32_t n, uint32_t* out);
+/* The shading code's sinf/cosf of lobe and aperture angles (correctly
+ * rounded for 0 <= x <= 8) for n arguments: s[k] = sinf(x[k]), c[k] = cosf(x[k]). */
+int fh_debug_sincos(fh_device dev, const float* x, uint32_t n, float* s, float* c);
 /* Primary rays and closest hits of pass `sample` for every pixel of a Path
  * tracer's image: rays[6p] = org.xyz, dir.xyz; geom_id/prim = 0xffffffff on a
  * miss; t = +inf on a miss.  Any pointer may be NULL. */

@@ -60,6 +60,8 @@ def lib():
     L.oracle_count_ties.argtypes = [P, fp, sz, F, up]
     L.oracle_sphere_t64.argtypes = [fp, fp, fp, F, ctypes.c_double]
     L.oracle_sphere_t64.restype = ctypes.c_double
+    L.oracle_cr_sincosf.argtypes = [fp, sz, fp, fp]
+    L.oracle_sphere_t64_batch.argtypes = [fp, fp, fp, sz, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 

@@ -319,6 +319,20 @@ static rgb brdf_specular(const Material& m, v3 l, v3 v, v3 n, rgb sc, float ligh
     float V = V1 * V2;
     return D * F * V;
     }
+// cosf / sinf of the lobe and aperture samples (Material.h:240,287-288,
+// Camera.h:201).  The C library's cosf/sinf are not specified to the last bit
+// (glibc's, CUDA's and other libraries' differ in rare last-ulp cases); the
+// checker takes the correctly rounded value (evaluated in double, rounded
+// once), which those libraries return for all but near-midpoint arguments and
+// which the device computes the same way.
+static inline float cr_cosf(float x)
+    {
+    return (float)std::cos((double)x);
+    }
+static inline float cr_sinf(float x)
+    {
+    return (float)std::sin((double)x);
+    }
 static inline void tangent_frame(v3 n, v3& t_x, v3& t_y) // Material.h:243-249, 293-299
     {
     v3 up = mk(0, 0, 1.0f);
@@ -334,7 +348,7 @@ static v3 sample_ggx(const Material& m, v2 xi, v3 v, v3 n) // Material.h:230-256
     float phi = 2.0f * PI_F * xi.x;
     float cos_theta = sqrtf((1.0f - xi.y) / (1.0f + (alpha * alpha - 1.0f) * xi.y));
     float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
-    v3 h_t = mk(sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta);
+    v3 h_t = mk(sin_theta * cr_cosf(phi), sin_theta * cr_sinf(phi), cos_theta);
     v3 t_x, t_y;
     tangent_frame(n, t_x, t_y);
     v3 h = t_x * h_t.x + t_y * h_t.y + n * h_t.z;
@@ -360,8 +374,8 @@ static v3 sample_diffuse(v2 xi, v3 v, v3 n) // Material.h:282-304
     (void)v;
     const float r = sqrtf(xi.x);
     const float theta = 2 * PI_F * xi.y;
-    const float x = r * cosf(theta);
-    const float y = r * sinf(theta);
+    const float x = r * cr_cosf(theta);
+    const float y = r * cr_sinf(theta);
     v3 v_t = mk(x, y, sqrtf(1.0f - xi.x));
     v3 t_x, t_y;
     tangent_frame(n, t_x, t_y);
@@ -466,7 +480,7 @@ static void generate_ray(const Camera& c, unsigned i, unsigned j, unsigned sampl
     u4 r = philox(u4 {{0, 0, sample, RNG_APERTURE}}, j * c.W + i, c.seed);
     float theta = u01(r.v[0]) * 2.0f * PI_F;
     float rr = u01(r.v[1]) * c.a * 0.5f;
-    v3 offset = rr * cosf(theta) * c.u + rr * sinf(theta) * c.v;
+    v3 offset = rr * cr_cosf(theta) * c.u + rr * cr_sinf(theta) * c.v;
     o = c.p + offset;
     d = C - o;
     d = d * (1.0f / sqrtf(dot(d, d)));
@@ -1943,33 +1957,82 @@ extern "C"
             th.join();
         }
     // Count of candidate primitives whose reported t equals the winner's t
-    // exactly (exact ties), for the hit of each ray.
+    // exactly (exact ties), for the hit of each ray.  A primitive whose
+    // computed distance equals the winner's lies in no box whose cull bound
+    // (a lower bound of the computed distances of its primitives) exceeds that
+    // distance, so the hierarchy is searched with that bound; a brute-force
+    // scene tests every primitive.
     void oracle_count_ties(void* sp, const float* rays, size_t n, float tnear, unsigned* ties)
         {
         const Scene& s = *(Scene*)sp;
-        for (size_t k = 0; k < n; k++)
-            {
+        auto count_one = [&](size_t k)
+        {
             const float* r = rays + 6 * k;
             v3 o = ld3(r), d = ld3(r + 3);
             HitRec h = closest_hit(s, o, d, tnear);
             unsigned c = 0;
-            if (h.hit)
+            if (!h.hit)
+                return c;
+            v3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+            auto test = [&](unsigned q)
+            {
+                float tmin, tmax;
+                slab(o, inv, s.ref_box[q], tmin, tmax);
+                if (!(tmin <= tmax && tmax >= tnear))
+                    return;
+                Hit hh;
+                const PrimRef& pr = s.refs[q];
+                if (intersect_prim(s.geoms[pr.geom], pr.prim, o, d, tnear, hh) && hh.t == h.t)
+                    c++;
+            };
+            if (s.brute)
                 {
-                v3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
                 for (unsigned q = 0; q < s.refs.size(); q++)
-                    {
-                    float tmin, tmax;
-                    slab(o, inv, s.ref_box[q], tmin, tmax);
-                    if (!(tmin <= tmax && tmax >= tnear))
-                        continue;
-                    Hit hh;
-                    const PrimRef& pr = s.refs[q];
-                    if (intersect_prim(s.geoms[pr.geom], pr.prim, o, d, tnear, hh) && hh.t == h.t)
-                        c++;
-                    }
+                    test(q);
+                return c;
                 }
-            ties[k] = c;
-            }
+            std::vector<int> stack;
+            stack.push_back(0);
+            while (!stack.empty())
+                {
+                const Node& nd = s.nodes[stack.back()];
+                stack.pop_back();
+                float tmin, tmax;
+                slab(o, inv, nd.box, tmin, tmax);
+                if (!(tmin <= tmax && tmax >= tnear))
+                    continue;
+                if (cull_bound(tmin, tmax, s.cull_margin) > h.t)
+                    continue;
+                if (nd.left < 0)
+                    {
+                    for (unsigned q = 0; q < nd.count; q++)
+                        test(nd.first + q);
+                    continue;
+                    }
+                stack.push_back(nd.left);
+                stack.push_back(nd.right);
+                }
+            return c;
+        };
+        const int nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+        std::atomic<size_t> next {0};
+        auto worker = [&]()
+        {
+            for (;;)
+                {
+                const size_t k0 = next.fetch_add(64);
+                if (k0 >= n)
+                    return;
+                for (size_t k = k0; k < std::min(n, k0 + 64); k++)
+                    ties[k] = count_one(k);
+                }
+        };
+        std::vector<std::thread> pool;
+        for (int k = 1; k < nthreads; k++)
+            pool.emplace_back(worker);
+        worker();
+        for (auto& th : pool)
+            th.join();
         }
     // FP64 sphere distance: the root of |o + t d - p|^2 = r^2 (d taken as
     // given, a = d.d) nearest past tnear, with the discriminant in the stable
@@ -1992,5 +2055,22 @@ extern "C"
         if (t1 > tnear)
             return t1;
         return NAN;
+        }
+    // The checker's cosf / sinf of lobe and aperture angles, for n arguments.
+    void oracle_cr_sincosf(const float* x, size_t n, float* s, float* c)
+        {
+        for (size_t k = 0; k < n; k++)
+            {
+            s[k] = cr_sinf(x[k]);
+            c[k] = cr_cosf(x[k]);
+            }
+        }
+    // oracle_sphere_t64 for n rays (6 floats each) against the sphere
+    // (xyz[3k..3k+2], r[k]) each one hit.
+    void oracle_sphere_t64_batch(const float* rays, const float* xyz, const float* r, size_t n, double tnear,
+                                 double* out)
+        {
+        for (size_t k = 0; k < n; k++)
+            out[k] = oracle_sphere_t64(rays + 6 * k, rays + 6 * k + 3, xyz + 3 * k, r[k], tnear);
         }
     }

@@ -85,3 +85,27 @@ def test_fastdiv_matches_integer_division():
         k = rng.integers(0, ((1 << 32) - 1) // d + 1, 200, dtype=numpy.uint64)
         n = numpy.concatenate([n, k * numpy.uint64(d), k * numpy.uint64(d) - numpy.uint64(1)]) % numpy.uint64(1 << 32)
         assert numpy.array_equal(_fastdiv(n, d), n // numpy.uint64(d)), d
+
+
+def test_sincos_hard_table_is_complete_and_correct(tmp_path):
+    """The device's table of hard-to-round sine/cosine arguments equals a fresh exhaustive search over every
+    float in [0, 8] (gen_sincos_hard.cpp --check), and each tabulated value is the nearest float to the
+    sine/cosine evaluated in 80-bit extended precision."""
+    import pathlib
+    import re
+    import subprocess
+    rt = pathlib.Path(__file__).resolve().parent.parent / 'fresnel_amd' / 'csrc' / 'rt'
+    exe = tmp_path / 'gen_sincos_hard'
+    subprocess.run(['g++', '-O2', '-std=c++17', '-ffp-contract=off', '-pthread', str(rt / 'gen_sincos_hard.cpp'),
+                    '-lquadmath', '-o', str(exe)], check=True)
+    subprocess.run([str(exe), '--check', str(rt / 'sincos_hard.h')], check=True)
+    rows = re.findall(r'\{(0x[0-9a-fp.+-]+)f, (0x[0-9a-fp.+-]+)f, (0x[0-9a-fp.+-]+)f\}',
+                      (rt / 'sincos_hard.h').read_text())
+    assert len(rows) > 50
+    for xs, ss, cs in rows:
+        x = numpy.longdouble(float.fromhex(xs))
+        for got, exact in ((float.fromhex(ss), numpy.sin(x)), (float.fromhex(cs), numpy.cos(x))):
+            g = numpy.float32(got)
+            err = abs(numpy.longdouble(g) - exact)
+            for nb in (numpy.nextafter(g, numpy.float32(2)), numpy.nextafter(g, numpy.float32(-2))):
+                assert err <= abs(numpy.longdouble(nb) - exact)

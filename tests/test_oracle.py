@@ -70,6 +70,26 @@ def test_u01_closed_form():
     assert numpy.all(v > 0) and numpy.all(v <= 1) and numpy.all(numpy.diff(v) >= 0)
 
 
+def test_cr_sincosf_is_correctly_rounded():
+    """The checker's cosf/sinf of lobe and aperture angles are the correctly rounded values: each lies
+    within half an ulp of cos/sin evaluated in 80-bit extended precision, and is the nearest float to it."""
+    rng = numpy.random.default_rng(2)
+    x = (numpy.float32(2.0) * numpy.float32(math.pi)
+         * rng.uniform(0, 1, 200000).astype(numpy.float32)).astype(numpy.float32)
+    x[:4] = [0.0, numpy.float32(math.pi / 2), numpy.float32(math.pi), numpy.float32(2 * math.pi)]
+    s = numpy.zeros_like(x)
+    c = numpy.zeros_like(x)
+    oracle.lib().oracle_cr_sincosf(ptr(x), len(x), ptr(s), ptr(c))
+    xl = x.astype(numpy.longdouble)
+    for got, exact in ((s, numpy.sin(xl)), (c, numpy.cos(xl))):
+        err = numpy.abs(got.astype(numpy.longdouble) - exact)
+        # the neighbouring floats are no closer than the result
+        up = numpy.nextafter(got, numpy.float32(numpy.inf)).astype(numpy.longdouble)
+        dn = numpy.nextafter(got, numpy.float32(-numpy.inf)).astype(numpy.longdouble)
+        assert numpy.all(err <= numpy.abs(up - exact)) and numpy.all(err <= numpy.abs(dn - exact))
+    assert c[0] == 1.0 and s[0] == 0.0
+
+
 def _sphere_scene(pos, rad, brute=False):
     L = oracle.lib()
     h = L.oracle_scene_new()
@@ -153,6 +173,32 @@ def test_bvh_equals_brute_force(seed):
     b = _trace(hf, rays)
     for x, y in zip(a[:3], b[:3]):
         numpy.testing.assert_array_equal(x, y)
+    oracle.lib().oracle_scene_free(hb)
+    oracle.lib().oracle_scene_free(hf)
+
+
+def test_tie_count_bvh_equals_brute_force():
+    """Exact-tie counts found through the hierarchy equal brute force over every primitive,
+    on a scene where a quarter of the spheres are duplicated (every hit on them is a tie)."""
+    rng = numpy.random.default_rng(11)
+    n = 2000
+    pos = rng.uniform(-5, 5, (n, 3)).astype(numpy.float32)
+    rad = rng.uniform(0.1, 0.6, n).astype(numpy.float32)
+    pos = numpy.concatenate([pos, pos[:n // 4]])
+    rad = numpy.concatenate([rad, rad[:n // 4]])
+    hb = _sphere_scene(pos, rad)
+    hf = _sphere_scene(pos, rad, brute=True)
+    m = 4000
+    o = rng.uniform(-6, 6, (m, 3)).astype(numpy.float32)
+    d = rng.normal(size=(m, 3))
+    d = (d / numpy.linalg.norm(d, axis=1, keepdims=True)).astype(numpy.float32)
+    rays = numpy.ascontiguousarray(numpy.concatenate([o, d], axis=1), numpy.float32)
+    tb = numpy.zeros(m, numpy.uint32)
+    tf = numpy.zeros(m, numpy.uint32)
+    oracle.lib().oracle_count_ties(hb, ptr(rays), m, 1e-3, ptr(tb, U32))
+    oracle.lib().oracle_count_ties(hf, ptr(rays), m, 1e-3, ptr(tf, U32))
+    numpy.testing.assert_array_equal(tb, tf)
+    assert (tf >= 2).sum() > 50  # the duplicates do produce ties
     oracle.lib().oracle_scene_free(hb)
     oracle.lib().oracle_scene_free(hf)
 
