@@ -644,6 +644,7 @@ PYBIND11_MODULE(_gpu, m)
                  check(fh_tracer_histogram(t->h, out.mutable_data()));
                  return out;
              })
+        .def("handle", [](std::shared_ptr<Tracer> t) { return (uintptr_t)t->h; })
         .def("enableHighlightWarning", &Tracer::enableHighlightWarning)
         .def("disableHighlightWarning", &Tracer::disableHighlightWarning)
         .def("getSeed", &Tracer::getSeed)

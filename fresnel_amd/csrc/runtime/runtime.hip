@@ -122,6 +122,17 @@ static unsigned blocks(size_t n, unsigned b)
     return (unsigned)((n + b - 1) / b);
     }
 
+// ====================================================================== allocation-free scratch
+// Device memory of fh_tracer_histogram_noalloc: the 4 x 512 bin counts its
+// kernel accumulates.  Each GPU the library runs on holds its own copy of
+// this array until the process ends.  In host code the name is only a
+// stand-in: DeviceObj::scratch holds the array's device address on that GPU,
+// taken with hipGetSymbolAddress when the Device is created, and that address
+// is what kernels and copies receive.  Every call on one GPU shares the
+// array (calls that overlap on one GPU race); each call clears it first.
+constexpr size_t NOALLOC_SCRATCH_WORDS = 4 * HIST_BINS; // 8 KiB
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
 // ====================================================================== device
 constexpr int TRAV_BLOCK = 128;
 constexpr int TRAV_SS = 16;
@@ -161,6 +172,7 @@ struct DeviceObj
     RenderScratch rs, rs2; // render scratch of the two batches in flight
     hipEvent_t acc_done[2] = {nullptr, nullptr}; // last accumulation enqueued on stream / stream2
     DBuf<unsigned long long> counters;
+    uint32_t* scratch = nullptr;  // device address of this GPU's allocation-free scratch array
     uint32_t* h_pinned = nullptr; // 16 uint32 of pinned host memory for counts
     uint32_t* h_counts = nullptr; // pinned copy of a queue's segment counters and flag
     bool stats_on = false;
@@ -2097,6 +2109,8 @@ extern "C"
                 D.limit = limit;
                 D.hip_dev = 0;
                 HIPCHECK(hipSetDevice(D.hip_dev));
+                // this GPU's copy of the allocation-free scratch array
+                HIPCHECK(hipGetSymbolAddress((void**)&D.scratch, HIP_SYMBOL(scratch)));
                 hipDeviceProp_t p;
                 HIPCHECK(hipGetDeviceProperties(&p, D.hip_dev));
                 D.n_cu = p.multiProcessorCount;
@@ -2849,6 +2863,34 @@ extern "C"
                 k_histogram<<<std::min(blocks(n, 256), 1024u), 256, 0, s>>>(T.linear.p, n, T.hist.p);
                 HIPCHECK(hipGetLastError());
                 HIPCHECK(hipMemcpyAsync(counts, T.hist.p, 4 * HIST_BINS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                HIPCHECK(hipStreamSynchronize(s));
+            });
+        }
+    int fh_tracer_histogram_noalloc(fh_tracer tr, uint32_t* counts)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(counts);
+                TracerObj& T = tr->obj;
+                DeviceObj& D = *T.dev;
+                static_assert(4 * HIST_BINS <= NOALLOC_SCRATCH_WORDS, "histogram bins exceed the scratch array");
+                const uint64_t n = (uint64_t)T.W * T.H;
+                if (n == 0 || n > 0xffffffffull)
+                    throw Error(FH_ERR_INVALID, "histogram: the image has " + std::to_string(n)
+                                                    + " pixels; the allocation-free histogram takes 1 to 2^32 - 1");
+                if (!D.scratch)
+                    throw Error(FH_ERR_STATE, "histogram: the device's scratch array has no address");
+                std::lock_guard<std::mutex> lock(D.mu);
+                hipStream_t s = D.stream;
+                // cleared by a kernel (no memset node if a caller captures the stream)
+                k_fill_u32<<<blocks(4 * HIST_BINS, 256), 256, 0, s>>>(D.scratch, 4 * HIST_BINS, 0u);
+                HIPCHECK(hipGetLastError());
+                k_histogram<<<std::min(blocks(n, 256), 1024u), 256, 0, s>>>(T.linear.p, (uint32_t)n, D.scratch);
+                HIPCHECK(hipGetLastError());
+                HIPCHECK(hipMemcpyAsync(counts, D.scratch, 4 * HIST_BINS * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                        s));
                 HIPCHECK(hipStreamSynchronize(s));
             });
         }

@@ -208,6 +208,17 @@ int fh_tracer_output_host(fh_tracer tr, int which, void** data);
  * alpha > 0, lightness = 0.21 R + 0.72 G + 0.07 B.  Only the 8 KiB of counts
  * cross PCIe; the bin positions are k / 512, k = 1..512. */
 int fh_tracer_histogram(fh_tracer tr, uint32_t counts[2048]);
+/* fh_tracer_histogram without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The bin
+ * counts accumulate in a file-scope device array of 8 KiB (the library's
+ * fixed maximum, exactly the 4 x 512 counts; every GPU the library runs on
+ * holds one until the process ends) instead of a buffer the tracer
+ * allocates.  Images of 1 to 2^32 - 1 pixels; FH_ERR_INVALID otherwise.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several threads or several Device handles) race.  Same result as
+ * fh_tracer_histogram. */
+int fh_tracer_histogram_noalloc(fh_tracer tr, uint32_t counts[2048]);
 /* enable/disableHighlightWarning, get/setSeed: cpu/Tracer.h:57-87 */
 int fh_tracer_enable_highlight_warning(fh_tracer tr, fh_rgb color);
 int fh_tracer_disable_highlight_warning(fh_tracer tr);
