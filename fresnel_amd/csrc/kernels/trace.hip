@@ -448,6 +448,84 @@ struct Queue
     uint32_t adj[QSEG];   // consumer: slot - index within segment s (s seg_cap - pre[s])
     };
 
+// Consumer view of a queue, written on the device by k_queue_view from its
+// producer's segment counters: the entry count, the segment prefix (as
+// Queue::pre/adj) and which of the next kernels act on it.  A bounce step's
+// kernels all read it, so no count goes back to the host between bounces.
+constexpr uint32_t VIEW_IDLE = 0, VIEW_BOUNCE = 1, VIEW_FINISH = 2;
+struct QueueView
+    {
+    uint32_t n;
+    uint32_t mode;
+    uint32_t pad[2];
+    uint32_t pre[QSEG];
+    uint32_t adj[QSEG];
+    };
+
+// A kernel of `mode` reading a queue: false when it has nothing to do;
+// otherwise n and q's segment prefix come from the view (no view: the values
+// passed by the host stand).  The view is the same for every thread.
+__device__ __forceinline__ bool take_view(const QueueView* __restrict__ view, uint32_t mode, Queue& q, uint32_t& n)
+    {
+    if (!view)
+        return true;
+    if (view->mode != mode)
+        return false;
+    n = view->n;
+#pragma unroll
+    for (uint32_t s = 0; s < QSEG; s++)
+        {
+        q.pre[s] = view->pre[s];
+        q.adj[s] = view->adj[s];
+        }
+    return true;
+    }
+
+// One wavefront: the consumer view of the queue whose segment counters are
+// `count` (filled by the previous producer), then those counters and the
+// traversal's work counters cleared for the next producer and consumer.  The
+// view's mode is FINISH when at most `tail` entries are left or `force` is
+// set, BOUNCE for more, IDLE when the queue is empty or a segment overflowed
+// (then CTR_QUEUE_OVERFLOW is raised; the host reports it).  steps[2 step],
+// steps[2 step + 1] record the count and mode.
+__global__ __launch_bounds__(64) void k_queue_view(uint32_t* count, uint32_t seg_cap, uint32_t tail, uint32_t force,
+                                                   QueueView* view, uint32_t* work, uint32_t* steps, uint32_t step,
+                                                   unsigned long long* counters)
+    {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t c = lane < QSEG ? count[lane * QSEG_STRIDE] : 0u;
+    const uint32_t flag = count[QSEG_FLAG];
+    uint32_t incl = c;
+    for (uint32_t off = 1; off < QSEG; off <<= 1)
+        {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += t;
+        }
+    const uint32_t total = __shfl(incl, QSEG - 1, 64);
+    const bool bad = flag != 0 || __ballot(lane < QSEG && c > seg_cap) != 0;
+    const uint32_t mode = (bad || total == 0) ? VIEW_IDLE : ((total <= tail || force) ? VIEW_FINISH : VIEW_BOUNCE);
+    if (lane < QSEG)
+        {
+        const uint32_t pre = incl - c;
+        view->pre[lane] = pre;
+        view->adj[lane] = lane * seg_cap - pre;
+        count[lane * QSEG_STRIDE] = 0u;
+        }
+    if (lane < 8)
+        work[lane * QSEG_STRIDE] = 0u;
+    if (lane == 0)
+        {
+        view->n = total;
+        view->mode = mode;
+        steps[2 * step] = total;
+        steps[2 * step + 1] = mode;
+        count[QSEG_FLAG] = 0u;
+        if (bad)
+            atomicOr((unsigned int*)(counters + CTR_QUEUE_OVERFLOW), 1u);
+        }
+    }
+
 // slot - index for dense index i (pre is non-decreasing, pre[0] = 0)
 __device__ __forceinline__ uint32_t seg_adj(const Queue& q, uint32_t i)
     {
@@ -686,13 +764,22 @@ struct Claims
     {
     uint32_t* work; // one counter per range, QSEG_STRIDE words apart
     uint32_t chunk, n, r, ranges, tried;
+    // q.pre[2 k] (constant indices only, so the prefix stays in registers)
+    __device__ __forceinline__ static uint32_t even_pre(const Queue& q, uint32_t k)
+        {
+        uint32_t v = 0;
+#pragma unroll
+        for (uint32_t j = 1; j < QSEG / 2; j++)
+            v = (k == j) ? q.pre[2 * j] : v;
+        return v;
+        }
     __device__ __forceinline__ uint32_t lo(const Queue& q, uint32_t k) const
         {
-        return ranges == 1 ? 0u : q.pre[2 * k];
+        return ranges == 1 ? 0u : even_pre(q, k);
         }
     __device__ __forceinline__ uint32_t hi(const Queue& q, uint32_t k) const
         {
-        return (ranges == 1 || k == ranges - 1) ? n : q.pre[2 * k + 2];
+        return (ranges == 1 || k == ranges - 1) ? n : even_pre(q, k + 1);
         }
     // next claimed [base, end) of the calling wavefront, or base = EMPTY
     __device__ __forceinline__ void next(const Queue& q, uint32_t lane, uint32_t& base, uint32_t& end)
@@ -721,11 +808,13 @@ struct Claims
 // The spheres-only variant is held to 71 registers (7 wavefronts per SIMD, no
 // spills): its search is latency-bound, so wavefronts in flight matter.
 template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TYPES == T_SPHERES ? 7 : 1))) void k_traverse(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
-                                                    uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
-                                                    uint32_t bands, uint2* gstack, uint32_t gstride,
-                                                    unsigned long long* counters)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TYPES == T_SPHERES ? 7 : 1))) void k_traverse(
+    const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q, uint32_t n, const QueueView* __restrict__ view,
+    uint4* hits, uint32_t* work, uint32_t chunk, uint32_t bands, uint2* gstack, uint32_t gstride,
+    unsigned long long* counters)
     {
+    if (!take_view(view, VIEW_BOUNCE, q, n))
+        return;
     __shared__ uint2 lds[SS * BLOCK];
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     LdsStack<SS> st;
@@ -963,34 +1052,51 @@ __device__ __forceinline__ bool shade_queued(const JobDev& J, const Queue& in, c
 // single-job batch, most of a multi-job one, whose queues are filled chunk by
 // chunk) reads the job record through scalar loads; otherwise each ray reads
 // its own job's record.
-template<int BS, uint32_t TYPES>
-__global__ __launch_bounds__(BS) void k_shade(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue in,
-                                              const uint4* __restrict__ hits, uint32_t n, Queue out,
-                                              uint4* records, uint32_t max_depth, unsigned long long* counters)
+// Units of BS rays from unit0 + blockIdx.x; LOOP: grid-stride over every
+// further unit, else one unit per workgroup (the host sizes the grid from the
+// live-path fraction recent renders had at this bounce and launches the LOOP
+// form from the first unit beyond it, so no ray is left out).  Unit u appends
+// to segment u % QSEG.  The LOOP form is held to 7 wavefronts per SIMD: its
+// loop would otherwise keep the loop's constants (Philox, the sine/cosine
+// polynomial) in registers across iterations instead of rematerializing them.
+template<int BS, uint32_t TYPES, bool LOOP>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(LOOP ? 7 : 1))) void k_shade(
+    const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue in, uint32_t n, const QueueView* __restrict__ view,
+    uint32_t unit0, const uint4* __restrict__ hits, Queue out, uint4* records, uint32_t max_depth,
+    unsigned long long* counters)
     {
+    if (!take_view(view, VIEW_BOUNCE, in, n))
+        return;
     __shared__ uint32_t sh[BS / 64 + 1];
-    const uint32_t i = blockIdx.x * BS + threadIdx.x;
-    const uint32_t first = blockIdx.x * BS;
-    const uint32_t sl = q_slot(in, i, first, min(n, first + BS) - 1u);
-    bool cont = false;
-    v3 att, no, nd;
-    uint4 mt = make_uint4(0, 0, 0, 0);
-    if (i < n)
+    const uint32_t units = (n + BS - 1) / BS;
+    for (uint32_t unit = unit0 + blockIdx.x; unit < units; unit += gridDim.x)
         {
-        mt = in.meta[sl];
-        const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
-        if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
-            cont = shade_queued<TYPES>(jobs[j0], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
-        else
-            cont = shade_queued<TYPES>(jobs[mt.x], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
-        }
-    const uint32_t slot = block_append<BS>(cont, out, blockIdx.x % QSEG, sh);
-    if (cont && slot != EMPTY)
-        {
-        out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
-        out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
-        out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
-        out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
+        const uint32_t first = unit * BS;
+        const uint32_t i = first + threadIdx.x;
+        const uint32_t sl = q_slot(in, i, first, min(n, first + BS) - 1u);
+        bool cont = false;
+        v3 att, no, nd;
+        uint4 mt = make_uint4(0, 0, 0, 0);
+        if (i < n)
+            {
+            mt = in.meta[sl];
+            const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
+            if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
+                cont = shade_queued<TYPES>(jobs[j0], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
+            else
+                cont = shade_queued<TYPES>(jobs[mt.x], in, hits, i, sl, mt, records, max_depth, counters, att, no,
+                                           nd);
+            }
+        const uint32_t slot = block_append<BS>(cont, out, unit % QSEG, sh);
+        if (cont && slot != EMPTY)
+            {
+            out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
+            out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
+            out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
+            out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
+            }
+        if (!LOOP)
+            break;
         }
     }
 
@@ -1440,10 +1546,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 // record.  Same arithmetic as k_traverse + k_shade.
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ jobs, Queue in, uint32_t n,
-                                                  uint32_t* work, uint32_t chunk, uint2* gstack, uint32_t gstride,
-                                                  Queue out, uint4* records, uint32_t max_depth,
-                                                  unsigned long long* counters)
+                                                  const QueueView* __restrict__ view, uint32_t* work, uint32_t chunk,
+                                                  uint2* gstack, uint32_t gstride, Queue out, uint4* records,
+                                                  uint32_t max_depth, unsigned long long* counters)
     {
+    if (!take_view(view, VIEW_BOUNCE, in, n))
+        return;
     __shared__ uint2 lds[SS * BLOCK];
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     LdsStack<SS> st;
@@ -1526,10 +1634,12 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ job
 // per atomic, one chunk ahead.  Same per-ray arithmetic as k_traverse.
 template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
-                                                        uint32_t n, uint4* hits, uint32_t* work, uint32_t chunk,
-                                                        uint32_t refill, uint2* gstack, uint32_t gstride,
-                                                        unsigned long long* counters)
+                                                        uint32_t n, const QueueView* __restrict__ view, uint4* hits,
+                                                        uint32_t* work, uint32_t chunk, uint32_t refill,
+                                                        uint2* gstack, uint32_t gstride, unsigned long long* counters)
     {
+    if (!take_view(view, VIEW_BOUNCE, q, n))
+        return;
     __shared__ uint2 lds[SS * BLOCK];
     const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
     LdsStack<SS> st;
@@ -1662,37 +1772,43 @@ __device__ __forceinline__ void finish_path(const JobDev& J, const uint4& mt, v3
     }
 
 // A wavefront whose paths all belong to one job reads the job record through
-// scalar loads, as in k_shade.
+// scalar loads, as in k_shade.  Grid-stride: each thread finishes paths
+// i, i + G, ... (G = grid threads; gstride must be G).
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_finish(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue in,
-                                                  uint32_t n, uint4* records, uint32_t max_depth, uint2* gstack,
-                                                  uint32_t gstride, unsigned long long* counters)
+                                                  uint32_t n, const QueueView* __restrict__ view, uint4* records,
+                                                  uint32_t max_depth, uint2* gstack, uint32_t gstride,
+                                                  unsigned long long* counters)
     {
-    __shared__ uint2 lds[SS * BLOCK];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n)
+    if (!take_view(view, VIEW_FINISH, in, n))
         return;
+    __shared__ uint2 lds[SS * BLOCK];
     LdsStack<SS> st;
     st.lds = lds + threadIdx.x;
     st.lstride = BLOCK;
-    st.glob = gstack + i;
+    st.glob = gstack + blockIdx.x * BLOCK + threadIdx.x;
     st.gstride = gstride;
     st.sp = 0;
-    const uint32_t first = blockIdx.x * BLOCK;
-    const uint32_t sl = q_slot(in, i, first, min(n, first + BLOCK) - 1u);
-    const uint4 mt = in.meta[sl];
-    const float4 a = in.o[sl];
-    const float4 b = in.d[sl];
-    const float4 at4 = in.att[sl];
-    const v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
     uint32_t n_nodes = 0, n_prims = 0, n_rays = 0, err = 0;
-    const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
-    if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
-        finish_path<SS, COUNT, TYPES>(jobs[j0], mt, o, d, att, st, records, max_depth, counters, n_nodes, n_prims,
-                                      n_rays, err);
-    else
-        finish_path<SS, COUNT, TYPES>(jobs[mt.x], mt, o, d, att, st, records, max_depth, counters, n_nodes, n_prims,
-                                      n_rays, err);
+    for (uint32_t first = blockIdx.x * BLOCK; first < n; first += gridDim.x * BLOCK)
+        {
+        const uint32_t i = first + threadIdx.x;
+        if (i >= n)
+            break;
+        const uint32_t sl = q_slot(in, i, first, min(n, first + BLOCK) - 1u);
+        const uint4 mt = in.meta[sl];
+        const float4 a = in.o[sl];
+        const float4 b = in.d[sl];
+        const float4 at4 = in.att[sl];
+        const v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z), att = mk3(at4.x, at4.y, at4.z);
+        const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
+        if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
+            finish_path<SS, COUNT, TYPES>(jobs[j0], mt, o, d, att, st, records, max_depth, counters, n_nodes,
+                                          n_prims, n_rays, err);
+        else
+            finish_path<SS, COUNT, TYPES>(jobs[mt.x], mt, o, d, att, st, records, max_depth, counters, n_nodes,
+                                          n_prims, n_rays, err);
+        }
     if (COUNT)
         {
         atomicAdd(counters + CTR_TAIL + 0, (unsigned long long)n_rays);

@@ -143,6 +143,11 @@ struct KernelTime
     {
     std::string name;
     hipEvent_t a, b;
+    // a bounce-step kernel: the step's mode word (pinned host copy, valid once
+    // the render has finished) and the mode the kernel acts on; launches of
+    // other modes exit at once and are counted as "<name>_idle"
+    const uint32_t* mode = nullptr;
+    uint32_t acts_on = 0;
     };
 
 struct RenderScratch
@@ -157,7 +162,41 @@ struct RenderScratch
     DBuf<uint2> tail_stack;    // stack overflow of the tail kernel
     DBuf<JobDev> jobs;
     DBuf<uint32_t> pix_base;
+    DBuf<QueueView> view;      // consumer view of the queue a bounce step reads (k_queue_view)
+    DBuf<uint32_t> steps;      // count and mode of every bounce step (2 words each)
     };
+
+// Pinned host memory that grows on demand (contents are not preserved).
+struct PinnedBuf
+    {
+    uint8_t* p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf()
+        {
+        if (p)
+            (void)hipHostFree(p);
+        }
+    // only while no copy from it is pending
+    void ensure(size_t bytes)
+        {
+        if (bytes <= cap && p)
+            return;
+        if (p)
+            (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        void* q = nullptr;
+        HIPCHECK(hipHostMalloc(&q, std::max<size_t>(bytes, 64), hipHostMallocDefault));
+        p = (uint8_t*)q;
+        cap = std::max<size_t>(bytes, 64);
+        }
+    };
+
+// Bounce steps a batch can be given: the last one finishes every path left.
+constexpr uint32_t MAX_STEPS = 64;
 
 struct DeviceObj
     {
@@ -170,6 +209,11 @@ struct DeviceObj
     int limit = -1;
     uint32_t trav_grid = 0;
     RenderScratch rs, rs2; // render scratch of the two batches in flight
+    PinnedBuf stage;       // job tables, pixel bases and step records of one render's batches
+    uint32_t steps_est = 8; // bounce steps per batch (adapted to the depths of recent renders)
+    float live_frac[MAX_STEPS] = {}; // fraction of a batch's paths queued at each bounce step, recent maximum
+    const uint32_t* tag_mode = nullptr; // mode word for the launch being recorded (stats)
+    uint32_t tag_acts_on = 0;
     hipEvent_t acc_done[2] = {nullptr, nullptr}; // last accumulation enqueued on stream / stream2
     DBuf<unsigned long long> counters;
     uint32_t* scratch = nullptr;  // device address of this GPU's allocation-free scratch array
@@ -232,6 +276,8 @@ struct DeviceObj
         kt.name = nm;
         kt.a = event();
         kt.b = event();
+        kt.mode = tag_mode;
+        kt.acts_on = tag_acts_on;
         hipStream_t ls = lstream ? lstream : stream;
         HIPCHECK(hipEventRecord(kt.a, ls));
         f();
@@ -260,6 +306,8 @@ struct DeviceObj
             {
             float ms = 0;
             HIPCHECK(hipEventElapsedTime(&ms, t.a, t.b));
+            if (t.mode && *t.mode != t.acts_on)
+                t.name += "_idle";
             bool found = false;
             for (auto& s : last_stats)
                 if (s.first == t.name)
@@ -286,6 +334,7 @@ struct TravArgs
     uint32_t n_jobs;
     Queue q;
     uint32_t n;
+    const QueueView* view; // device-written count and segment prefix, or nullptr (n, q.pre/adj stand)
     uint4* hits;
     uint32_t* work;
     uint32_t chunk;
@@ -318,7 +367,7 @@ static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const T
         D.launch(nm,
                  [&]
                  {
-                     kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.chunk,
+                     kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.view, a.hits, a.work, a.chunk,
                                                      D.trav_refill, a.gstack, a.gstride, a.counters);
                  });
         return;
@@ -329,7 +378,7 @@ static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const T
     D.launch(nm,
              [&]
              {
-                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.hits, a.work, a.chunk,
+                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.view, a.hits, a.work, a.chunk,
                                                  D.xcd_bands == 2 ? 1u : 0u, a.gstack, a.gstride, a.counters);
              });
     }
@@ -410,6 +459,7 @@ struct BounceArgs
     const JobDev* jobs;
     Queue in;
     uint32_t n;
+    const QueueView* view;
     uint32_t* work;
     uint32_t chunk;
     uint2* gstack;
@@ -428,8 +478,8 @@ static void launch_bounce(DeviceObj& D, hipStream_t s, const BounceArgs& a)
     D.launch("bounce",
              [&]
              {
-                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.in, a.n, a.work, a.chunk, a.gstack, a.gstride, a.out,
-                                                 a.records, MAX_DEPTH, a.counters);
+                 kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.in, a.n, a.view, a.work, a.chunk, a.gstack, a.gstride,
+                                                 a.out, a.records, MAX_DEPTH, a.counters);
              });
     }
 
@@ -1339,27 +1389,6 @@ static uint32_t scene_types(const SceneObj& sc)
     return m;
     }
 
-// Counts of the QSEG segments a producer filled (counters QSEG_STRIDE words
-// apart, then the overflow flag): sets the consumer view of q (pre, adj) and
-// returns the number of entries.
-static uint32_t read_segments(DeviceObj& D, hipStream_t s, const uint32_t* dcount, Queue& q)
-    {
-    const size_t words = QSEG_FLAG + 1;
-    HIPCHECK(hipMemcpyAsync(D.h_counts, dcount, words * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    REQUIRE(D.h_counts[QSEG_FLAG] == 0, "internal error: a ray queue segment overflowed");
-    uint64_t total = 0;
-    for (uint32_t k = 0; k < QSEG; k++)
-        {
-        const uint32_t c = D.h_counts[k * QSEG_STRIDE];
-        REQUIRE(c <= q.seg_cap, "internal error: ray queue segment count exceeds its capacity");
-        q.pre[k] = (uint32_t)total;
-        q.adj[k] = k * q.seg_cap - (uint32_t)total;
-        total += c;
-        }
-    return (uint32_t)total;
-    }
-
 // Slots per queue segment for a batch: every producer appends at most
 // `per_unit` entries per unit, and segment s receives the units u with
 // u % QSEG == s.
@@ -1388,11 +1417,24 @@ static void set_order_divisors(JobDev& J)
     J.div_w = make_fastdiv(std::max(1u, J.W));
     }
 
-// One batch in flight: every job advances by jobs[j].K camera samples.  The
-// batch runs on one render scratch and stream; start() enqueues the primary
-// rays, each step() waits for the last producer's queue counts (on this
-// stream only) and enqueues the next bounce or the tail, and accumulate()
-// enqueues the Welford accumulation once every path has a record.
+// Pinned staging bytes of one batch: its job table, pixel bases and step
+// records (JobDev alignment kept for each part).
+static size_t stage_bytes(size_t n_jobs)
+    {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return up(n_jobs * sizeof(JobDev)) + up(n_jobs * sizeof(uint32_t)) + up(2 * MAX_STEPS * sizeof(uint32_t));
+    }
+
+// One batch: every job advances by jobs[j].K camera samples, on one render
+// scratch and stream.  start() enqueues the primary rays; enqueue_steps(B)
+// enqueues B bounce steps, each a k_queue_view that turns the last
+// producer's segment counters into the consumer view on the device, then the
+// traversal and shading (bounce) and the tail kernel (finish), which act only
+// when the view selects them; the last step finishes every path still
+// queued.  So the host never waits for a count: a batch is enqueued in one
+// go and its kernels find their work on the device.  accumulate() enqueues
+// the Welford accumulation.  The job table and step records pass through
+// `stage` (pinned host memory that must outlive the batch's copies).
 struct BatchExec
     {
     DeviceObj& D;
@@ -1403,13 +1445,15 @@ struct BatchExec
     bool spheres;
     std::vector<uint32_t> pix_base;
     uint64_t n_pri = 0, n_paths = 0, n_pix = 0, n_rec = 0;
-    uint32_t nj = 0, gstride = 0, n = 0;
+    uint32_t nj = 0, gstride = 0, seg_cap = 0, tail_threads = 0;
+    uint32_t n_steps = 0;
     int cur = 1;
-    bool paths_done = false;
     Queue Q[2];
+    uint8_t* stage;
+    uint32_t* h_steps = nullptr; // pinned copy of the step records (valid after the batch has run)
 
-    BatchExec(DeviceObj& d, RenderScratch& r, hipStream_t st, std::vector<JobDev> j, uint32_t t)
-        : D(d), R(r), s(st), jobs(std::move(j)), types(t), spheres(t == T_SPHERES)
+    BatchExec(DeviceObj& d, RenderScratch& r, hipStream_t st, std::vector<JobDev> j, uint32_t t, uint8_t* stg)
+        : D(d), R(r), s(st), jobs(std::move(j)), types(t), spheres(t == T_SPHERES), stage(stg)
         {
         }
 
@@ -1420,15 +1464,16 @@ struct BatchExec
         D.lstream = nullptr;
         }
 
-    void reset_segments()
+    // zero n words with a kernel (no memset node if a caller captures the stream)
+    void clear(uint32_t* p, uint32_t n)
         {
-        HIPCHECK(hipMemsetAsync(R.qseg.p, 0, (QSEG_FLAG + 1) * sizeof(uint32_t), s));
+        k_fill_u32<<<blocks(n, 256), 256, 0, s>>>(p, n, 0u);
+        HIPCHECK(hipGetLastError());
         }
 
-    void traverse(const Queue& q, uint32_t count, bool primary)
+    void traverse(const Queue& q, uint32_t count, const QueueView* view, bool primary)
         {
-        HIPCHECK(hipMemsetAsync(R.counters32.p, 0, 8 * QSEG_STRIDE * sizeof(uint32_t), s));
-        TravArgs a {R.jobs.p, nj, q, count, R.hits.p, R.counters32.p, D.trav_chunk, R.gstack.p, gstride,
+        TravArgs a {R.jobs.p, nj, q, count, view, R.hits.p, R.counters32.p, D.trav_chunk, R.gstack.p, gstride,
                     D.counters.p};
         D.lstream = s;
         dispatch_traverse(D, primary ? "traverse_primary" : "traverse", s, a, primary, spheres);
@@ -1457,7 +1502,7 @@ struct BatchExec
         for (const auto& J : jobs)
             max_l = std::max(max_l, J.L);
         const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
-        const uint32_t seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * groups, D.trav_chunk);
+        seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * groups, D.trav_chunk);
         const uint64_t cap = (uint64_t)QSEG * seg_cap;
         R.hits.ensure(std::max(n_pri, n_paths));
         for (int q = 0; q < 2; q++)
@@ -1472,11 +1517,23 @@ struct BatchExec
         R.qseg.ensure(QSEG_FLAG + 1);
         R.jobs.ensure(jobs.size());
         R.pix_base.ensure(jobs.size());
-        HIPCHECK(hipMemcpyAsync(R.jobs.p, jobs.data(), jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(R.pix_base.p, pix_base.data(), pix_base.size() * 4, hipMemcpyHostToDevice, s));
+        R.view.ensure(1);
+        R.steps.ensure(2 * MAX_STEPS);
+        // job table and pixel bases through pinned staging: the copies do not
+        // wait for the stream, so enqueueing never blocks on earlier batches
+        uint8_t* p = stage;
+        std::memcpy(p, jobs.data(), jobs.size() * sizeof(JobDev));
+        HIPCHECK(hipMemcpyAsync(R.jobs.p, p, jobs.size() * sizeof(JobDev), hipMemcpyHostToDevice, s));
+        p += (jobs.size() * sizeof(JobDev) + 255) & ~(size_t)255;
+        std::memcpy(p, pix_base.data(), pix_base.size() * 4);
+        HIPCHECK(hipMemcpyAsync(R.pix_base.p, p, pix_base.size() * 4, hipMemcpyHostToDevice, s));
+        p += (pix_base.size() * 4 + 255) & ~(size_t)255;
+        h_steps = (uint32_t*)p;
         gstride = D.trav_grid * TRAV_BLOCK; // max resident threads of any traversal variant
         R.gstack.ensure((size_t)(STACK_MAX - 8) * gstride);
-        R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * D.tail_rays);
+        // the tail kernel's grid-stride threads (at least tail_rays)
+        tail_threads = blocks(D.tail_rays, TRAV_BLOCK) * TRAV_BLOCK;
+        R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * tail_threads);
         for (int q = 0; q < 2; q++)
             {
             Q[q] = Queue {R.qo[q].p, R.qd[q].p, R.qatt[q].p, R.qmeta[q].p};
@@ -1484,10 +1541,10 @@ struct BatchExec
             Q[q].seg_cap = seg_cap;
             }
         nj = (uint32_t)jobs.size();
+        clear(R.counters32.p, 8 * QSEG_STRIDE);
+        clear(R.qseg.p, QSEG_FLAG + 1);
         if (D.fuse_primary == 2)
             {
-            HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), s));
-            reset_segments();
             PrimaryArgs a {R.jobs.p, nj, (uint32_t)n_pri, R.counters32.p, groups, R.gstack.p, gstride, Q[1],
                            R.records.p, D.counters.p};
             D.lstream = s;
@@ -1496,8 +1553,7 @@ struct BatchExec
             }
         else
             {
-            traverse(Q[0], (uint32_t)n_pri, true);
-            reset_segments();
+            traverse(Q[0], (uint32_t)n_pri, nullptr, true);
             launch("shade_primary",
                    [&]
                    {
@@ -1509,62 +1565,100 @@ struct BatchExec
         cur = 1;
         }
 
-    // Wait for the queue the last launches filled and enqueue the next work;
-    // false once every path has reached its record.
-    bool step()
+    // Enqueue `steps` bounce steps (1 <= steps <= MAX_STEPS); the last one
+    // finishes every path that is still queued.
+    void enqueue_steps(uint32_t steps)
         {
-        if (paths_done)
-            return false;
-        n = read_segments(D, s, R.qseg.p, Q[cur]);
-        if (n == 0)
+        auto* shade_one = spheres ? k_shade<SHADE_BLOCK, T_SPHERES, false> : k_shade<SHADE_BLOCK, T_ALL, false>;
+        auto* shade_rest = spheres ? k_shade<SHADE_BLOCK, T_SPHERES, true> : k_shade<SHADE_BLOCK, T_ALL, true>;
+        const uint32_t rest_grid = D.grid_for(reinterpret_cast<const void*>(shade_rest), SHADE_BLOCK);
+        for (uint32_t b = 0; b < steps; b++)
             {
-            paths_done = true;
-            return false;
-            }
-        if (n <= D.tail_rays)
-            {
-            const uint32_t cnt = n;
-            const Queue q = Q[cur];
+            const bool last = b + 1 == steps;
+            const uint32_t* mode = h_steps + 2 * b + 1;
+            launch("queue_view",
+                   [&]
+                   {
+                       k_queue_view<<<1, 64, 0, s>>>(R.qseg.p, seg_cap, D.tail_rays, last ? 1u : 0u, R.view.p,
+                                                     R.counters32.p, R.steps.p, b, D.counters.p);
+                   });
+            const Queue in = Q[cur], out = Q[1 - cur];
+            D.tag_mode = mode;
+            D.tag_acts_on = VIEW_BOUNCE;
+            if (D.fuse_bounce == 2)
+                {
+                BounceArgs a {R.jobs.p, in, 0u, R.view.p, R.counters32.p, D.trav_chunk, R.gstack.p, gstride, out,
+                              R.records.p, D.counters.p};
+                D.lstream = s;
+                dispatch_bounce(D, s, a, spheres);
+                D.lstream = nullptr;
+                }
+            else
+                {
+                traverse(in, 0u, R.view.p, false);
+                // one workgroup per 512 rays up to the expected count (the live
+                // fraction recent renders had at this step, with margin), the
+                // rest (if more are queued) by the grid-stride form
+                const double frac = std::min(1.0, 1.25 * (double)D.live_frac[b]);
+                const uint32_t one_grid = blocks((uint64_t)(frac * (double)n_paths) + 4096, SHADE_BLOCK);
+                launch("shade",
+                       [&]
+                       {
+                           shade_one<<<one_grid, SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, in, 0u, R.view.p, 0u, R.hits.p,
+                                                                     out, R.records.p, MAX_DEPTH, D.counters.p);
+                       });
+                D.tag_mode = nullptr;
+                launch("shade_rest",
+                       [&]
+                       {
+                           shade_rest<<<rest_grid, SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, in, 0u, R.view.p, one_grid,
+                                                                       R.hits.p, out, R.records.p, MAX_DEPTH,
+                                                                       D.counters.p);
+                       });
+                D.tag_mode = mode;
+                }
+            D.tag_acts_on = VIEW_FINISH;
             launch("finish",
                    [&]
                    {
-                       const dim3 g(blocks(cnt, TRAV_BLOCK));
                        auto* f = spheres ? (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_SPHERES>
                                                        : k_finish<TRAV_BLOCK, TRAV_SS, false, T_SPHERES>)
                                          : (D.count_on ? k_finish<TRAV_BLOCK, TRAV_SS, true, T_ALL>
                                                        : k_finish<TRAV_BLOCK, TRAV_SS, false, T_ALL>);
-                       f<<<g, TRAV_BLOCK, 0, s>>>(R.jobs.p, nj, q, cnt, R.records.p, MAX_DEPTH, R.tail_stack.p,
-                                                  cnt, D.counters.p);
+                       f<<<tail_threads / TRAV_BLOCK, TRAV_BLOCK, 0, s>>>(R.jobs.p, nj, in, 0u, R.view.p,
+                                                                          R.records.p, MAX_DEPTH, R.tail_stack.p,
+                                                                          tail_threads, D.counters.p);
                    });
-            paths_done = true;
-            return false;
+            D.tag_mode = nullptr;
+            cur = 1 - cur;
             }
-        if (D.fuse_bounce == 2)
+        n_steps = steps;
+        HIPCHECK(hipMemcpyAsync(h_steps, R.steps.p, 2 * steps * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        }
+
+    // Fold this batch's queued fraction at each bounce step into the device's
+    // recent maximum (valid once the batch has run).
+    void note_live(float* live) const
+        {
+        for (uint32_t b = 0; b < n_steps; b++)
             {
-            HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), s));
-            reset_segments();
-            BounceArgs a {R.jobs.p, Q[cur], n, R.counters32.p, D.trav_chunk, R.gstack.p, gstride, Q[1 - cur],
-                          R.records.p, D.counters.p};
-            D.lstream = s;
-            dispatch_bounce(D, s, a, spheres);
-            D.lstream = nullptr;
+            const float f = n_paths ? (float)((double)h_steps[2 * b] / (double)n_paths) : 0.0f;
+            live[b] = std::max(live[b], f);
             }
-        else
-            {
-            traverse(Q[cur], n, false);
-            reset_segments();
-            const uint32_t cnt = n;
-            launch("shade",
-                   [&]
-                   {
-                       auto* f = spheres ? k_shade<SHADE_BLOCK, T_SPHERES> : k_shade<SHADE_BLOCK, T_ALL>;
-                       f<<<blocks(cnt, SHADE_BLOCK), SHADE_BLOCK, 0, s>>>(R.jobs.p, nj, Q[cur], R.hits.p, cnt,
-                                                                          Q[1 - cur], R.records.p, MAX_DEPTH,
-                                                                          D.counters.p);
-                   });
-            }
-        cur = 1 - cur;
-        return true;
+        }
+
+    // Bounce steps this batch needed (its last BOUNCE step + the FINISH one);
+    // valid once the batch has run.  A batch that reached its last step with
+    // paths left needed more than it was given.
+    uint32_t steps_needed() const
+        {
+        uint32_t need = 1;
+        for (uint32_t b = 0; b < n_steps; b++)
+            if (h_steps[2 * b + 1] != VIEW_IDLE)
+                need = b + 1;
+        if (need == n_steps && h_steps[2 * (n_steps - 1)] > D.tail_rays)
+            need = n_steps + 2;
+        return need;
         }
 
     // Enqueue the accumulation after `after` (the previous batch's, if any).
@@ -1606,49 +1700,71 @@ static void split_groups(const std::vector<JobDev>& jobs, const std::vector<uint
         out.emplace_back(std::move(mixed), mixed_types);
     }
 
-// Run batches (job lists with their geometry-type sets) with up to
-// D.inflight of them in flight, one per render scratch and stream: while the
-// host waits for one batch's queue counts the other batch's kernels run, and
-// a batch's tail overlaps the next batch's primary rays.  Accumulations are
-// enqueued in batch order, each after the previous one (event), so every
-// pixel's samples are accumulated in order.  Returns with both streams idle.
+// Next estimate of the bounce steps per batch from the steps the batches of
+// a render needed: their maximum plus one spare step, and never more than
+// one below the previous estimate (the depth of paths depends on the scene).
+static void update_steps_est(DeviceObj& D, uint32_t needed_max)
+    {
+    const uint32_t est = std::max(needed_max + 1, D.steps_est > 1 ? D.steps_est - 1 : 1u);
+    D.steps_est = std::min<uint32_t>(std::max<uint32_t>(est, 2), MAX_STEPS);
+    }
+
+// Next live fractions per bounce step: the render's maxima, or 3/4 of the
+// previous estimate when that is larger (it decays over renders of shallower
+// scenes).
+static void update_live_frac(DeviceObj& D, const float* live)
+    {
+    for (uint32_t b = 0; b < MAX_STEPS; b++)
+        D.live_frac[b] = std::max(live[b], 0.75f * D.live_frac[b]);
+    }
+
+// Run batches (job lists with their geometry-type sets), alternating between
+// the device's two render scratches and streams when D.inflight is 2: each
+// batch is enqueued whole (primary rays, D.steps_est bounce steps,
+// accumulation) without waiting for the device, so one batch's kernels
+// overlap the other's.  Accumulations are chained in batch order (each waits
+// for the previous one's event), so every pixel's samples are accumulated in
+// order.  Returns with both streams idle.
 static void run_inflight(DeviceObj& D, std::deque<std::pair<std::vector<JobDev>, uint32_t>>& pending)
     {
-    struct Slot
-        {
-        std::unique_ptr<BatchExec> e;
-        int slot;
-        };
-    std::deque<Slot> active;
     HIPCHECK(hipStreamSynchronize(D.stream)); // scene, tile lists and counters ready for both streams
+    HIPCHECK(hipStreamSynchronize(D.stream2));
+    size_t bytes = 0;
+    for (const auto& b : pending)
+        bytes += stage_bytes(b.first.size());
+    D.stage.ensure(bytes);
+    std::vector<std::unique_ptr<BatchExec>> batches;
     hipEvent_t last_acc = nullptr;
-    for (;;)
+    int slot = 0;
+    size_t off = 0;
+    const uint32_t steps = D.steps_est;
+    while (!pending.empty())
         {
-        while (active.size() < D.inflight && !pending.empty())
-            {
-            const int slot = active.empty() ? 0 : 1 - active.front().slot;
-            Slot sl;
-            sl.slot = slot;
-            sl.e.reset(new BatchExec(D, slot ? D.rs2 : D.rs, slot ? D.stream2 : D.stream,
-                                     std::move(pending.front().first), pending.front().second));
-            pending.pop_front();
-            sl.e->start();
-            active.push_back(std::move(sl));
-            }
-        if (active.empty())
-            break;
-        for (auto& a : active)
-            a.e->step();
-        while (!active.empty() && active.front().e->paths_done)
-            {
-            Slot& f = active.front();
-            f.e->accumulate(last_acc, D.acc_done[f.slot]);
-            last_acc = D.acc_done[f.slot];
-            active.pop_front();
-            }
+        const size_t n_jobs = pending.front().first.size();
+        batches.emplace_back(new BatchExec(D, slot ? D.rs2 : D.rs, slot ? D.stream2 : D.stream,
+                                           std::move(pending.front().first), pending.front().second,
+                                           D.stage.p + off));
+        pending.pop_front();
+        off += stage_bytes(n_jobs);
+        BatchExec& e = *batches.back();
+        e.start();
+        e.enqueue_steps(steps);
+        e.accumulate(last_acc, D.acc_done[slot]);
+        last_acc = D.acc_done[slot];
+        if (D.inflight == 2)
+            slot = 1 - slot;
         }
     HIPCHECK(hipStreamSynchronize(D.stream));
     HIPCHECK(hipStreamSynchronize(D.stream2));
+    uint32_t need = 1;
+    float live[MAX_STEPS] = {};
+    for (const auto& e : batches)
+        {
+        need = std::max(need, e->steps_needed());
+        e->note_live(live);
+        }
+    update_steps_est(D, need);
+    update_live_frac(D, live);
     }
 
 // Render `samples` Path passes for each (scene, tracer) pair.
@@ -1749,21 +1865,26 @@ struct RequestObj
     int state = 0; // 0 queued, 1 active, 2 done, 3 rejected
     };
 
-// A batch of the server in flight: its requests and the samples it adds.
+// A batch of the server: its requests and the samples it adds.  Planned
+// batches wait (slot -1) until a render scratch is free; a started batch is
+// enqueued whole on that slot's stream and is done when its accumulation
+// event has completed.
 struct ServerBatch
     {
     std::unique_ptr<BatchExec> e;
-    int slot = -1; // render scratch and stream of the server, -1 while pending
+    int slot = -1; // render scratch and stream of the server, -1 while waiting
     std::vector<JobDev> jobs;
     uint32_t types = 0;
     std::vector<std::pair<uint64_t, uint32_t>> adds; // (request, samples)
-    bool accumulated = false;
     uint64_t serial = 0;
     int kind() const
         {
         return types == T_SPHERES ? 0 : 1;
         }
     };
+
+// Request states (fh_server_poll)
+constexpr int REQ_QUEUED = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_REJECTED = 3, REQ_FAILED = 4;
 
 struct ServerObj
     {
@@ -1778,6 +1899,7 @@ struct ServerObj
     // two batches in flight on the server's own render scratches and streams,
     // batches in plan order (front: oldest); accumulations chained by events
     RenderScratch rs[2];
+    PinnedBuf stage[2]; // job tables and step records of the batch on each slot
     hipStream_t st[2] = {nullptr, nullptr};
     hipEvent_t acc[2] = {nullptr, nullptr};
     std::deque<ServerBatch> batches;
@@ -1796,9 +1918,48 @@ struct ServerObj
         }
     };
 
+// Device memory the server cannot give to requests: its own two render
+// scratches sized for target_paths, and whatever render scratch the device
+// holds for Path.sample (allocated by earlier renders).
+static uint64_t server_reserved(const ServerObj& S)
+    {
+    const DeviceObj& D = *S.dev;
+    const uint64_t fixed = (uint64_t)(STACK_MAX - 8) * D.trav_grid * TRAV_BLOCK * sizeof(uint2)
+                           + (uint64_t)(STACK_MAX - TRAV_SS) * blocks(D.tail_rays, TRAV_BLOCK) * TRAV_BLOCK
+                                 * sizeof(uint2);
+    uint64_t device_scratch = 0;
+    for (const RenderScratch* R : {&D.rs, &D.rs2})
+        {
+        for (int q = 0; q < 2; q++)
+            device_scratch += R->qo[q].bytes() + R->qd[q].bytes() + R->qatt[q].bytes() + R->qmeta[q].bytes();
+        device_scratch += R->hits.bytes() + R->records.bytes() + R->gstack.bytes() + R->tail_stack.bytes();
+        }
+    return 2 * scratch_bytes(S.target_paths, fixed) + device_scratch;
+    }
+
+// Free a render scratch's device memory (its streams must be idle).
+static void release_scratch(RenderScratch& R)
+    {
+    for (int q = 0; q < 2; q++)
+        {
+        R.qo[q].release();
+        R.qd[q].release();
+        R.qatt[q].release();
+        R.qmeta[q].release();
+        }
+    R.hits.release();
+    R.records.release();
+    R.gstack.release();
+    R.tail_stack.release();
+    }
+
+// Largest light-sample path count of one batch (the queue indices are 31-bit).
+constexpr uint64_t MAX_BATCH_PATHS = (1ull << 31) - 1;
+
 // Plan the next server step: every active request with unplanned samples
-// gets its share of target_paths (policy_allocate); the step's spheres-only
-// and mixed jobs become one or two batches.
+// gets its share of target_paths (policy_allocate), trimmed so a batch stays
+// within MAX_BATCH_PATHS paths; the step's spheres-only and mixed jobs become
+// one or two batches.
 static void server_plan(ServerObj& S)
     {
     DeviceObj& D = *S.dev;
@@ -1820,27 +1981,35 @@ static void server_plan(ServerObj& S)
     std::vector<JobDev> sph, mixed;
     std::vector<std::pair<uint64_t, uint32_t>> add_sph, add_mixed;
     uint32_t mixed_types = 0;
+    uint64_t paths[2] = {0, 0}; // per batch kind
     for (size_t i = 0; i < ids.size(); i++)
         {
         if (k[i] == 0)
             continue;
         RequestObj& r = S.reqs[ids[i]];
+        const uint32_t t = scene_types(*r.scene);
+        const int kd = (t & ~T_SPHERES) == 0 ? 0 : 1;
+        // fewer samples (or none this step) when the batch would exceed its path limit
+        const uint64_t pps = items[i].paths_per_sample;
+        const uint32_t fit = (uint32_t)std::min<uint64_t>(k[i], (MAX_BATCH_PATHS - paths[kd]) / pps);
+        if (fit == 0)
+            continue;
+        paths[kd] += fit * pps;
         update_tiles(D, *r.scene, *r.tracer, make_cam(r.scene->camera));
         JobDev J = make_job(*r.scene, *r.tracer, r.scene->camera);
-        J.K = k[i];
+        J.K = fit;
         J.n_base = r.tracer->n_samples; // samples planned before this batch
-        r.tracer->n_samples += k[i];
-        r.scheduled += k[i];
-        const uint32_t t = scene_types(*r.scene);
-        if ((t & ~T_SPHERES) == 0)
+        r.tracer->n_samples += fit;
+        r.scheduled += fit;
+        if (kd == 0)
             {
             sph.push_back(J);
-            add_sph.push_back({ids[i], k[i]});
+            add_sph.push_back({ids[i], fit});
             }
         else
             {
             mixed.push_back(J);
-            add_mixed.push_back({ids[i], k[i]});
+            add_mixed.push_back({ids[i], fit});
             mixed_types |= t;
             }
         }
@@ -1874,7 +2043,7 @@ static void server_retire(ServerObj& S, double now)
         RequestObj& r = S.reqs[id];
         if (r.done >= r.samples)
             {
-            r.state = 2;
+            r.state = REQ_DONE;
             r.t_done = S.clock(now);
             S.in_use -= r.footprint;
             r.tracer->L = 1; // as Path.sample leaves it (fresnel/tracer.py:259-261)
@@ -1885,29 +2054,99 @@ static void server_retire(ServerObj& S, double now)
     S.active.swap(still);
     }
 
-// One server iteration: admit; plan a step when no planned batch is waiting;
-// start planned batches on free slots; advance every batch in flight by one
-// host round trip; accumulate finished batches in plan order and retire the
-// requests they complete.  Work stays in flight between calls.
+// A request that cannot complete (a batch holding some of its samples failed
+// to start): it leaves the active set and frees its footprint.
+static void server_fail(ServerObj& S, uint64_t id, double now)
+    {
+    RequestObj& r = S.reqs[id];
+    if (r.state != REQ_ACTIVE)
+        return;
+    r.state = REQ_FAILED;
+    r.t_done = S.clock(now);
+    S.in_use -= r.footprint;
+    r.tracer->L = 1;
+    S.active.erase(std::remove(S.active.begin(), S.active.end(), id), S.active.end());
+    }
+
+// Finish a started batch whose accumulation has completed: check the device's
+// overflow flags, count its samples as done, free its slot.
+static void server_finish_batch(ServerObj& S, size_t i)
+    {
+    DeviceObj& D = *S.dev;
+    ServerBatch& b = S.batches[i];
+    uint64_t flags[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(flags, D.counters.p + CTR_QUEUE_OVERFLOW, sizeof(flags), hipMemcpyDeviceToHost,
+                            S.st[b.slot]));
+    HIPCHECK(hipStreamSynchronize(S.st[b.slot]));
+    update_steps_est(D, b.e->steps_needed());
+    float live[MAX_STEPS] = {};
+    b.e->note_live(live);
+    update_live_frac(D, live);
+    if (flags[0])
+        throw Error(FH_ERR_STATE, "internal error: a ray queue segment overflowed");
+    if (flags[1])
+        throw Error(FH_ERR_STATE, "traversal stack overflow");
+    for (auto& a : b.adds)
+        S.reqs[a.first].done += a.second;
+    const int kd = b.kind();
+    if (S.last_acc_batch[kd] == b.serial)
+        S.last_acc[kd] = nullptr; // finished: later batches of its kind need not wait
+    S.batches.erase(S.batches.begin() + (std::ptrdiff_t)i);
+    }
+
+// One server iteration: admit; retire started batches whose accumulation has
+// completed; plan a step when no planned batch is waiting; start waiting
+// batches on free slots (each enqueued whole); when both slots are busy and
+// nothing could start, wait for the oldest started batch.  Work stays in
+// flight between calls.
 static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& n_queued)
     {
     DeviceObj& D = *S.dev;
+    std::lock_guard<std::mutex> lock(D.mu);
+    const bool stats = D.stats_on; // per-launch timing is for Path.sample renders only
+    D.stats_on = false;
+    struct Restore
+        {
+        DeviceObj& D;
+        bool v;
+        ~Restore()
+            {
+            D.stats_on = v;
+            }
+        } restore {D, stats};
     std::vector<PolicyItem> admitted, rejected;
-    policy_admit(S.queue, S.budget, S.in_use, admitted, rejected);
+    uint64_t reserved = server_reserved(S);
+    if (!S.queue.empty() && S.in_use + S.queue.front().footprint + reserved > S.budget)
+        {
+        // the device's render scratch for Path.sample is idle while the server
+        // holds the device lock: release it (the next render allocates again)
+        release_scratch(D.rs);
+        release_scratch(D.rs2);
+        reserved = server_reserved(S);
+        }
+    const uint64_t room = S.budget > reserved ? S.budget - reserved : 0;
+    policy_admit(S.queue, room, S.in_use, admitted, rejected);
     for (auto& it : rejected)
         {
         RequestObj& r = S.reqs[it.id];
-        r.state = 3;
+        r.state = REQ_REJECTED;
         r.t_done = S.clock(now);
         }
-    std::lock_guard<std::mutex> lock(D.mu);
     for (auto& it : admitted)
         {
         RequestObj& r = S.reqs[it.id];
-        r.state = 1;
+        r.state = REQ_ACTIVE;
         r.t_admit = S.clock(now);
-        sync_scene(*r.scene); // BVH build and uploads happen at admission
         S.active.push_back(it.id);
+        try
+            {
+            sync_scene(*r.scene); // BVH build and uploads happen at admission
+            }
+        catch (...)
+            {
+            server_fail(S, it.id, now);
+            throw;
+            }
         }
     if (!S.st[0])
         for (int k = 0; k < 2; k++)
@@ -1918,6 +2157,15 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
     if (S.draining)
         while (!S.batches.empty() && S.batches.back().slot < 0)
             S.batches.pop_back();
+    // retire started batches that have completed
+    for (size_t i = 0; i < S.batches.size();)
+        {
+        ServerBatch& b = S.batches[i];
+        if (b.slot >= 0 && hipEventQuery(S.acc[b.slot]) == hipSuccess)
+            server_finish_batch(S, i);
+        else
+            i++;
+        }
     bool waiting = false;
     for (auto& b : S.batches)
         waiting = waiting || b.slot < 0;
@@ -1929,9 +2177,10 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
     for (auto& b : S.batches)
         if (b.slot >= 0)
             used[b.slot] = true;
-    bool synced = false;
-    for (auto& b : S.batches)
+    bool synced = false, started = false;
+    for (size_t i = 0; i < S.batches.size(); i++)
         {
+        ServerBatch& b = S.batches[i];
         if (b.slot >= 0)
             continue;
         const int slot = !used[0] ? 0 : (!used[1] ? 1 : -1);
@@ -1942,63 +2191,61 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
             HIPCHECK(hipStreamSynchronize(D.stream));
             synced = true;
             }
+        try
+            {
+            // the slot's previous batch has completed, so its staging is free
+            S.stage[slot].ensure(stage_bytes(b.jobs.size()));
+            b.e.reset(new BatchExec(D, S.rs[slot], S.st[slot], b.jobs, b.types, S.stage[slot].p));
+            b.e->start();
+            b.e->enqueue_steps(D.steps_est);
+            const int kd = b.kind();
+            b.e->accumulate(S.last_acc[kd], S.acc[slot]);
+            S.last_acc[kd] = S.acc[slot];
+            S.last_acc_batch[kd] = b.serial;
+            }
+        catch (...)
+            {
+            // whatever was enqueued drains; the batch's requests cannot complete
+            (void)hipStreamSynchronize(S.st[slot]);
+            for (auto& a : b.adds)
+                server_fail(S, a.first, now);
+            S.batches.erase(S.batches.begin() + (std::ptrdiff_t)i);
+            throw;
+            }
         b.slot = slot;
         used[slot] = true;
-        b.e.reset(new BatchExec(D, S.rs[slot], S.st[slot], b.jobs, b.types));
-        b.e->start();
+        started = true;
         }
-    for (auto& b : S.batches)
-        if (b.slot >= 0 && !b.accumulated)
-            b.e->step();
-    // accumulate each finished batch after the earlier batches of its kind
-    bool pending_kind[2] = {false, false};
-    for (auto& b : S.batches)
-        {
-        const int kd = b.kind();
-        if (b.slot >= 0 && !b.accumulated && b.e->paths_done && !pending_kind[kd])
-            {
-            b.e->accumulate(S.last_acc[kd], S.acc[b.slot]);
-            S.last_acc[kd] = S.acc[b.slot];
-            S.last_acc_batch[kd] = b.serial;
-            b.accumulated = true;
-            }
-        if (!b.accumulated)
-            pending_kind[kd] = true;
-        }
-    // retire batches whose accumulation has finished; wait for one only when
-    // no batch is left to advance
-    bool any_running = false;
-    for (auto& b : S.batches)
-        any_running = any_running || (b.slot >= 0 && !b.accumulated);
-    for (size_t i = 0; i < S.batches.size();)
-        {
-        ServerBatch& b = S.batches[i];
-        if (!b.accumulated)
-            {
-            i++;
-            continue;
-            }
-        if (hipEventQuery(S.acc[b.slot]) == hipErrorNotReady)
-            {
-            if (any_running)
+    // nothing could start and both slots are busy: wait for the oldest
+    // started batch rather than let the caller spin
+    if (!started && used[0] && used[1])
+        for (size_t i = 0; i < S.batches.size(); i++)
+            if (S.batches[i].slot >= 0)
                 {
-                i++;
-                continue;
+                HIPCHECK(hipEventSynchronize(S.acc[S.batches[i].slot]));
+                server_finish_batch(S, i);
+                break;
                 }
-            HIPCHECK(hipEventSynchronize(S.acc[b.slot]));
+    // with nothing waiting to start, the last started batches are waited for
+    // too (the caller has nothing else to overlap)
+    if (S.draining && !S.batches.empty() && S.batches.front().slot >= 0)
+        {
+        HIPCHECK(hipEventSynchronize(S.acc[S.batches.front().slot]));
+        server_finish_batch(S, 0);
+        }
+    else if (S.queue.empty())
+        {
+        bool any_waiting = false;
+        for (auto& b : S.batches)
+            any_waiting = any_waiting || b.slot < 0;
+        bool plannable = false;
+        for (uint64_t id : S.active)
+            plannable = plannable || S.reqs[id].scheduled < S.reqs[id].samples;
+        if (!any_waiting && !plannable && !S.batches.empty())
+            {
+            HIPCHECK(hipEventSynchronize(S.acc[S.batches.front().slot]));
+            server_finish_batch(S, 0);
             }
-        uint64_t overflow = 0;
-        HIPCHECK(hipMemcpyAsync(&overflow, D.counters.p + CTR_OVERFLOW, sizeof(overflow), hipMemcpyDeviceToHost,
-                                S.st[b.slot]));
-        HIPCHECK(hipStreamSynchronize(S.st[b.slot]));
-        if (overflow)
-            throw Error(FH_ERR_STATE, "traversal stack overflow");
-        for (auto& a : b.adds)
-            S.reqs[a.first].done += a.second;
-        const int kd = b.kind();
-        if (S.last_acc_batch[kd] == b.serial)
-            S.last_acc[kd] = nullptr; // finished: later batches of its kind need not wait
-        S.batches.erase(S.batches.begin() + (std::ptrdiff_t)i);
         }
     server_retire(S, now);
     n_active = (uint32_t)S.active.size();
@@ -3069,8 +3316,11 @@ extern "C"
                 REQUIRE(scene->obj.dev == S.dev && tr->obj.dev == S.dev, "Scene and Tracer devices do not match");
                 REQUIRE(tr->obj.type == T_PATH, "requests need a Path tracer");
                 REQUIRE(light_samples >= 1 && light_samples < 65536, "light_samples must be in [1, 65535]");
+                REQUIRE((uint64_t)tr->obj.W * tr->obj.H * light_samples <= MAX_BATCH_PATHS,
+                        "request too large: one camera sample exceeds 2^31 - 1 light-sample paths");
                 for (auto& kv : S.reqs)
-                    REQUIRE(kv.second.tracer != &tr->obj || kv.second.state >= 2, "tracer already has an open request");
+                    REQUIRE(kv.second.tracer != &tr->obj || kv.second.state >= REQ_DONE,
+                            "tracer already has an open request");
                 RequestObj r;
                 r.id = S.next_id++;
                 r.scene = &scene->obj;
@@ -3089,7 +3339,7 @@ extern "C"
                 S.reqs[r.id] = r;
                 if (samples == 0)
                     {
-                    S.reqs[r.id].state = 2;
+                    S.reqs[r.id].state = REQ_DONE;
                     S.reqs[r.id].t_done = r.t_arrival;
                     }
                 else
@@ -3140,7 +3390,7 @@ extern "C"
                 NONNULL(srv);
                 auto it = srv->obj.reqs.find(id);
                 REQUIRE(it != srv->obj.reqs.end(), "unknown request id");
-                REQUIRE(it->second.state >= 2, "request is still queued or active");
+                REQUIRE(it->second.state >= REQ_DONE, "request is still queued or active");
                 srv->obj.reqs.erase(it);
             });
         }
@@ -3158,6 +3408,15 @@ extern "C"
         }
 
     // ---------------------------------------------------------------- policy (no device needed)
+    int fh_policy_scratch_bytes(uint64_t paths, uint64_t fixed, uint64_t* out)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(out);
+                *out = scratch_bytes(paths, fixed);
+            });
+        }
     int fh_policy_footprint(uint64_t n_prims, uint64_t prim_bytes, uint64_t W, uint64_t H, uint64_t L, uint64_t* out)
         {
         return guard(
@@ -3495,8 +3754,8 @@ extern "C"
                 D.begin_stats();
                 HIPCHECK(hipMemcpyAsync(R.jobs.p, &J, sizeof(J), hipMemcpyHostToDevice, D.stream));
                 HIPCHECK(hipMemsetAsync(R.counters32.p, 0, sizeof(uint32_t), D.stream));
-                TravArgs a {R.jobs.p, 1, Queue {}, npix, R.hits.p, R.counters32.p, D.trav_chunk, R.gstack.p, gstride,
-                            D.counters.p};
+                TravArgs a {R.jobs.p, 1, Queue {}, npix, nullptr, R.hits.p, R.counters32.p, D.trav_chunk, R.gstack.p,
+                            gstride, D.counters.p};
                 dispatch_traverse(D, "traverse_primary", D.stream, a, true,
                                   (scene_types(sc) & ~T_SPHERES) == 0);
                 D.end_stats();

@@ -43,6 +43,16 @@ inline uint64_t request_footprint(uint64_t n_prims, uint64_t prim_bytes, uint64_
     return scene + outputs + per_sample;
     }
 
+// Bytes of one render scratch for batches of `paths` light-sample paths: two
+// ray queues (64 B per path each), closest hits and termination records
+// (16 B each), plus `fixed` (overflow stacks, counters) that does not depend
+// on the batch size.  The server keeps two such scratches beside the
+// requests' footprints, so admission uses the budget minus them.
+inline uint64_t scratch_bytes(uint64_t paths, uint64_t fixed)
+    {
+    return paths * (2 * 64 + 16 + 16) + fixed;
+    }
+
 // Admit from the queue front while the budget allows.  Returns the admitted
 // items (removed from `queue`) and the rejected ones (larger than the budget).
 inline void policy_admit(std::deque<PolicyItem>& queue, uint64_t budget, uint64_t& in_use,

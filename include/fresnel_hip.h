@@ -245,12 +245,15 @@ int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t s
  * counterpart).  A server keeps admitted requests' scenes (BVH, primitives)
  * and accumulation buffers resident and advances all active requests together:
  * each fh_server_step admits queued requests first-come first-served while
- * their HBM footprints fit the budget, plans a step (every active request
- * takes some camera samples) when none is waiting, keeps up to two planned
- * batches in flight on the server's own two render scratches and streams,
- * advances each by one host round trip, and retires requests whose samples
- * have all been accumulated.  Work stays in flight between calls;
- * fh_server_destroy finishes the batches already started.
+ * their HBM footprints fit the budget (less the render scratches, see
+ * fh_policy_scratch_bytes), retires started batches that have completed,
+ * plans a step (every active request takes some camera samples) when none is
+ * waiting, starts planned batches on the server's two render scratches and
+ * streams (a started batch is enqueued whole; the device finds each bounce's
+ * rays itself), and retires requests whose samples have all been
+ * accumulated.  When nothing can start it waits for the oldest started batch.
+ * Work stays in flight between calls; fh_server_destroy finishes the batches
+ * already started.
  * A request's image is bit-identical to Path.sample(scene, samples,
  * light_samples) on its own: submission resets the tracer like sample()'s
  * reset, and scheduling never changes per-path arithmetic.
@@ -259,7 +262,7 @@ int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t s
  * Path.sample leaves it.
  * Times are seconds; pass now < 0 to use the server's monotonic clock.
  * target_paths_per_step = 0 selects 128 Mi; the server's two render scratches
- * take about 2 x 160 B x target_paths_per_step of HBM beyond the budget. */
+ * (about 2 x 160 B x target_paths_per_step) are charged against the budget. */
 typedef struct fh_server_s* fh_server;
 int fh_server_create(fh_device dev, uint64_t hbm_budget_bytes, uint64_t target_paths_per_step,
                      fh_server* out);
@@ -267,8 +270,10 @@ int fh_server_destroy(fh_server srv);
 int fh_server_submit(fh_server srv, fh_scene scene, fh_tracer tracer, uint32_t samples,
                      uint32_t light_samples, double now, uint64_t* id);
 int fh_server_step(fh_server srv, double now, uint32_t* n_active, uint32_t* n_queued);
-/* state: 0 queued, 1 active, 2 done, 3 rejected (footprint above the budget);
- * times[0..2] = arrival, admission, completion (-1 if not yet). */
+/* state: 0 queued, 1 active, 2 done, 3 rejected (footprint above the budget),
+ * 4 failed (a batch holding some of its samples could not start; that
+ * fh_server_step returned the error); times[0..2] = arrival, admission,
+ * completion (-1 if not yet). */
 int fh_server_poll(fh_server srv, uint64_t id, int* state, uint32_t* samples_done, double times[3]);
 /* Forget a done or rejected request (its id becomes unknown). */
 int fh_server_release(fh_server srv, uint64_t id);
@@ -277,6 +282,12 @@ int fh_server_memory(fh_server srv, uint64_t* in_use, uint64_t* budget);
  * admission and completion iterations for synthetic requests. */
 int fh_policy_footprint(uint64_t n_prims, uint64_t prim_bytes, uint64_t W, uint64_t H, uint64_t L,
                         uint64_t* out);
+/* Bytes of one render scratch for batches of `paths` light-sample paths plus
+ * its size-independent part `fixed`.  A server's admission budget is its HBM
+ * budget minus two such scratches (sized for target_paths_per_step) and any
+ * render scratch the device holds for Path.sample (released when it would
+ * keep the queue's head out). */
+int fh_policy_scratch_bytes(uint64_t paths, uint64_t fixed, uint64_t* out);
 int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* paths_per_sample,
                        const uint32_t* samples, const uint32_t* arrive, uint64_t budget,
                        uint64_t target_paths, uint32_t max_iterations, int32_t* admit_iter,
