@@ -515,6 +515,13 @@ PYBIND11_MODULE(_gpu, m)
                                        (float*)c.request().ptr));
                  return py::make_tuple(s, c);
              })
+        .def("graphStats",
+             [](Device& d)
+             {
+                 uint64_t c = 0, r = 0;
+                 check(fh_stats_graphs(d.h, &c, &r));
+                 return py::make_tuple(c, r);
+             })
         .def("debugPhilox",
              [](Device& d, py::array_t<uint32_t, py::array::c_style | py::array::forcecast> in)
              {

@@ -210,6 +210,15 @@ struct DeviceObj
     uint32_t trav_grid = 0;
     RenderScratch rs, rs2; // render scratch of the two batches in flight
     PinnedBuf stage;       // job tables, pixel bases and step records of one render's batches
+    uint32_t graph = 1;     // 2: a batch's kernels are captured once into a HIP graph and replayed; 1: launched
+    struct GraphEntry
+        {
+        std::vector<uint64_t> key; // every parameter of the captured launches
+        hipGraphExec_t exec;
+        uint64_t used;
+        };
+    std::vector<GraphEntry> graph_cache;
+    uint64_t graph_clock = 0, graph_captures = 0, graph_replays = 0;
     uint32_t steps_est = 8; // bounce steps per batch (adapted to the depths of recent renders)
     float live_frac[MAX_STEPS] = {}; // fraction of a batch's paths queued at each bounce step, recent maximum
     const uint32_t* tag_mode = nullptr; // mode word for the launch being recorded (stats)
@@ -1445,7 +1454,7 @@ struct BatchExec
     bool spheres;
     std::vector<uint32_t> pix_base;
     uint64_t n_pri = 0, n_paths = 0, n_pix = 0, n_rec = 0;
-    uint32_t nj = 0, gstride = 0, seg_cap = 0, tail_threads = 0;
+    uint32_t nj = 0, gstride = 0, seg_cap = 0, tail_threads = 0, rest_grid = 0;
     uint32_t n_steps = 0;
     int cur = 1;
     Queue Q[2];
@@ -1541,6 +1550,53 @@ struct BatchExec
             Q[q].seg_cap = seg_cap;
             }
         nj = (uint32_t)jobs.size();
+        // resident grid of the grid-stride shading form (queried here, not
+        // while a stream captures)
+        rest_grid = D.grid_for(reinterpret_cast<const void*>(spheres ? k_shade<SHADE_BLOCK, T_SPHERES, true>
+                                                                     : k_shade<SHADE_BLOCK, T_ALL, true>),
+                               SHADE_BLOCK);
+        }
+
+    // Workgroups of the one-unit shading launch at bounce step b: the live
+    // fraction recent renders had at this step with a quarter margin, rounded
+    // up to an eighth of its power of two (so small drifts of the estimate
+    // keep a captured graph's shape).
+    uint32_t shade_grid(uint32_t b) const
+        {
+        const double frac = std::min(1.0, 1.25 * (double)D.live_frac[b]);
+        uint64_t g = blocks((uint64_t)(frac * (double)n_paths) + 4096, SHADE_BLOCK);
+        uint64_t q = 1;
+        while ((q << 4) <= g)
+            q <<= 1;
+        g = (g + q - 1) / q * q;
+        return (uint32_t)std::min<uint64_t>(g, blocks(n_paths, SHADE_BLOCK) + 1);
+        }
+
+    // Everything the kernels of enqueue_kernels(steps) depend on: equal keys
+    // enqueue identical launches.
+    std::vector<uint64_t> graph_key(uint32_t steps) const
+        {
+        std::vector<uint64_t> k = {(uint64_t)(uintptr_t)s, nj, n_pri, n_paths, n_pix, n_rec, seg_cap, gstride,
+                                   tail_threads, types, steps, D.trav_chunk, D.primary_chunk, D.xcd_bands,
+                                   D.fuse_primary, D.fuse_bounce, D.tail_rays, (uint64_t)D.stack_lds, D.trav_refill,
+                                   D.trav_packet, D.count_on ? 1u : 0u, rest_grid};
+        for (const void* p : {(const void*)R.qo[0].p, (const void*)R.qo[1].p, (const void*)R.qd[0].p,
+                              (const void*)R.qd[1].p, (const void*)R.qatt[0].p, (const void*)R.qatt[1].p,
+                              (const void*)R.qmeta[0].p, (const void*)R.qmeta[1].p, (const void*)R.hits.p,
+                              (const void*)R.records.p, (const void*)R.counters32.p, (const void*)R.qseg.p,
+                              (const void*)R.jobs.p, (const void*)R.pix_base.p, (const void*)R.view.p,
+                              (const void*)R.steps.p, (const void*)R.gstack.p, (const void*)R.tail_stack.p,
+                              (const void*)D.counters.p})
+            k.push_back((uint64_t)(uintptr_t)p);
+        for (uint32_t b = 0; b < steps; b++)
+            k.push_back(shade_grid(b));
+        return k;
+        }
+
+    // The batch's kernels: counter clears, primary rays, `steps` bounce steps.
+    void enqueue_kernels(uint32_t steps)
+        {
+        const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
         clear(R.counters32.p, 8 * QSEG_STRIDE);
         clear(R.qseg.p, QSEG_FLAG + 1);
         if (D.fuse_primary == 2)
@@ -1563,6 +1619,7 @@ struct BatchExec
                    });
             }
         cur = 1;
+        enqueue_steps(steps);
         }
 
     // Enqueue `steps` bounce steps (1 <= steps <= MAX_STEPS); the last one
@@ -1571,7 +1628,6 @@ struct BatchExec
         {
         auto* shade_one = spheres ? k_shade<SHADE_BLOCK, T_SPHERES, false> : k_shade<SHADE_BLOCK, T_ALL, false>;
         auto* shade_rest = spheres ? k_shade<SHADE_BLOCK, T_SPHERES, true> : k_shade<SHADE_BLOCK, T_ALL, true>;
-        const uint32_t rest_grid = D.grid_for(reinterpret_cast<const void*>(shade_rest), SHADE_BLOCK);
         for (uint32_t b = 0; b < steps; b++)
             {
             const bool last = b + 1 == steps;
@@ -1599,8 +1655,7 @@ struct BatchExec
                 // one workgroup per 512 rays up to the expected count (the live
                 // fraction recent renders had at this step, with margin), the
                 // rest (if more are queued) by the grid-stride form
-                const double frac = std::min(1.0, 1.25 * (double)D.live_frac[b]);
-                const uint32_t one_grid = blocks((uint64_t)(frac * (double)n_paths) + 4096, SHADE_BLOCK);
+                const uint32_t one_grid = shade_grid(b);
                 launch("shade",
                        [&]
                        {
@@ -1633,7 +1688,64 @@ struct BatchExec
             cur = 1 - cur;
             }
         n_steps = steps;
-        HIPCHECK(hipMemcpyAsync(h_steps, R.steps.p, 2 * steps * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        }
+
+    // Copy the step records to the pinned host copy (after the steps).
+    void copy_steps()
+        {
+        HIPCHECK(hipMemcpyAsync(h_steps, R.steps.p, 2 * n_steps * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        }
+
+    // The batch's kernels, launched or (D.graph == 2 and no per-launch timing
+    // or counting) captured once per key into a HIP graph and replayed; then
+    // the step records' copy.  The job table copies (start) and the
+    // accumulation, which waits on another stream's event, stay outside the
+    // graph, as no memset is inside it.
+    void run(uint32_t steps)
+        {
+        if (D.graph != 2 || D.stats_on || D.count_on)
+            {
+            enqueue_kernels(steps);
+            copy_steps();
+            return;
+            }
+        const std::vector<uint64_t> key = graph_key(steps);
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : D.graph_cache)
+            if (g.key == key)
+                {
+                exec = g.exec;
+                g.used = ++D.graph_clock;
+                D.graph_replays++;
+                break;
+                }
+        if (!exec)
+            {
+            HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try
+                {
+                enqueue_kernels(steps);
+                }
+            catch (...)
+                {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g)
+                    (void)hipGraphDestroy(g);
+                throw;
+                }
+            hipGraph_t g = nullptr;
+            HIPCHECK(hipStreamEndCapture(s, &g));
+            const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHECK(e);
+            D.graph_cache.push_back(DeviceObj::GraphEntry {key, exec, ++D.graph_clock});
+            D.graph_captures++;
+            }
+        else
+            n_steps = steps; // as enqueue_kernels sets it
+        HIPCHECK(hipGraphLaunch(exec, s));
+        copy_steps();
         }
 
     // Fold this batch's queued fraction at each bounce step into the device's
@@ -1729,6 +1841,15 @@ static void run_inflight(DeviceObj& D, std::deque<std::pair<std::vector<JobDev>,
     {
     HIPCHECK(hipStreamSynchronize(D.stream)); // scene, tile lists and counters ready for both streams
     HIPCHECK(hipStreamSynchronize(D.stream2));
+    // both streams are idle: the least recently used graphs beyond 32 can go
+    while (D.graph_cache.size() > 32)
+        {
+        auto lru = std::min_element(D.graph_cache.begin(), D.graph_cache.end(),
+                                    [](const DeviceObj::GraphEntry& a, const DeviceObj::GraphEntry& b)
+                                    { return a.used < b.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        D.graph_cache.erase(lru);
+        }
     size_t bytes = 0;
     for (const auto& b : pending)
         bytes += stage_bytes(b.first.size());
@@ -1748,7 +1869,7 @@ static void run_inflight(DeviceObj& D, std::deque<std::pair<std::vector<JobDev>,
         off += stage_bytes(n_jobs);
         BatchExec& e = *batches.back();
         e.start();
-        e.enqueue_steps(steps);
+        e.run(steps);
         e.accumulate(last_acc, D.acc_done[slot]);
         last_acc = D.acc_done[slot];
         if (D.inflight == 2)
@@ -2197,7 +2318,8 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
             S.stage[slot].ensure(stage_bytes(b.jobs.size()));
             b.e.reset(new BatchExec(D, S.rs[slot], S.st[slot], b.jobs, b.types, S.stage[slot].p));
             b.e->start();
-            b.e->enqueue_steps(D.steps_est);
+            b.e->enqueue_kernels(D.steps_est);
+            b.e->copy_steps();
             const int kd = b.kind();
             b.e->accumulate(S.last_acc[kd], S.acc[slot]);
             S.last_acc[kd] = S.acc[slot];
@@ -2399,6 +2521,10 @@ extern "C"
                     }
                 for (auto e : dev->obj.event_pool)
                     (void)hipEventDestroy(e);
+                (void)hipStreamSynchronize(dev->obj.stream2);
+                for (auto& g : dev->obj.graph_cache)
+                    (void)hipGraphExecDestroy(g.exec);
+                dev->obj.graph_cache.clear();
                 if (dev->obj.h_pinned)
                     (void)hipHostFree(dev->obj.h_pinned);
                 if (dev->obj.h_counts)
@@ -3534,6 +3660,15 @@ extern "C"
                         D.xcd_bands = (uint32_t)value;
                         }
                     }
+                else if (k == "graph")
+                    {
+                    prev = D.graph;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 2, "graph must be 1 (launch) or 2 (capture and replay)");
+                        D.graph = (uint32_t)value;
+                        }
+                    }
                 else if (k == "inflight")
                     {
                     prev = D.inflight;
@@ -3884,6 +4019,18 @@ extern "C"
                 copy_str(all, names, names_cap);
                 if (n_kernels)
                     *n_kernels = k;
+            });
+        }
+    int fh_stats_graphs(fh_device dev, uint64_t* captures, uint64_t* replays)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(dev);
+                if (captures)
+                    *captures = dev->obj.graph_captures;
+                if (replays)
+                    *replays = dev->obj.graph_replays;
             });
         }
     int fh_counters_enable(fh_device dev, int enable)

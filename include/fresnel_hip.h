@@ -316,7 +316,11 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * for the other's queue counts) or 1; "xcd_bands": the primary kernel fills
  * the 16 queue segments with 16 image bands and the secondary traversal's
  * workgroups claim the rays of their own XCD's two bands first (2, default)
- * or the segments hold interleaved chunks and claims are global (1).
+ * or the segments hold interleaved chunks and claims are global (1);
+ * "graph": a render's batch kernels (primary rays and the bounce steps) are
+ * captured into a HIP graph on first use and replayed for batches of the
+ * same shape (2), or launched one by one (1, default); renders with
+ * per-kernel timing or work counting always launch.
  * value <= 0 only reads the current value into *previous. */
 int fh_device_tune(fh_device dev, const char* key, int64_t value, int64_t* previous);
 
@@ -355,6 +359,9 @@ int fh_debug_bvh(fh_scene scene, void* nodes, uint64_t cap, uint64_t* n_nodes);
  * names: '\n'-separated kernel names; ms[k]: summed time; launches[k]: count. */
 int fh_stats_last(fh_device dev, char* names, size_t names_cap, double* ms,
                   uint32_t* launches, uint32_t cap, uint32_t* n_kernels);
+/* Batches whose kernels were captured into a HIP graph, and captured graphs
+ * replayed, since the device was created (tuning key "graph" = 2). */
+int fh_stats_graphs(fh_device dev, uint64_t* captures, uint64_t* replays);
 /* Enable per-kernel event timing (adds one event pair per launch). */
 int fh_stats_enable(fh_device dev, int enable);
 /* Deterministic work counters of the most recent render: rays traced, BVH

@@ -236,6 +236,31 @@ def test_batches_in_flight_equal_one_batch(gpu_device):
         assert numpy.array_equal(one_alone[k], one_batch[k]), f"scene {k}: batched differs from alone"
 
 
+def test_graph_replays_equal_eager(gpu_device):
+    """Batches captured into HIP graphs and replayed (tuning key 'graph' = 2) give the eager images bit for
+    bit on every replay, with eager renders and synchronizations in between; several batches per render on
+    both streams, spheres-only and mixed scenes."""
+    d = gpu_device._device
+    saved = {k: d.tune(k, 0) for k in ('graph', 'batch_paths')}
+    try:
+        d.tune('batch_paths', 20000)  # several batches per render
+        for b in (lambda dv: c1_scene(dv, n=3000, w_seed=7), _spheres_and_cylinders):
+            d.tune('graph', 1)
+            ref = _linear(gpu_device, b, 64, 48, 6, 2)[0]
+            c0, r0 = d.graphStats()
+            for k in range(4):
+                d.tune('graph', 2)
+                img = _linear(gpu_device, b, 64, 48, 6, 2)[0]
+                assert numpy.array_equal(img, ref), f"replay {k} differs"
+                d.tune('graph', 1)
+                assert numpy.array_equal(_linear(gpu_device, b, 64, 48, 6, 2)[0], ref)
+            c1, r1 = d.graphStats()
+            assert c1 > c0 and r1 > r0, (c0, r0, c1, r1)
+    finally:
+        for k, v in saved.items():
+            d.tune(k, v)
+
+
 def test_random_rays_vs_checker(gpu_device, cpu_device):
     """Closest hits of random rays (many starting inside spheres) are bit-exact."""
     rng = numpy.random.default_rng(3)
@@ -275,7 +300,7 @@ def test_tuning_knobs_do_not_change_image(builder, gpu_device):
     """
     d = gpu_device._device
     keys = ('stack_lds', 'batch_paths', 'tail_rays', 'trav_chunk', 'trav_refill', 'trav_packet', 'primary_tiles',
-            'primary_chunk', 'fuse_primary', 'fuse_bounce', 'inflight', 'xcd_bands')
+            'primary_chunk', 'fuse_primary', 'fuse_bounce', 'inflight', 'xcd_bands', 'graph')
     saved = {k: d.tune(k, 0) for k in keys}
     settings = [dict(stack_lds=16, batch_paths=1 << 25, tail_rays=32768, trav_chunk=256, trav_refill=65,
                      trav_packet=1, primary_tiles=1, fuse_primary=1, fuse_bounce=1, xcd_bands=1),
@@ -288,7 +313,11 @@ def test_tuning_knobs_do_not_change_image(builder, gpu_device):
                 dict(stack_lds=16, batch_paths=10000, tail_rays=100, trav_chunk=192, trav_refill=64,
                      trav_packet=2, primary_tiles=2, fuse_bounce=1, inflight=1),
                 dict(stack_lds=16, batch_paths=1 << 26, tail_rays=100, trav_chunk=64, trav_refill=17,
-                     trav_packet=1, primary_tiles=2)]
+                     trav_packet=1, primary_tiles=2),
+                dict(stack_lds=8, batch_paths=3000, tail_rays=200, trav_chunk=256, trav_refill=65, trav_packet=2,
+                     primary_tiles=2, fuse_primary=2, fuse_bounce=1, inflight=2, graph=2),
+                dict(stack_lds=8, batch_paths=1 << 20, tail_rays=1, trav_chunk=128, trav_refill=65, trav_packet=2,
+                     primary_tiles=1, fuse_primary=1, fuse_bounce=2, inflight=1, graph=2)]
     images = []
     try:
         for st in settings:
