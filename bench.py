@@ -9,19 +9,32 @@ Default workload (the large particle scene, one GPU):
   1 light sample.
 
 A step renders one frame: Path.sample(scene, samples=256) on an HBM-resident
-scene (the BVH is built during warmup).  Mpaths/s = W*H*samples*light_samples
-per second.  The JSON line also carries:
-  roofline      the kernel with the most time per frame (the fused
-                primary-ray kernel or the secondary-ray traversal):
-                algorithmic bytes per launch (64 B per BVH node visited +
-                prim_bytes per primitive tested + the ray's own bytes, from
-                the instrumented traversal counters) over the average launch
-                duration (HIP events on the stream it runs on), against 8 TB/s
-                HBM; with its VALU busy and active-lane fractions from the
-                committed rocprofv3 counters (profiles/traffic_<workload>.json).
-  cpu_baseline  the CPU checker (a scalar restatement of the reference's CPU
-                path, oracle/) on the same scene, full-resolution passes on
-                host threads for about 10 s.
+scene (the BVH is built during warmup); the image stays on the device (the
+reference's ImageArray maps on construction; here a read maps, once, outside
+the timed region).  Mpaths/s = W*H*samples*light_samples per second.  The
+JSON line also carries:
+  roofline         the kernel with the most time per frame (the
+                   secondary-ray traversal or the fused primary kernel):
+                   algorithmic bytes per launch (128 B per 4-wide node visit
+                   of a spheres-only hierarchy, 64 B per binary node visit,
+                   prim_bytes per primitive tested, the ray's own bytes, from
+                   the instrumented work counters) over the average launch
+                   duration (HIP events on the stream it runs on), against
+                   8 TB/s HBM; beside it the same launches' model FP32 flops
+                   against 157.3 TFLOP/s, the HBM bytes and VALU busy /
+                   active-lane fractions the committed rocprofv3 counters
+                   measured (profiles/traffic_<workload>.json), and the
+                   resource those counters name as binding.
+  rmse_vs_checker  the same scene at 480x270, 4 samples, rendered by the GPU
+                   and by the CPU checker (oracle/): linear RMSE and the
+                   fraction of bit-identical pixels.
+  cpu_baseline     the CPU checker (a scalar restatement of the reference's
+                   CPU path, oracle/) on the same scene: full-resolution passes
+                   on host threads for about 10 s.
+  batched          the trajectory workload (below), batched vs plain.
+  served           the mixed request stream (below), served vs plain, at
+                   several arrival rates and as a burst, with p50/p95/p99.
+  served_small     a stream of small requests, served vs plain.
 
 Other workloads (--workload), each printing one JSON line of the same form:
   c1  4096 spheres, 256x256, samples 16, light_samples 8 (plain).
@@ -36,7 +49,7 @@ Other workloads (--workload), each printing one JSON line of the same form:
       time (plain) vs the render server; Mpaths/s and p50/p95/p99 latency per
       arrival rate; images compared bit for bit.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5] [--no-legs]
 """
 
 import argparse
@@ -55,6 +68,13 @@ if ROOT not in sys.path:
 
 METRIC = "Mpaths/s + p50/p99 frame latency, served/batched vs plain, at fixed image RMSE"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# Model FP32 operations of the traversal: a child box slab test (3 axes x
+# (2 subtractions + 2 multiplications) + 4 min/max + the cull bound) ~20, so
+# 80 per 4-wide node visit (4 children) and 40 per binary node visit (2);
+# a sphere test (centre offset, two dot products, discriminant, root) ~20; a
+# ray's setup (3 reciprocals, 3 multiplications) ~6.
+FLOPS_PER_NODE4, FLOPS_PER_NODE2, FLOPS_PER_SPHERE, FLOPS_PER_RAY = 80, 40, 20, 6
 
 
 # ---------------------------------------------------------------- workloads
@@ -189,15 +209,15 @@ def c3_scene(device, n_poly=200000, n_cyl=2000000, n_mesh=50000, side=120.0, see
 
 
 WORKLOADS = {
-    'c2': dict(build=c2_scene, W=1920, H=1080, samples=256, light_samples=1, prim_bytes=16,
+    'c2': dict(build=c2_scene, W=1920, H=1080, samples=256, light_samples=1, prim_bytes=16, node_bytes=128,
                name="1,048,576 spheres, 64^3 FCC lattice a=1.5 jitter 0.05, r~U[0.4,0.6], HSV colors, "
                "roughness 0.3 specular 0.5, outline 0.05, light.cloudy(), perspective fit; "
                "1920x1080, samples=256, light_samples=1"),
-    'c1': dict(build=c1_scene, W=256, H=256, samples=16, light_samples=8, prim_bytes=16,
+    'c1': dict(build=c1_scene, W=256, H=256, samples=16, light_samples=8, prim_bytes=16, node_bytes=128,
                name="4096 spheres uniform in a cube of side 20 (seed 0), r=0.5, outline 0.1, "
                "color linear(0.1,0.8,0.1) roughness 0.8 specular 0.2, light.cloudy(), perspective fit; "
                "256x256, samples=16, light_samples=8"),
-    'c3': dict(build=c3_scene, W=1920, H=1080, samples=128, light_samples=1, prim_bytes=28,
+    'c3': dict(build=c3_scene, W=1920, H=1080, samples=128, light_samples=1, prim_bytes=28, node_bytes=64,
                name="200k cuboid polyhedra 1x1x0.5 random orientations + 2M spherocylinders r=0.1 joining "
                "1st/2nd nearest neighbours of 1M points + 50k icosphere meshes (320 triangles, r=0.4), "
                "cube of side 120, materials rough 0.1/0.5/0.9 metal 0/1/0, light.cloudy(), perspective fit; "
@@ -237,8 +257,9 @@ def cpu_baseline(wl, seconds=10.0, threads=None):
     import fresnel_amd as fresnel
     import oracle.backend
     fresnel._register_cpu_module(oracle.backend)
+    host_cores = os.cpu_count() or 1
     if threads is None:
-        threads = min(16, os.cpu_count() or 1)
+        threads = min(16, host_cores)
     dev = fresnel.Device(mode='cpu', n=threads)
     scene = wl['build'](dev)
     tr = fresnel.tracer.Path(dev, w=wl['W'], h=wl['H'])
@@ -258,11 +279,45 @@ def cpu_baseline(wl, seconds=10.0, threads=None):
         "value": round(paths / dt / 1e6, 4),
         "unit": "Mpaths/s",
         "cores": threads,
+        "host_cores": host_cores,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": f"{passes} full-resolution pass(es) ({wl['W']}x{wl['H']}, light_samples="
                   f"{wl['light_samples']}) of the same scene in {dt:.1f} s; CPU checker (scalar restatement "
-                  f"of the reference CPU path, own BVH) on {threads} threads",
+                  f"of the reference CPU path, own BVH) on {threads} threads ({threads} of {host_cores} host "
+                  f"cores, {cpu_model()})",
     }
+
+
+def cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def rmse_vs_checker(wl, W=480, H=270, samples=4, seed=9):
+    """The workload's scene rendered at W x H, `samples` samples by the GPU and by the CPU checker
+    (same camera, seed and light samples): linear RMSE and the fraction of bit-identical pixels."""
+    import fresnel_amd as fresnel
+    import oracle.backend
+    fresnel._register_cpu_module(oracle.backend)
+    imgs = {}
+    for mode in ('gpu', 'cpu'):
+        dev = fresnel.Device(mode=mode)
+        tr = fresnel.tracer.Path(dev, w=W, h=H)
+        tr.seed = seed
+        tr.sample(wl['build'](dev), samples=samples, light_samples=wl['light_samples'], reset=False)
+        imgs[mode] = numpy.array(tr.linear_output[:])
+    g, c = imgs['gpu'], imgs['cpu']
+    return {"size": f"{W}x{H}", "samples": samples, "light_samples": wl['light_samples'],
+            "rmse": float(numpy.sqrt(numpy.mean((g - c)**2))),
+            "bit_identical_pixels": float(numpy.mean(numpy.all(g.view(numpy.uint32) == c.view(numpy.uint32), axis=2))),
+            "bar": "linear RMSE <= 1e-3 at equal samples"}
 
 
 def run_plain(args):
@@ -322,25 +377,54 @@ def run_plain(args):
         valu_busy = prof.get('valu_busy_pct', {})
         valu_lanes = prof.get('valu_active_lanes_pct', {})
 
+    node_bytes = wl.get('node_bytes', 64)
+    flops_node = FLOPS_PER_NODE4 if node_bytes == 128 else FLOPS_PER_NODE2
+
     def kernel_roofline(name, rays, nodes, prims, ray_bytes, what):
-        """Algorithmic bytes per launch (64 B per BVH node visited, prim_bytes per
-        primitive tested, ray_bytes per ray) over the kernel's mean launch time."""
+        """Algorithmic bytes per launch (node_bytes per hierarchy node visited, prim_bytes per
+        primitive tested, ray_bytes per ray) over the kernel's mean launch time, against HBM; the
+        model FP32 flops of the same launches against the FP32 peak; the binding resource named
+        from the committed counters."""
         ms = sum(st.get(name, (0.0, 0))[0] for st in kstats) / len(kstats)
         launches = max(1, kstats[-1].get(name, (0.0, 1))[1])
-        alg = 64 * nodes + pb * prims + ray_bytes * rays
+        alg = node_bytes * nodes + pb * prims + ray_bytes * rays
         achieved = alg / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+        flops = flops_node * nodes + FLOPS_PER_SPHERE * prims + FLOPS_PER_RAY * rays
+        tflops = flops / (ms / 1e3) / 1e12 if ms > 0 else 0.0
+        t = traffic.get(name)
+        measured_gbs = (t / (ms / launches / 1e3) / 1e9) if (t and ms > 0) else None
+        busy = valu_busy.get(name)
+        lanes = valu_lanes.get(name)
+        if measured_gbs is not None and measured_gbs > 0.6 * HBM_PEAK_GBS:
+            binding = "HBM bandwidth"
+        elif busy is not None and busy >= 90:
+            binding = (f"VALU instruction issue (VALU busy {busy:.0f}%, {lanes:.0f}% of lanes active; "
+                       f"HBM traffic {measured_gbs / HBM_PEAK_GBS:.0%} of peak)" if measured_gbs else
+                       f"VALU instruction issue (VALU busy {busy:.0f}%)")
+        elif busy is not None:
+            binding = (f"latency of dependent loads and divergence (VALU busy {busy:.0f}%, {lanes:.0f}% of lanes "
+                       f"active, HBM traffic {measured_gbs / HBM_PEAK_GBS:.0%} of peak: the hierarchy is served "
+                       f"from L2 and the Infinity Cache)" if measured_gbs else
+                       f"latency of dependent loads and divergence (VALU busy {busy:.0f}%)")
+        else:
+            binding = None
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(name),
-                "kernel": what, "alg_bytes_per_launch": int(alg / launches), "launches_per_step": int(launches),
-                "avg_launch_ms": round(ms / launches, 3), "rays_per_launch": int(rays / launches),
-                "nodes_per_ray": round(nodes / max(rays, 1), 2), "prims_per_ray": round(prims / max(rays, 1), 2),
-                "ms_per_step": round(ms, 3),
-                # the vector ALU side of the same kernel, from the committed
-                # rocprofv3 counters (VALUBusy, VALUUtilization): a kernel far
-                # below the HBM peak is bound by its FP32/integer instruction
-                # issue instead when its VALU is busy most of the time
-                "valu_busy_frac": (round(valu_busy[name] / 100, 3) if name in valu_busy else None),
-                "valu_active_lanes_frac": (round(valu_lanes[name] / 100, 3) if name in valu_lanes else None)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": t,
+                "kernel": what, "alg_bytes_per_launch": int(alg / launches), "node_bytes": node_bytes,
+                "launches_per_step": int(launches), "avg_launch_ms": round(ms / launches, 3),
+                "rays_per_launch": int(rays / launches), "nodes_per_ray": round(nodes / max(rays, 1), 2),
+                "prims_per_ray": round(prims / max(rays, 1), 2), "ms_per_step": round(ms, 3),
+                "valu": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                         "flops_per_launch": int(flops / launches),
+                         "model": f"{flops_node} per node visit, {FLOPS_PER_SPHERE} per primitive test, "
+                                  f"{FLOPS_PER_RAY} per ray"},
+                "measured_hbm_GBs": round(measured_gbs, 1) if measured_gbs else None,
+                # the vector ALU side from the committed rocprofv3 counters
+                # (VALUBusy, VALUUtilization)
+                "valu_busy_frac": (round(busy / 100, 3) if busy is not None else None),
+                "valu_active_lanes_frac": (round(lanes / 100, 3) if lanes is not None else None),
+                "binding": binding}
 
     # primary rays are generated in the kernel; the fused primary kernel
     # (k_primary) writes at least a 16 B record or queue entry per ray, the
@@ -357,8 +441,10 @@ def run_plain(args):
     # whole path: every traced segment (primary, secondary, tail) plus 36 B of
     # accumulation per pixel-sample, over the frame time
     seg_rays = ctr[0] + ctr[3] + ctr[6]
-    path_bytes = (64 * (ctr[1] + ctr[4] + ctr[7]) + pb * (ctr[2] + ctr[5] + ctr[8])
+    path_bytes = (node_bytes * (ctr[1] + ctr[4] + ctr[7]) + pb * (ctr[2] + ctr[5] + ctr[8])
                   + 36 * W * H * S)
+    path_flops = (flops_node * (ctr[1] + ctr[4] + ctr[7]) + FLOPS_PER_SPHERE * (ctr[2] + ctr[5] + ctr[8])
+                  + FLOPS_PER_RAY * seg_rays)
     kernel_ms = {k: round(v[0], 3) for k, v in kstats[-1].items()}
     kernel_launches = {k: int(v[1]) for k, v in kstats[-1].items()}
 
@@ -372,7 +458,10 @@ def run_plain(args):
         "roofline_other": {k: v for k, v in roof.items() if k != dominant},
         "path_roofline": {"alg_bytes_per_path": round(path_bytes / paths, 1),
                           "achieved_GBs": round(path_bytes / mean_s / 1e9, 1),
-                          "frac": round(path_bytes / mean_s / 1e9 / HBM_PEAK_GBS, 4)},
+                          "frac": round(path_bytes / mean_s / 1e9 / HBM_PEAK_GBS, 4),
+                          "model_flops_per_path": round(path_flops / paths, 1),
+                          "achieved_TFLOPs": round(path_flops / mean_s / 1e12, 2),
+                          "valu_frac": round(path_flops / mean_s / 1e12 / FP32_PEAK_TFLOPS, 4)},
         "tail": {"rays": int(ctr[6]), "nodes_per_ray": round(ctr[7] / max(ctr[6], 1), 2)},
         "primary_tile_entries_per_ray": round(ctr[10] / max(ctr[3], 1), 2),
         "kernel_ms_last_step": kernel_ms,
@@ -381,8 +470,10 @@ def run_plain(args):
         "capped_paths": int(ctr[9]),
     })
     if not args.no_cpu_baseline:
+        out["rmse_vs_checker"] = rmse_vs_checker(wl)
+        log(f"rmse vs checker: {out['rmse_vs_checker']}")
         out["cpu_baseline"] = cpu_baseline(wl, seconds=args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    return out
 
 
 # ---------------------------------------------------------------- c4: trajectory
@@ -406,10 +497,11 @@ def trajectory_scene(device, pos, camera=None):
     return scene, g
 
 
-def run_trajectory(args):
-    """64 frames: 64 plain renders (rebuild / refit) vs one batched submission of every frame."""
+def trajectory_leg(frames_n):
+    """`frames_n` frames of the trajectory: plain renders (rebuild / refit) vs one batched submission of every
+    frame, images compared bit for bit.  Returns (summary, seconds per mode)."""
     import fresnel_amd as fresnel
-    W, H, S, F = 1280, 720, 32, args.frames
+    W, H, S, F = 1280, 720, 32, frames_n
     dev = fresnel.Device(mode='gpu', n=1)
     frames = trajectory_frames(F)
     cam = trajectory_scene(dev, frames[0])[0].camera
@@ -453,18 +545,29 @@ def run_trajectory(args):
         results[name] = (dt, dig)
         log(f"c4 {name}: {dt:.3f} s, {paths / dt / 1e6:.1f} Mpaths/s")
     ref = results['plain_rebuild'][1]
-    identical = {k: v[1] == ref for k, v in results.items()}
-    dt_b = results['batched'][0]
-    out = base_record(round(paths / dt_b / 1e6, 3), "Mpaths/s", 1, 1, round(dt_b * 1e3, 2), {
+    identical = all(v[1] == ref for v in results.values())
+    summary = {
         "workload": f"{F} frames of 262,144 spheres (r=0.5, uniform in a cube of side 80 at frame 0, Brownian "
                     f"steps sigma 0.02 per axis per frame, seed 4), 1280x720, samples=32, light_samples=1",
-        "mode": "batched (all frames in one submission)", "paths": paths})
-    out["modes"] = {k: {"seconds": round(v[0], 4), "mpaths_per_s": round(paths / v[0] / 1e6, 2),
-                        "bit_identical_to_plain_rebuild": identical[k]} for k, v in results.items()}
-    out["speedup_batched_vs_plain_rebuild"] = round(results['plain_rebuild'][0] / dt_b, 3)
-    out["frame_digest_0"] = ref[0][:16]
+        "paths": paths,
+        "modes": {k: {"seconds": round(v[0], 4), "mpaths_per_s": round(paths / v[0] / 1e6, 2)}
+                  for k, v in results.items()},
+        "batched_vs_plain": round(results['plain_rebuild'][0] / results['batched'][0], 3),
+        "batched_vs_plain_refit": round(results['plain_refit'][0] / results['batched'][0], 3),
+        "bit_identical": identical,
+    }
+    return summary, {k: v[0] for k, v in results.items()}
+
+
+def run_trajectory(args):
+    """64 frames: 64 plain renders (rebuild / refit) vs one batched submission of every frame."""
+    summary, secs = trajectory_leg(args.frames)
+    dt_b = secs['batched']
+    out = base_record(round(summary['paths'] / dt_b / 1e6, 3), "Mpaths/s", 1, 1, round(dt_b * 1e3, 2), {
+        "workload": summary['workload'], "mode": "batched (all frames in one submission)", "paths": summary['paths']})
+    out.update({k: v for k, v in summary.items() if k not in ('workload', 'paths')})
     print(json.dumps(out), flush=True)
-    if not all(identical.values()):
+    if not summary['bit_identical']:
         sys.exit(1)
 
 
@@ -513,17 +616,30 @@ def request_scene(device, spec, arrays):
     return scene
 
 
-def run_served(args):
-    """Poisson request streams: plain (one request at a time) vs the render server."""
+def small_requests(n_req, seed=7):
+    """Small requests: spheres or spherocylinders, N log-uniform in [1e3, 1e5], 256x256 or 512x512,
+    8 samples (interactive previews), from the same generators."""
+    rng = numpy.random.default_rng(seed)
+    reqs = []
+    for i in range(n_req):
+        n = int(round(math.exp(rng.uniform(math.log(1e3), math.log(1e5)))))
+        kind = 'sphere' if rng.uniform() < 0.5 else 'cylinder'
+        w = int(rng.choice([256, 512]))
+        reqs.append(dict(kind=kind, n=n, w=w, h=w, samples=8, seed=2000 + i))
+    return reqs
+
+
+def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
+    """Poisson streams of `specs` at each rate (req/s; 0 = a burst: every request at time 0): one request at a
+    time as it arrives (plain, Path.sample) vs the render server.  Returns the summary with Mpaths/s and
+    p50/p95/p99 latency (arrival to complete image) per rate, images compared bit for bit."""
     import fresnel_amd as fresnel
     from fresnel_amd import serve
     dev = fresnel.Device(mode='gpu', n=1)
-    specs = make_requests(args.requests)
     t0 = time.perf_counter()
     arrays = [request_arrays(sp) for sp in specs]
-    log(f"c5: {len(specs)} requests generated in {time.perf_counter() - t0:.1f} s")
+    log(f"served leg: {len(specs)} requests generated in {time.perf_counter() - t0:.1f} s")
     total_paths = sum(sp['w'] * sp['h'] * sp['samples'] for sp in specs)
-    rates = [float(x) for x in args.rates.split(',')]
     gaps = numpy.random.default_rng(6).exponential(1.0, len(specs))
 
     def fresh():
@@ -535,26 +651,19 @@ def run_served(args):
         while time.perf_counter() - clock0 < t:
             time.sleep(min(0.002, max(0.0, t - (time.perf_counter() - clock0))))
 
-    def run_plain_stream(arrive, loop):
+    def run_plain_stream(arrive):
         scenes, tracers = fresh()
         done = numpy.zeros(len(specs))
         c0 = time.perf_counter()
         for i in numpy.argsort(arrive, kind='stable'):
             wait_until(c0, arrive[i])
-            t, sp = tracers[i], specs[i]
-            if loop:  # the reference's Path.sample: one render() per camera sample
-                t.reset()
-                t._tracer.setLightSamples(1)
-                for _ in range(sp['samples']):
-                    t._tracer.render(scenes[i]._scene)
-            else:
-                t.sample(scenes[i], samples=sp['samples'], light_samples=1)
+            tracers[i].sample(scenes[i], samples=specs[i]['samples'], light_samples=1)
             done[i] = time.perf_counter() - c0
         return done, [image_digest(t) for t in tracers]
 
     def run_served_stream(arrive):
         scenes, tracers = fresh()
-        srv = serve.Server(dev, hbm_budget=240 << 30, target_paths=args.target_paths)
+        srv = serve.Server(dev, hbm_budget=budget, target_paths=target_paths)
         order = numpy.argsort(arrive, kind='stable')
         handles = [None] * len(specs)
         done = numpy.full(len(specs), numpy.nan)
@@ -578,44 +687,54 @@ def run_served(args):
     table = []
     identical = True
     for lam in rates:
-        arrive = numpy.cumsum(gaps) / lam
-        arrive -= arrive[0]
-        row = {"lambda_per_s": lam}
-        modes = [('plain', lambda: run_plain_stream(arrive, False)), ('served', lambda: run_served_stream(arrive))]
-        if not args.no_loop:
-            modes.insert(0, ('plain_render_loop', lambda: run_plain_stream(arrive, True)))
+        if lam > 0:
+            arrive = numpy.cumsum(gaps) / lam
+            arrive -= arrive[0]
+        else:
+            arrive = numpy.zeros(len(specs))
+        row = {"lambda_per_s": lam if lam > 0 else "burst"}
         digests = {}
-        for name, fn in modes:
-            done, dig = fn()
+        for name, fn in (('plain', run_plain_stream), ('served', run_served_stream)):
+            done, dig = fn(arrive)
             digests[name] = dig
             lat = done - arrive
             span = done.max()
             row[name] = {"mpaths_per_s": round(total_paths / span / 1e6, 2), "makespan_s": round(span, 3),
                          "latency_s": {p: round(float(numpy.percentile(lat, q)), 4)
                                        for p, q in (("p50", 50), ("p95", 95), ("p99", 99))}}
-            log(f"c5 lambda={lam} {name}: {row[name]}")
-        same = all(d == digests['plain'] for d in digests.values())
+            log(f"served leg lambda={lam} {name}: {row[name]}")
+        same = digests['served'] == digests['plain']
         identical = identical and same
         row["served_vs_plain"] = round(row['served']['mpaths_per_s'] / row['plain']['mpaths_per_s'], 3)
-        if 'plain_render_loop' in row:
-            row["served_vs_plain_render_loop"] = round(
-                row['served']['mpaths_per_s'] / row['plain_render_loop']['mpaths_per_s'], 3)
         row["bit_identical"] = same
         table.append(row)
-    top = table[-1]
+    return {"workload": describe, "requests": len(specs), "total_paths": total_paths,
+            "target_paths_per_step": target_paths, "hbm_budget_bytes": budget, "by_rate": table,
+            "best_served_vs_plain": max(r["served_vs_plain"] for r in table), "bit_identical": identical}
+
+
+C5_DESCRIPTION = ("spheres (FCC generator) or spherocylinders (nearest-neighbour generator), N log-uniform in "
+                  "[1e3, 2e6], resolution uniform over {256^2, 1280x720, 1920x1080, 3840x2160}, samples uniform "
+                  "over {8, 64, 256}, light_samples 1; Poisson arrivals; 240 GB budget")
+SMALL_DESCRIPTION = ("small requests: spheres or spherocylinders, N log-uniform in [1e3, 1e5], 256^2 or 512^2, "
+                     "8 samples, light_samples 1; Poisson arrivals; 240 GB budget")
+
+
+def run_served(args):
+    """Poisson request streams: plain (one request at a time) vs the render server."""
+    rates = [float(x) for x in args.rates.split(',')]
+    summary = served_leg(make_requests(args.requests), rates, args.target_paths,
+                         f"{args.requests} requests: " + C5_DESCRIPTION)
+    top = [r for r in summary['by_rate'] if r['lambda_per_s'] != 'burst'][-1]
     out = base_record(top['served']['mpaths_per_s'], "Mpaths/s", len(rates), 0,
                       round(top['served']['makespan_s'] * 1e3, 2), {
-                          "workload": f"{len(specs)} requests: spheres (FCC generator) or spherocylinders "
-                                      "(nearest-neighbour generator), N log-uniform in [1e3, 2e6], resolution "
-                                      "uniform over {256^2, 1280x720, 1920x1080, 3840x2160}, samples uniform "
-                                      "over {8, 64, 256}, light_samples 1; Poisson arrivals; 240 GB budget",
-                          "mode": "served", "total_paths": total_paths,
+                          "workload": summary['workload'], "mode": "served", "total_paths": summary['total_paths'],
                           "target_paths_per_step": args.target_paths})
     out["latency_s"] = top['served']['latency_s']
-    out["by_rate"] = table
-    out["bit_identical"] = identical
+    out["by_rate"] = summary['by_rate']
+    out["bit_identical"] = summary['bit_identical']
     print(json.dumps(out), flush=True)
-    if not identical:
+    if not summary['bit_identical']:
         sys.exit(1)
 
 
@@ -628,13 +747,13 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--no-tune', action='store_true', help='skip the launch-shape autotuner')
+    ap.add_argument('--no-legs', action='store_true', help='c2: skip the batched and served legs')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='first run a plain device copy of MIB MiB (the reference kernel for HBM byte counters)')
     ap.add_argument('--frames', type=int, default=64, help='c4: trajectory frames')
     ap.add_argument('--requests', type=int, default=24, help='c5: requests per stream')
-    ap.add_argument('--rates', default='5,10,20,50,100,200', help='c5: arrival rates (req/s)')
-    ap.add_argument('--target-paths', type=int, default=128 << 20, help='c5: paths per server step')
-    ap.add_argument('--no-loop', action='store_true', help='c5: skip the render()-per-sample plain form')
+    ap.add_argument('--rates', default='5,10,20,50,100,200,0', help='c5: arrival rates (req/s; 0 = burst)')
+    ap.add_argument('--target-paths', type=int, default=256 << 20, help='c5: paths per server step')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -645,7 +764,17 @@ def main():
     elif args.workload == 'c5':
         run_served(args)
     else:
-        run_plain(args)
+        out = run_plain(args)
+        if args.workload == 'c2' and not args.no_legs:
+            # the other two forms of the metric, each against its plain form
+            t0 = time.perf_counter()
+            out["batched"] = trajectory_leg(64)[0]
+            out["served"] = served_leg(make_requests(24), [5.0, 50.0, 200.0, 0.0], 256 << 20,
+                                       "24 requests: " + C5_DESCRIPTION)
+            out["served_small"] = served_leg(small_requests(96), [200.0, 0.0], 32 << 20,
+                                             "96 requests: " + SMALL_DESCRIPTION)
+            log(f"batched and served legs: {time.perf_counter() - t0:.1f} s")
+        print(json.dumps(out), flush=True)
 
 
 if __name__ == '__main__':

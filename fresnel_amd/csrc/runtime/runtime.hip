@@ -1431,7 +1431,8 @@ static void set_order_divisors(JobDev& J)
 static size_t stage_bytes(size_t n_jobs)
     {
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return up(n_jobs * sizeof(JobDev)) + up(n_jobs * sizeof(uint32_t)) + up(2 * MAX_STEPS * sizeof(uint32_t));
+    return up(n_jobs * sizeof(JobDev)) + up(n_jobs * sizeof(uint32_t)) + up(2 * MAX_STEPS * sizeof(uint32_t))
+           + up(2 * sizeof(unsigned long long));
     }
 
 // One batch: every job advances by jobs[j].K camera samples, on one render
@@ -1460,6 +1461,7 @@ struct BatchExec
     Queue Q[2];
     uint8_t* stage;
     uint32_t* h_steps = nullptr; // pinned copy of the step records (valid after the batch has run)
+    unsigned long long* h_flags = nullptr; // pinned copy of the overflow flags after the batch
 
     BatchExec(DeviceObj& d, RenderScratch& r, hipStream_t st, std::vector<JobDev> j, uint32_t t, uint8_t* stg)
         : D(d), R(r), s(st), jobs(std::move(j)), types(t), spheres(t == T_SPHERES), stage(stg)
@@ -1513,6 +1515,17 @@ struct BatchExec
         const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
         seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * groups, D.trav_chunk);
         const uint64_t cap = (uint64_t)QSEG * seg_cap;
+        // growing a buffer frees it first: an earlier batch on this stream
+        // (the only user of this scratch) must have finished with it
+        gstride = D.trav_grid * TRAV_BLOCK; // max resident threads of any traversal variant
+        tail_threads = blocks(D.tail_rays, TRAV_BLOCK) * TRAV_BLOCK;
+        const bool fits = R.hits.cap >= std::max(n_pri, n_paths) && R.qo[0].cap >= cap && R.qo[1].cap >= cap
+                          && R.records.cap >= n_rec && R.jobs.cap >= jobs.size() && R.pix_base.cap >= jobs.size()
+                          && R.gstack.cap >= (size_t)(STACK_MAX - 8) * gstride
+                          && R.tail_stack.cap >= (size_t)(STACK_MAX - TRAV_SS) * tail_threads && R.view.p && R.steps.p
+                          && R.counters32.p && R.qseg.p;
+        if (!fits)
+            HIPCHECK(hipStreamSynchronize(s));
         R.hits.ensure(std::max(n_pri, n_paths));
         for (int q = 0; q < 2; q++)
             {
@@ -1538,10 +1551,10 @@ struct BatchExec
         HIPCHECK(hipMemcpyAsync(R.pix_base.p, p, pix_base.size() * 4, hipMemcpyHostToDevice, s));
         p += (pix_base.size() * 4 + 255) & ~(size_t)255;
         h_steps = (uint32_t*)p;
-        gstride = D.trav_grid * TRAV_BLOCK; // max resident threads of any traversal variant
+        p += (2 * MAX_STEPS * sizeof(uint32_t) + 255) & ~(size_t)255;
+        h_flags = (unsigned long long*)p;
         R.gstack.ensure((size_t)(STACK_MAX - 8) * gstride);
         // the tail kernel's grid-stride threads (at least tail_rays)
-        tail_threads = blocks(D.tail_rays, TRAV_BLOCK) * TRAV_BLOCK;
         R.tail_stack.ensure((size_t)(STACK_MAX - TRAV_SS) * tail_threads);
         for (int q = 0; q < 2; q++)
             {
@@ -1690,10 +1703,13 @@ struct BatchExec
         n_steps = steps;
         }
 
-    // Copy the step records to the pinned host copy (after the steps).
+    // Copy the step records and the device's overflow flags (queue segment,
+    // traversal stack) to the pinned staging, after the steps.
     void copy_steps()
         {
         HIPCHECK(hipMemcpyAsync(h_steps, R.steps.p, 2 * n_steps * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(h_flags, D.counters.p + CTR_QUEUE_OVERFLOW, 2 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, s));
         }
 
     // The batch's kernels, launched or (D.graph == 2 and no per-launch timing
@@ -1987,13 +2003,17 @@ struct RequestObj
     };
 
 // A batch of the server: its requests and the samples it adds.  Planned
-// batches wait (slot -1) until a render scratch is free; a started batch is
-// enqueued whole on that slot's stream and is done when its accumulation
-// event has completed.
+// batches wait (stream -1) until one of the server's two streams has fewer
+// than SERVER_DEPTH batches queued; a started batch is enqueued whole on that
+// stream (which owns one render scratch) and is done when its `done` event
+// (recorded after its accumulation) has completed.
+constexpr int SERVER_DEPTH = 2;
 struct ServerBatch
     {
     std::unique_ptr<BatchExec> e;
-    int slot = -1; // render scratch and stream of the server, -1 while waiting
+    std::unique_ptr<PinnedBuf> stage; // job table, step records, flags of this batch
+    hipEvent_t done = nullptr;
+    int stream = -1; // server stream (and render scratch), -1 while waiting
     std::vector<JobDev> jobs;
     uint32_t types = 0;
     std::vector<std::pair<uint64_t, uint32_t>> adds; // (request, samples)
@@ -2010,19 +2030,20 @@ constexpr int REQ_QUEUED = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_REJECTED = 3, RE
 struct ServerObj
     {
     DeviceObj* dev = nullptr;
-    uint64_t budget = 0, in_use = 0, target_paths = 128ull << 20;
+    uint64_t budget = 0, in_use = 0, target_paths = 256ull << 20;
     std::deque<PolicyItem> queue;
     std::vector<uint64_t> active; // admission order
     std::map<uint64_t, RequestObj> reqs;
     uint64_t next_id = 1;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     uint64_t iterations = 0;
-    // two batches in flight on the server's own render scratches and streams,
-    // batches in plan order (front: oldest); accumulations chained by events
+    // batches in plan order (front: oldest) on the server's two streams and
+    // render scratches, up to SERVER_DEPTH per stream; accumulations chained
+    // by events
     RenderScratch rs[2];
-    PinnedBuf stage[2]; // job tables and step records of the batch on each slot
     hipStream_t st[2] = {nullptr, nullptr};
-    hipEvent_t acc[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> free_events;
+    std::vector<hipEvent_t> all_events;
     std::deque<ServerBatch> batches;
     // last accumulation enqueued per batch kind (spheres-only / mixed): a
     // request's batches are all of one kind, so only batches of one kind
@@ -2189,30 +2210,28 @@ static void server_fail(ServerObj& S, uint64_t id, double now)
     S.active.erase(std::remove(S.active.begin(), S.active.end(), id), S.active.end());
     }
 
-// Finish a started batch whose accumulation has completed: check the device's
-// overflow flags, count its samples as done, free its slot.
+// Finish a started batch whose accumulation has completed: check the overflow
+// flags copied after it, count its samples as done, free its event.
 static void server_finish_batch(ServerObj& S, size_t i)
     {
     DeviceObj& D = *S.dev;
     ServerBatch& b = S.batches[i];
-    uint64_t flags[2] = {0, 0};
-    HIPCHECK(hipMemcpyAsync(flags, D.counters.p + CTR_QUEUE_OVERFLOW, sizeof(flags), hipMemcpyDeviceToHost,
-                            S.st[b.slot]));
-    HIPCHECK(hipStreamSynchronize(S.st[b.slot]));
     update_steps_est(D, b.e->steps_needed());
     float live[MAX_STEPS] = {};
     b.e->note_live(live);
     update_live_frac(D, live);
-    if (flags[0])
-        throw Error(FH_ERR_STATE, "internal error: a ray queue segment overflowed");
-    if (flags[1])
-        throw Error(FH_ERR_STATE, "traversal stack overflow");
+    const unsigned long long q_flag = b.e->h_flags[0], s_flag = b.e->h_flags[1];
     for (auto& a : b.adds)
         S.reqs[a.first].done += a.second;
     const int kd = b.kind();
     if (S.last_acc_batch[kd] == b.serial)
         S.last_acc[kd] = nullptr; // finished: later batches of its kind need not wait
+    S.free_events.push_back(b.done);
     S.batches.erase(S.batches.begin() + (std::ptrdiff_t)i);
+    if (q_flag)
+        throw Error(FH_ERR_STATE, "internal error: a ray queue segment overflowed");
+    if (s_flag)
+        throw Error(FH_ERR_STATE, "traversal stack overflow");
     }
 
 // One server iteration: admit; retire started batches whose accumulation has
@@ -2271,41 +2290,38 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         }
     if (!S.st[0])
         for (int k = 0; k < 2; k++)
-            {
             HIPCHECK(hipStreamCreateWithFlags(&S.st[k], hipStreamNonBlocking));
-            HIPCHECK(hipEventCreateWithFlags(&S.acc[k], hipEventDisableTiming));
-            }
     if (S.draining)
-        while (!S.batches.empty() && S.batches.back().slot < 0)
+        while (!S.batches.empty() && S.batches.back().stream < 0)
             S.batches.pop_back();
     // retire started batches that have completed
     for (size_t i = 0; i < S.batches.size();)
         {
         ServerBatch& b = S.batches[i];
-        if (b.slot >= 0 && hipEventQuery(S.acc[b.slot]) == hipSuccess)
+        if (b.stream >= 0 && hipEventQuery(b.done) == hipSuccess)
             server_finish_batch(S, i);
         else
             i++;
         }
     bool waiting = false;
     for (auto& b : S.batches)
-        waiting = waiting || b.slot < 0;
+        waiting = waiting || b.stream < 0;
     if (!waiting && !S.draining)
         server_plan(S);
-    // start waiting batches on free slots (scene, tile and output work on the
-    // device stream completes first)
-    bool used[2] = {false, false};
+    // start waiting batches on streams with room (scene, tile and output work
+    // on the device stream completes first)
+    int queued[2] = {0, 0};
     for (auto& b : S.batches)
-        if (b.slot >= 0)
-            used[b.slot] = true;
+        if (b.stream >= 0)
+            queued[b.stream]++;
     bool synced = false, started = false;
     for (size_t i = 0; i < S.batches.size(); i++)
         {
         ServerBatch& b = S.batches[i];
-        if (b.slot >= 0)
+        if (b.stream >= 0)
             continue;
-        const int slot = !used[0] ? 0 : (!used[1] ? 1 : -1);
-        if (slot < 0)
+        const int k = queued[0] <= queued[1] ? 0 : 1;
+        if (queued[k] >= SERVER_DEPTH)
             break;
         if (!synced)
             {
@@ -2314,61 +2330,62 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
             }
         try
             {
-            // the slot's previous batch has completed, so its staging is free
-            S.stage[slot].ensure(stage_bytes(b.jobs.size()));
-            b.e.reset(new BatchExec(D, S.rs[slot], S.st[slot], b.jobs, b.types, S.stage[slot].p));
+            if (S.free_events.empty())
+                {
+                hipEvent_t ev;
+                HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+                S.all_events.push_back(ev);
+                S.free_events.push_back(ev);
+                }
+            b.stage.reset(new PinnedBuf());
+            b.stage->ensure(stage_bytes(b.jobs.size()));
+            b.e.reset(new BatchExec(D, S.rs[k], S.st[k], b.jobs, b.types, b.stage->p));
             b.e->start();
             b.e->enqueue_kernels(D.steps_est);
             b.e->copy_steps();
+            b.done = S.free_events.back();
             const int kd = b.kind();
-            b.e->accumulate(S.last_acc[kd], S.acc[slot]);
-            S.last_acc[kd] = S.acc[slot];
+            b.e->accumulate(S.last_acc[kd], b.done);
+            S.free_events.pop_back();
+            S.last_acc[kd] = b.done;
             S.last_acc_batch[kd] = b.serial;
             }
         catch (...)
             {
             // whatever was enqueued drains; the batch's requests cannot complete
-            (void)hipStreamSynchronize(S.st[slot]);
+            (void)hipStreamSynchronize(S.st[k]);
             for (auto& a : b.adds)
                 server_fail(S, a.first, now);
             S.batches.erase(S.batches.begin() + (std::ptrdiff_t)i);
             throw;
             }
-        b.slot = slot;
-        used[slot] = true;
+        b.stream = k;
+        queued[k]++;
         started = true;
         }
-    // nothing could start and both slots are busy: wait for the oldest
-    // started batch rather than let the caller spin
-    if (!started && used[0] && used[1])
-        for (size_t i = 0; i < S.batches.size(); i++)
-            if (S.batches[i].slot >= 0)
-                {
-                HIPCHECK(hipEventSynchronize(S.acc[S.batches[i].slot]));
-                server_finish_batch(S, i);
-                break;
-                }
-    // with nothing waiting to start, the last started batches are waited for
-    // too (the caller has nothing else to overlap)
-    if (S.draining && !S.batches.empty() && S.batches.front().slot >= 0)
-        {
-        HIPCHECK(hipEventSynchronize(S.acc[S.batches.front().slot]));
-        server_finish_batch(S, 0);
-        }
-    else if (S.queue.empty())
+    // wait for the oldest started batch rather than let the caller spin: when
+    // nothing could start and both streams are full, when destroying, or when
+    // nothing is left to plan or start
+    bool wait = !started && queued[0] >= SERVER_DEPTH && queued[1] >= SERVER_DEPTH;
+    wait = wait || S.draining;
+    if (!wait && S.queue.empty())
         {
         bool any_waiting = false;
         for (auto& b : S.batches)
-            any_waiting = any_waiting || b.slot < 0;
+            any_waiting = any_waiting || b.stream < 0;
         bool plannable = false;
         for (uint64_t id : S.active)
             plannable = plannable || S.reqs[id].scheduled < S.reqs[id].samples;
-        if (!any_waiting && !plannable && !S.batches.empty())
-            {
-            HIPCHECK(hipEventSynchronize(S.acc[S.batches.front().slot]));
-            server_finish_batch(S, 0);
-            }
+        wait = !any_waiting && !plannable;
         }
+    if (wait)
+        for (size_t i = 0; i < S.batches.size(); i++)
+            if (S.batches[i].stream >= 0)
+                {
+                HIPCHECK(hipEventSynchronize(S.batches[i].done));
+                server_finish_batch(S, i);
+                break;
+                }
     server_retire(S, now);
     n_active = (uint32_t)S.active.size();
     n_queued = (uint32_t)S.queue.size();
@@ -3414,12 +3431,10 @@ extern "C"
                     {
                     }
                 for (int k = 0; k < 2; k++)
-                    {
                     if (S.st[k])
                         (void)hipStreamSynchronize(S.st[k]);
-                    if (S.acc[k])
-                        (void)hipEventDestroy(S.acc[k]);
-                    }
+                for (hipEvent_t e : S.all_events)
+                    (void)hipEventDestroy(e);
                 hipStream_t st0 = S.st[0], st1 = S.st[1];
                 delete srv;
                 if (st0)

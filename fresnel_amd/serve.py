@@ -47,13 +47,16 @@ class Server:
 
     Args:
         device: a GPU device.
-        hbm_budget (int): bytes of HBM the active requests may hold.
-        target_paths (int): light-sample paths traced per step.  The server's
-            two render scratches take about 2 x 160 B x target_paths of HBM
-            beyond the budget (43 GB at the default 128 Mi).
+        hbm_budget (int): bytes of HBM the server may hold: the active
+            requests' footprints plus its two render scratches (about
+            2 x 160 B x target_paths, 86 GB at the default 256 Mi) plus any
+            render scratch the device keeps for Path.sample (released when
+            it would keep a queued request out).
+        target_paths (int): light-sample paths traced per step (the batch
+            shape of Path.sample).
     """
 
-    def __init__(self, device, hbm_budget=240 << 30, target_paths=128 << 20):
+    def __init__(self, device, hbm_budget=240 << 30, target_paths=256 << 20):
         if device.mode != 'gpu':
             raise RuntimeError("the render server needs a GPU device")
         self.device = device

@@ -261,7 +261,7 @@ int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t s
  * tracer's light-sample count is set back to 1 at completion, as
  * Path.sample leaves it.
  * Times are seconds; pass now < 0 to use the server's monotonic clock.
- * target_paths_per_step = 0 selects 128 Mi; the server's two render scratches
+ * target_paths_per_step = 0 selects 256 Mi; the server's two render scratches
  * (about 2 x 160 B x target_paths_per_step) are charged against the budget. */
 typedef struct fh_server_s* fh_server;
 int fh_server_create(fh_device dev, uint64_t hbm_budget_bytes, uint64_t target_paths_per_step,

@@ -107,3 +107,44 @@ def test_random_stream_invariants(lib):
     for it in range(int(done.max()) + 1):
         live = (adm <= it) & (done >= it)
         assert fp[live].sum() <= budget
+
+
+def scratch(lib, paths, fixed):
+    out = ctypes.c_uint64()
+    assert lib.fh_policy_scratch_bytes(ctypes.c_uint64(paths), ctypes.c_uint64(fixed), ctypes.byref(out)) == 0
+    return out.value
+
+
+def test_scratch_bytes_formula(lib):
+    # two ray queues of 64 B, hits and records of 16 B per path, plus the size-independent part
+    assert scratch(lib, 1000, 7) == 1000 * 160 + 7
+    assert scratch(lib, 256 << 20, 0) == (256 << 20) * 160
+
+
+def test_full_budget_with_scratches_stays_in_hbm(lib):
+    """A 240 GB budget holds the server's two render scratches (256 Mi paths per step), the device's two
+    Path.sample scratches (256 Mi-path batches) and the admitted requests: admission gets only what is left,
+    so the peak of everything together stays within the budget and inside the 288 GB of one MI355X."""
+    GB = 1 << 30
+    budget = 240 * GB
+    fixed = 300 << 20  # overflow stacks and counters per scratch (upper bound)
+    server = 2 * scratch(lib, 256 << 20, fixed)
+    device = 2 * scratch(lib, 256 << 20, fixed)
+    room = budget - server - device
+    assert room > 0
+    rng = numpy.random.default_rng(5)
+    n = 200
+    res = [(256, 256), (1280, 720), (1920, 1080), (3840, 2160)]
+    fp, pps, samples = [], [], []
+    for _ in range(n):
+        prims = int(numpy.exp(rng.uniform(numpy.log(1e3), numpy.log(2e6))))
+        w, h = res[rng.integers(0, 4)]
+        fp.append(footprint(lib, prims, 32, w, h, 1))
+        pps.append(w * h)
+        samples.append(int(rng.choice([8, 64, 256])))
+    adm, done, peak = simulate(lib, fp, pps, samples, numpy.zeros(n), budget=room, target=256 << 20)
+    assert (done >= 0).all()
+    assert peak <= room
+    assert peak + server + device <= budget <= 288 * GB
+    # the budget binds: not everything is admitted at once
+    assert sum(fp) > room and adm.max() > 0
