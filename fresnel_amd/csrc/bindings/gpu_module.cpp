@@ -652,6 +652,28 @@ PYBIND11_MODULE(_gpu, m)
                  return out;
              })
         .def("handle", [](std::shared_ptr<Tracer> t) { return (uintptr_t)t->h; })
+        .def("outputAsync", [](std::shared_ptr<Tracer> t, int which) { check(fh_tracer_output_async(t->h, which)); })
+        .def("outputReady",
+             [](std::shared_ptr<Tracer> t, int which)
+             {
+                 int r = 0;
+                 check(fh_tracer_output_ready(t->h, which, &r));
+                 return r != 0;
+             })
+        .def("outputWait",
+             [](std::shared_ptr<Tracer> t, int which)
+             {
+                 void* p = nullptr;
+                 uint32_t w = 0, hh = 0;
+                 check(fh_tracer_output_wait(t->h, which, &p, &w, &hh));
+                 // a view of the tracer's pinned buffer (kept alive by the tracer)
+                 py::object base = py::cast(t);
+                 if (which == FH_OUT_SRGB)
+                     return py::array(py::dtype::of<uint8_t>(), {(py::ssize_t)hh, (py::ssize_t)w, (py::ssize_t)4},
+                                      {(py::ssize_t)w * 4, (py::ssize_t)4, (py::ssize_t)1}, p, base);
+                 return py::array(py::dtype::of<float>(), {(py::ssize_t)hh, (py::ssize_t)w, (py::ssize_t)4},
+                                  {(py::ssize_t)w * 16, (py::ssize_t)16, (py::ssize_t)4}, p, base);
+             })
         .def("enableHighlightWarning", &Tracer::enableHighlightWarning)
         .def("disableHighlightWarning", &Tracer::disableHighlightWarning)
         .def("getSeed", &Tracer::getSeed)

@@ -203,6 +203,7 @@ struct DeviceObj
     int hip_dev = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr; // second render stream: two batches in flight
+    hipStream_t copy_stream = nullptr; // asynchronous output readback
     hipStream_t lstream = nullptr; // stream of the launches being enqueued (timing events)
     int n_cu = 0;
     std::string name;
@@ -1201,11 +1202,41 @@ struct TracerObj
     DBuf<uint32_t> hist; // histogram counts (512 x 4)
     std::vector<float> h_linear;
     std::vector<uint8_t> h_srgb;
+    // asynchronous readback (fh_tracer_output_async): a pinned copy of each
+    // output, the event after its copy, and whether a render must still wait
+    // for that copy before writing the output again
+    PinnedBuf h_async[2];
+    hipEvent_t async_ev[2] = {nullptr, nullptr};
+    bool async_unordered[2] = {false, false};
+    bool async_started[2] = {false, false};
+    uint32_t async_w[2] = {0, 0}, async_h[2] = {0, 0};
+
+    // make the render streams wait for outstanding output copies
+    void order_after_copies(hipStream_t a, hipStream_t b)
+        {
+        for (int k = 0; k < 2; k++)
+            if (async_unordered[k])
+                {
+                HIPCHECK(hipStreamWaitEvent(a, async_ev[k], 0));
+                if (b)
+                    HIPCHECK(hipStreamWaitEvent(b, async_ev[k], 0));
+                async_unordered[k] = false;
+                }
+        }
+    // wait until outstanding output copies are done (before the outputs are
+    // cleared or freed)
+    void finish_copies()
+        {
+        for (int k = 0; k < 2; k++)
+            if (async_started[k])
+                HIPCHECK(hipEventSynchronize(async_ev[k]));
+        }
 
     void resize(uint32_t w, uint32_t h)
         {
         if (w == 0 || h == 0)
             throw Error(FH_ERR_INVALID, "Invalid dimensions");
+        finish_copies();
         W = w;
         H = h;
         linear.release();
@@ -1218,6 +1249,7 @@ struct TracerObj
         }
     void clear()
         {
+        finish_copies();
         HIPCHECK(hipMemsetAsync(linear.p, 0, (size_t)W * H * sizeof(float4), dev->stream));
         HIPCHECK(hipMemsetAsync(srgb.p, 0, (size_t)W * H * sizeof(uchar4), dev->stream));
         HIPCHECK(hipStreamSynchronize(dev->stream));
@@ -1924,6 +1956,9 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
         sync_scene(*sc);
     for (size_t k = 0; k < scenes.size(); k++)
         update_tiles(D, *scenes[k], *tracers[k], make_cam(scenes[k]->camera));
+    // a readback of an output still in flight finishes before the accumulation
+    for (auto* tr : tracers)
+        tr->order_after_copies(D.stream, D.stream2);
     uint64_t per_sample = 0;
     for (auto* tr : tracers)
         per_sample += (uint64_t)tr->W * tr->H * tr->L;
@@ -1965,6 +2000,7 @@ static void render_preview(DeviceObj& D, SceneObj& sc, TracerObj& tr)
     cam.f_stop = __builtin_huge_valf();
     JobDev J = make_job(sc, tr, cam);
     J.K = 1;
+    tr.order_after_copies(D.stream, nullptr);
     RenderScratch& R = D.rs;
     R.jobs.ensure(1);
     HIPCHECK(hipMemcpyAsync(R.jobs.p, &J, sizeof(J), hipMemcpyHostToDevice, D.stream));
@@ -2137,6 +2173,7 @@ static void server_plan(ServerObj& S)
         if (fit == 0)
             continue;
         paths[kd] += fit * pps;
+        r.tracer->finish_copies(); // its accumulation may start on either stream
         update_tiles(D, *r.scene, *r.tracer, make_cam(r.scene->camera));
         JobDev J = make_job(*r.scene, *r.tracer, r.scene->camera);
         J.K = fit;
@@ -2503,6 +2540,7 @@ extern "C"
                 D.name = std::string(p.name) + " (" + p.gcnArchName + ")";
                 HIPCHECK(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
                 HIPCHECK(hipStreamCreateWithFlags(&D.stream2, hipStreamNonBlocking));
+                HIPCHECK(hipStreamCreateWithFlags(&D.copy_stream, hipStreamNonBlocking));
                 for (auto& e : D.acc_done)
                     HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 // overflow stacks are sized for the largest resident traversal grid
@@ -2550,12 +2588,16 @@ extern "C"
                 for (auto e : dev->obj.acc_done)
                     if (e)
                         (void)hipEventDestroy(e);
-                hipStream_t s = dev->obj.stream, s2 = dev->obj.stream2;
+                hipStream_t s = dev->obj.stream, s2 = dev->obj.stream2, s3 = dev->obj.copy_stream;
+                if (s3)
+                    (void)hipStreamSynchronize(s3);
                 delete dev;
                 if (s)
                     (void)hipStreamDestroy(s);
                 if (s2)
                     (void)hipStreamDestroy(s2);
+                if (s3)
+                    (void)hipStreamDestroy(s3);
             });
         }
 
@@ -3142,6 +3184,12 @@ extern "C"
                 if (!tr)
                     return;
                 (void)hipStreamSynchronize(tr->obj.dev->stream);
+                for (int k = 0; k < 2; k++)
+                    if (tr->obj.async_ev[k])
+                        {
+                        (void)hipEventSynchronize(tr->obj.async_ev[k]);
+                        (void)hipEventDestroy(tr->obj.async_ev[k]);
+                        }
                 delete tr;
             });
         }
@@ -3219,6 +3267,67 @@ extern "C"
                     }
                 else
                     throw Error(FH_ERR_INVALID, "unknown output buffer");
+            });
+        }
+    int fh_tracer_output_async(fh_tracer tr, int which)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                REQUIRE(which == FH_OUT_SRGB || which == FH_OUT_LINEAR, "unknown output buffer");
+                TracerObj& T = tr->obj;
+                DeviceObj& D = *T.dev;
+                std::lock_guard<std::mutex> lock(D.mu);
+                const size_t bytes = (size_t)T.W * T.H * (which == FH_OUT_SRGB ? 4 : 16);
+                const void* src = which == FH_OUT_SRGB ? (const void*)T.srgb.p : (const void*)T.linear.p;
+                if (T.async_started[which])
+                    HIPCHECK(hipEventSynchronize(T.async_ev[which])); // the pinned buffer is reused
+                else if (!T.async_ev[which])
+                    HIPCHECK(hipEventCreateWithFlags(&T.async_ev[which], hipEventDisableTiming));
+                T.h_async[which].ensure(bytes);
+                // the tracer's renders return complete, but a clear may still be queued
+                HIPCHECK(hipStreamSynchronize(D.stream));
+                HIPCHECK(hipMemcpyAsync(T.h_async[which].p, src, bytes, hipMemcpyDeviceToHost, D.copy_stream));
+                HIPCHECK(hipEventRecord(T.async_ev[which], D.copy_stream));
+                T.async_started[which] = true;
+                T.async_unordered[which] = true;
+                T.async_w[which] = T.W;
+                T.async_h[which] = T.H;
+            });
+        }
+    int fh_tracer_output_ready(fh_tracer tr, int which, int* ready)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(ready);
+                REQUIRE(which == FH_OUT_SRGB || which == FH_OUT_LINEAR, "unknown output buffer");
+                TracerObj& T = tr->obj;
+                REQUIRE(T.async_started[which], "no asynchronous copy of this output was started");
+                const hipError_t e = hipEventQuery(T.async_ev[which]);
+                if (e != hipSuccess && e != hipErrorNotReady)
+                    HIPCHECK(e);
+                *ready = e == hipSuccess ? 1 : 0;
+            });
+        }
+    int fh_tracer_output_wait(fh_tracer tr, int which, void** data, uint32_t* w, uint32_t* h)
+        {
+        return guard(
+            [&]
+            {
+                NONNULL(tr);
+                NONNULL(data);
+                REQUIRE(which == FH_OUT_SRGB || which == FH_OUT_LINEAR, "unknown output buffer");
+                TracerObj& T = tr->obj;
+                REQUIRE(T.async_started[which], "no asynchronous copy of this output was started");
+                HIPCHECK(hipEventSynchronize(T.async_ev[which]));
+                *data = T.h_async[which].p;
+                if (w)
+                    *w = T.async_w[which];
+                if (h)
+                    *h = T.async_h[which];
             });
         }
     int fh_tracer_output_host(fh_tracer tr, int which, void** data)

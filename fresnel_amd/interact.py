@@ -130,6 +130,7 @@ class ProgressiveView:
         self._rendering = False
         self._samples = 0
         self._batch = 1
+        self._pending = None
         self.restart()
 
     @property
@@ -169,6 +170,7 @@ class ProgressiveView:
         self._tracer.resize(w=w, h=h)
         f = self.low_res_factor
         self._low_res_tracer.resize(w=max(w // f, 1), h=max(h // f, 1))
+        self._pending = None
         self.restart()
 
     def restart(self):
@@ -191,6 +193,28 @@ class ProgressiveView:
 
     def repaint(self):
         """Render for one repaint and return the `ImageArray` to display."""
+        self._render()
+        return self._tracer.output if self._high_res else self._low_res_tracer.output
+
+    def repaint_async(self):
+        """Render for one repaint with the image read back asynchronously.
+
+        Renders the next samples, starts reading the new image back, and
+        returns (as an (h, w, 4) uint8 array) the image the previous call
+        started reading: that copy ran while the display drew and while this
+        call traced.  The first call waits for its own image.  GPU devices.
+        """
+        prev = None
+        if self._pending is not None:
+            prev = numpy.array(self._pending.wait())
+        self._render()
+        t = self._tracer if self._high_res else self._low_res_tracer
+        self._pending = t.output_async()
+        if prev is None:
+            prev = numpy.array(self._pending.wait())
+        return prev
+
+    def _render(self):
         if self._rendering:
             if self._high_res:
                 k = min(self._batch, self.max_samples - self._samples)
@@ -207,7 +231,6 @@ class ProgressiveView:
                 self._batch = max(1, min(4 * k, want))
             else:
                 self._low_res_tracer.render(self._scene)
-        return self._tracer.output if self._high_res else self._low_res_tracer.output
 
 
 if QtWidgets is not None:  # pragma: no cover - needs a Qt binding and a display

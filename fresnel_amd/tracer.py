@@ -62,6 +62,17 @@ class Tracer(object):
         """ImageArray: the (h, w, 4) sRGB uint8 output."""
         return util.ImageArray(self._tracer.getSRGBOutputBuffer(), geom=None)
 
+    def output_async(self, linear=False):
+        """Start reading the output back without waiting (GPU devices).
+
+        The copy runs on the device's copy engine while the host continues;
+        a later render of this tracer traces concurrently with it and writes
+        the output only after it.  Returns an `AsyncOutput`.
+        """
+        which = 1 if linear else 0
+        self._tracer.outputAsync(which)
+        return AsyncOutput(self, which)
+
     @property
     def linear_output(self):
         """Array: the (h, w, 4) linear float32 output."""
@@ -75,6 +86,24 @@ class Tracer(object):
     @seed.setter
     def seed(self, value):
         self._tracer.setSeed(value)
+
+
+class AsyncOutput(object):
+    """A readback started by `Tracer.output_async`."""
+
+    def __init__(self, tracer, which):
+        self._tracer = tracer
+        self._which = which
+
+    def ready(self):
+        """bool: whether the copy has completed."""
+        return self._tracer._tracer.outputReady(self._which)
+
+    def wait(self):
+        """The (h, w, 4) image (uint8 sRGB or float32 linear) as rendered when the copy
+        started: a view of the tracer's host buffer, valid until its next
+        `output_async` of the same output."""
+        return self._tracer._tracer.outputWait(self._which)
 
 
 class Preview(Tracer):

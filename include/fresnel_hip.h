@@ -198,6 +198,21 @@ int fh_tracer_render_samples(fh_tracer tr, fh_scene scene, uint32_t samples);
 #define FH_OUT_SRGB 0
 #define FH_OUT_LINEAR 1
 int fh_tracer_map_output(fh_tracer tr, int which, void** data);
+/* Asynchronous readback (no reference counterpart: the reference copies on
+ * map(), fresnel/gpu/Array.h:203-215).  fh_tracer_output_async enqueues a copy
+ * of the output (as rendered by the tracer's last completed render) into a
+ * pinned host buffer of the tracer on the device's copy stream and returns at
+ * once; fh_tracer_output_wait waits for that copy and returns the buffer and
+ * the image size it was taken at (h, w, 4 values of uint8 sRGB or float32
+ * linear), valid until the next fh_tracer_output_async of the same output or the
+ * tracer's destruction; fh_tracer_output_ready tells whether it has completed.
+ * A render of the tracer started meanwhile runs its traversal concurrently with
+ * the copy and writes the output only after it, so a progressive client can
+ * read frame k back while frame k + 1 is traced.  Not while a served request of
+ * the tracer is active. */
+int fh_tracer_output_async(fh_tracer tr, int which);
+int fh_tracer_output_ready(fh_tracer tr, int which, int* ready);
+int fh_tracer_output_wait(fh_tracer tr, int which, void** data, uint32_t* w, uint32_t* h);
 /* The same host staging buffer without copying (no reference counterpart):
  * what the last fh_tracer_map_output left there, for the buffer protocol of an
  * array that was mapped before it is read. */

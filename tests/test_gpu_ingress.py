@@ -142,6 +142,79 @@ def test_row_updates_equal_fresh_scene(kind, refit, gpu_device):
         assert numpy.array_equal(img, _render(gpu_device, s2)), f"{kind} after {which}[{index}]"
 
 
+@pytest.mark.parametrize('kind', sorted(EDITS))
+def test_row_updates_match_checker(kind, gpu_device, cpu_device):
+    """After every row edit of EDITS, the edited GPU scene's primary hits (renderer kernel) and linear image
+    equal, bit for bit, those of the CPU checker's scene built from the edited arrays."""
+    import ctypes
+    import oracle
+    from oracle import ptr
+    n = 120 if kind != 'mesh' else 16
+    rng = numpy.random.default_rng(7 + sum(map(ord, kind)))
+    d = _data(kind, rng, n)
+    s, g = _build(gpu_device, kind, d, n)
+    s.set_bvh_refit(True, limit=100.0)
+    s.camera = fresnel.camera.Orthographic.fit(s, margin=0.2)
+    _render(gpu_device, s)
+    for which, index in EDITS[kind]:
+        arr = d[which]
+        if isinstance(index, str):
+            index = rng.random(arr.shape[0]) < 0.1
+        new = arr.copy()
+        if which == 'color':
+            new[index] = rng.uniform(0, 1, new[index].shape)
+        else:
+            new[index] = new[index] + rng.normal(0, 0.3, new[index].shape).astype(numpy.float32)
+        if which == 'orientation':
+            # unit quaternions: a non-unit one scales the shape beyond its bounds (position +- circumradius,
+            # as in the reference), and the closest hit then depends on the hierarchy
+            q = numpy.atleast_2d(new[index])
+            new[index] = (q / numpy.linalg.norm(q, axis=1, keepdims=True)).reshape(new[index].shape)
+        d[which] = new
+        getattr(g, which)[index] = new[index]
+        img = _render(gpu_device, s)
+        cs, _ = _build(cpu_device, kind, d, n)
+        cs.camera = s.camera
+        assert numpy.array_equal(img.view(numpy.uint32), _render(cpu_device, cs).view(numpy.uint32)), \
+            f"{kind} after {which}[{index}] differs from the checker"
+        t = fresnel.tracer.Path(gpu_device, w=W, h=H)
+        t.seed = 3
+        gid, prim, tt = t._tracer.debugPrimaryRender(s._scene, 1)
+        rays = numpy.zeros((W * H, 6), numpy.float32)
+        oracle.lib().oracle_camera_rays(cs._scene.handle(), W, H, 3, 1, 1, ptr(rays))
+        rg = numpy.zeros(W * H, numpy.uint32)
+        rp = numpy.zeros(W * H, numpy.uint32)
+        rt = numpy.zeros(W * H, numpy.float32)
+        oracle.lib().oracle_trace(cs._scene.handle(), ptr(rays), W * H, 1e-3, ptr(rg, ctypes.c_uint32),
+                                  ptr(rp, ctypes.c_uint32), ptr(rt), None, 0)
+        assert numpy.array_equal(gid, rg) and numpy.array_equal(prim, rp), f"{kind} hits after {which}"
+        assert numpy.array_equal(tt.view(numpy.uint32), rt.view(numpy.uint32))
+
+
+def test_async_output_equals_map_and_survives_a_render(gpu_device):
+    """output_async returns the image of the last render, equal to map(); a render started before the
+    wait does not change it; both outputs; the size is the one at the copy's start."""
+    rng = numpy.random.default_rng(9)
+    d = _data('sphere', rng, 200)
+    s, _ = _build(gpu_device, 'sphere', d, 200)
+    s.camera = fresnel.camera.Orthographic.fit(s, margin=0.2)
+    t = fresnel.tracer.Path(gpu_device, w=W, h=H)
+    t.sample(s, samples=2)
+    first = (numpy.array(t.output[:]), numpy.array(t.linear_output[:]))
+    a = t.output_async()
+    b = t.output_async(linear=True)
+    t.sample(s, samples=3, reset=False)  # renders while the copies may still run
+    assert numpy.array_equal(a.wait(), first[0])
+    assert numpy.array_equal(b.wait(), first[1])
+    assert a.ready() and b.ready()
+    second = numpy.array(t.linear_output[:])
+    assert not numpy.array_equal(second, first[1])
+    c = t.output_async(linear=True)
+    t.resize(40, 30)
+    got = c.wait()
+    assert got.shape == (H, W, 4) and numpy.array_equal(got, second)
+
+
 def test_update_rows_rejects_rows_outside_the_buffer(gpu_device):
     s, g = _build(gpu_device, 'sphere', _data('sphere', numpy.random.default_rng(0), 10), 10)
     buf = g._geometry.getPositionBuffer()

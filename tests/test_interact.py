@@ -110,6 +110,43 @@ def test_progressive_view_on_the_gpu(gpu_device):
     assert n < 300
 
 
+@pytest.mark.gpu
+def test_progressive_view_on_the_gpu_equals_checker(gpu_device, cpu_device):
+    """The GPU's refined image equals, bit for bit in linear HDR, the checker's Path.sample with the same
+    seed and sample count."""
+    s = _scene(gpu_device)
+    view = interact.ProgressiveView(s, max_samples=40, frame_time=10.0)
+    view.resize(96, 64)
+    while view.rendering:
+        view.repaint()
+    got = numpy.array(view.tracer.linear_output[:])
+    cs = _scene(cpu_device)
+    t = fresnel.tracer.Path(cpu_device, w=96, h=64)
+    t.seed = view.tracer.seed - 1
+    t.sample(cs, samples=40)
+    assert numpy.array_equal(got.view(numpy.uint32), numpy.array(t.linear_output[:]).view(numpy.uint32))
+
+
+@pytest.mark.gpu
+def test_progressive_view_async_readback(gpu_device):
+    """repaint_async returns the image the previous call read back: the sequence equals repaint()'s, one
+    call behind."""
+    def run(method):
+        s = _scene(gpu_device)
+        view = interact.ProgressiveView(s, max_samples=30, frame_time=1e-9)  # one sample per repaint
+        view.resize(64, 48)
+        out = []
+        for _ in range(6):
+            out.append(numpy.array(getattr(view, method)()[:]))
+        return out
+
+    sync = run('repaint')
+    asy = run('repaint_async')
+    assert numpy.array_equal(asy[0], sync[0])
+    for k in range(1, 6):
+        assert numpy.array_equal(asy[k], sync[k - 1]), k
+
+
 def test_scene_view_needs_qt():
     if interact.QtWidgets is not None:
         pytest.skip("a Qt binding is installed")
