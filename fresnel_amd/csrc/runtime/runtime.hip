@@ -2035,7 +2035,8 @@ struct RequestObj
     uint32_t scheduled = 0; // samples planned into batches (done <= scheduled <= samples)
     double t_arrival = 0, t_admit = -1, t_done = -1;
     uint64_t footprint = 0;
-    int state = 0; // 0 queued, 1 active, 2 done, 3 rejected
+    int state = 0;      // 0 queued, 1 active, 2 done, 3 rejected, 4 failed
+    bool built = false; // scene synchronized (uploads, hierarchy) since admission
     };
 
 // A batch of the server: its requests and the samples it adds.  Planned
@@ -2077,6 +2078,7 @@ struct ServerObj
     // render scratches, up to SERVER_DEPTH per stream; accumulations chained
     // by events
     RenderScratch rs[2];
+    bool presized[2] = {false, false}; // rs[k] sized for target_paths
     hipStream_t st[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> free_events;
     std::vector<hipEvent_t> all_events;
@@ -2131,6 +2133,25 @@ static void release_scratch(RenderScratch& R)
     R.tail_stack.release();
     }
 
+// Size a render scratch once for batches of up to `paths` light-sample paths
+// (queues with the segment rounding of segment_capacity, hits, records), so
+// batches of varying size up to that never regrow it (regrowth frees and
+// reallocates tens of GB and waits for the stream).  Its stream must be idle.
+static void presize_scratch(const DeviceObj& D, RenderScratch& R, uint64_t paths)
+    {
+    const uint64_t slack = (uint64_t)QSEG * std::max<uint64_t>(D.primary_chunk, SHADE_BLOCK) * 64;
+    const uint64_t n = paths + slack;
+    for (int q = 0; q < 2; q++)
+        {
+        R.qo[q].ensure(n);
+        R.qd[q].ensure(n);
+        R.qatt[q].ensure(n);
+        R.qmeta[q].ensure(n);
+        }
+    R.hits.ensure(n);
+    R.records.ensure(n);
+    }
+
 // Largest light-sample path count of one batch (the queue indices are 31-bit).
 constexpr uint64_t MAX_BATCH_PATHS = (1ull << 31) - 1;
 
@@ -2146,7 +2167,7 @@ static void server_plan(ServerObj& S)
     for (uint64_t id : S.active)
         {
         const RequestObj& r = S.reqs[id];
-        if (r.scheduled < r.samples)
+        if (r.built && r.scheduled < r.samples)
             {
             items.push_back(PolicyItem {id, r.footprint, (uint64_t)r.tracer->W * r.tracer->H * r.L,
                                         r.samples - r.scheduled});
@@ -2315,16 +2336,36 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         r.state = REQ_ACTIVE;
         r.t_admit = S.clock(now);
         S.active.push_back(it.id);
+        }
+    // Scenes of admitted requests are built (uploads, hierarchy) a few per
+    // step: with batches running, for about 2 ms of host time, so the device
+    // keeps tracing earlier requests meanwhile; with none running, only until
+    // one request can be planned.
+    {
+    bool running = false;
+    for (auto& b : S.batches)
+        running = running || b.stream >= 0;
+    const auto t_build = std::chrono::steady_clock::now();
+    for (uint64_t id : std::vector<uint64_t>(S.active))
+        {
+        RequestObj& r = S.reqs[id];
+        if (r.built)
+            continue;
         try
             {
-            sync_scene(*r.scene); // BVH build and uploads happen at admission
+            sync_scene(*r.scene);
             }
         catch (...)
             {
-            server_fail(S, it.id, now);
+            server_fail(S, id, now);
             throw;
             }
+        r.built = true;
+        const double spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build).count();
+        if (!running || spent > 2e-3)
+            break;
         }
+    }
     if (!S.st[0])
         for (int k = 0; k < 2; k++)
             HIPCHECK(hipStreamCreateWithFlags(&S.st[k], hipStreamNonBlocking));
@@ -2374,6 +2415,12 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
                 S.all_events.push_back(ev);
                 S.free_events.push_back(ev);
                 }
+            if (!S.presized[k])
+                {
+                HIPCHECK(hipStreamSynchronize(S.st[k]));
+                presize_scratch(D, S.rs[k], S.target_paths);
+                S.presized[k] = true;
+                }
             b.stage.reset(new PinnedBuf());
             b.stage->ensure(stage_bytes(b.jobs.size()));
             b.e.reset(new BatchExec(D, S.rs[k], S.st[k], b.jobs, b.types, b.stage->p));
@@ -2401,9 +2448,13 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         started = true;
         }
     // wait for the oldest started batch rather than let the caller spin: when
-    // nothing could start and both streams are full, when destroying, or when
-    // nothing is left to plan or start
-    bool wait = !started && queued[0] >= SERVER_DEPTH && queued[1] >= SERVER_DEPTH;
+    // nothing could start and both streams are full (and no scene is left to
+    // build meanwhile), when destroying, or when nothing is left to plan or
+    // start
+    bool unbuilt = false;
+    for (uint64_t id : S.active)
+        unbuilt = unbuilt || !S.reqs[id].built;
+    bool wait = !started && !unbuilt && queued[0] >= SERVER_DEPTH && queued[1] >= SERVER_DEPTH;
     wait = wait || S.draining;
     if (!wait && S.queue.empty())
         {
