@@ -464,6 +464,10 @@ def run_plain(args):
                           "valu_frac": round(path_flops / mean_s / 1e12 / FP32_PEAK_TFLOPS, 4)},
         "tail": {"rays": int(ctr[6]), "nodes_per_ray": round(ctr[7] / max(ctr[6], 1), 2)},
         "primary_tile_entries_per_ray": round(ctr[10] / max(ctr[3], 1), 2),
+        "primary_tile_entries_by_type": {
+            "spherocylinder": round(ctr[11] / max(ctr[3], 1), 2), "polyhedron": round(ctr[12] / max(ctr[3], 1), 2),
+            "mesh_triangle": round(ctr[13] / max(ctr[3], 1), 2),
+            "other": round((ctr[10] - ctr[11] - ctr[12] - ctr[13]) / max(ctr[3], 1), 2)},
         "kernel_ms_last_step": kernel_ms,
         "kernel_launches_last_step": kernel_launches,
         "tuning": tuned,

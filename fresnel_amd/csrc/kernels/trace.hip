@@ -608,6 +608,29 @@ __device__ __forceinline__ uint32_t block_append(bool pred, const Queue& q, uint
     return slot;
     }
 
+// Rank of this thread among the workgroup's threads with `pred` (in thread
+// order) and, in `total`, their number.  Every thread must call it.
+template<int BS>
+__device__ __forceinline__ uint32_t block_rank(bool pred, uint32_t* sh, uint32_t& total)
+    {
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane == 0)
+        sh[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < BS / 64; w++)
+        {
+        const uint32_t c = sh[w];
+        before += (w < wave) ? c : 0u;
+        all += c;
+        }
+    __syncthreads(); // sh is reused by the next call
+    total = all;
+    return before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    }
+
 template<int BS>
 __device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t* sh)
     {
@@ -1015,27 +1038,17 @@ __device__ inline bool shade_bounce(const JobDev& J, const HitT& h, v3 o, v3 d, 
     return cont;
     }
 
-// Shade queued ray i (queue slot sl) of job J: returns whether its path
-// continues with (no, nd, att); a terminated path's record is written.
+// Shade the queued hit h of job J (ray o, d from tnear, entry meta mt, path
+// attenuation att): returns whether its path continues with (no, nd, att); a
+// terminated path's record is written.
 template<uint32_t TYPES>
-__device__ __forceinline__ bool shade_queued(const JobDev& J, const Queue& in, const uint4* __restrict__ hits,
-                                             uint32_t i, uint32_t sl, const uint4& mt, uint4* records,
-                                             uint32_t max_depth, unsigned long long* counters, v3& att, v3& no,
-                                             v3& nd)
+__device__ __forceinline__ bool shade_queued(const JobDev& J, const HitT& h, v3 o, v3 d, float tnear, const uint4& mt,
+                                             uint4* records, uint32_t max_depth, unsigned long long* counters,
+                                             v3& att, v3& no, v3& nd)
     {
-    const uint4 hh = hits[i];
-    HitT h;
-    h.t = __uint_as_float(hh.x);
-    h.geom = hh.y;
-    h.prim = hh.z;
-    const float4 a = in.o[sl];
-    const float4 b = in.d[sl];
-    const float4 at4 = in.att[sl];
-    att = mk3(at4.x, at4.y, at4.z);
     const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
     uint4 rec;
-    bool cont = shade_bounce<TYPES>(J, h, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, mt.y, k, ls, mt.w, att, no, nd,
-                                    rec);
+    bool cont = shade_bounce<TYPES>(J, h, o, d, tnear, mt.y, k, ls, mt.w, att, no, nd, rec);
     if (cont && mt.w + 1 >= max_depth)
         {
         // depth cap (the reference has none): stop without contribution
@@ -1048,10 +1061,97 @@ __device__ __forceinline__ bool shade_queued(const JobDev& J, const Queue& in, c
     return cont;
     }
 
-// A wavefront whose rays all belong to one job (every wavefront of a
-// single-job batch, most of a multi-job one, whose queues are filled chunk by
-// chunk) reads the job record through scalar loads; otherwise each ray reads
-// its own job's record.
+// Hit list of one shading unit: queue index, slot, geometry, primitive and
+// distance of each ray that hit, in queue order.
+template<int BS>
+struct ShadeLds
+    {
+    uint4 ent[BS];
+    float t[BS];
+    uint32_t sh[BS / 64 + 1];
+    };
+
+// Shade the BS queued rays of `unit`.  A ray that missed ends on its own
+// thread with its lights record; the rays that hit are listed in LDS in queue
+// order and shaded by the workgroup's first threads, so a wavefront's lanes
+// all run the surface interaction (on a lattice of spheres most bounce rays
+// escape, and a wavefront of mixed rays would run it with few lanes).  The
+// continuing paths are appended to segment unit % QSEG in queue order.  Each
+// path's arithmetic depends only on its own entry, so the result is the same
+// as shading every ray on its own thread.  A wavefront whose entries all
+// belong to one job (every one of a single-job batch, most of a multi-job
+// one, whose queues are filled chunk by chunk) reads the job record through
+// scalar loads; otherwise each ray reads its own job's record.  Every thread
+// of the workgroup must call it.
+template<int BS, uint32_t TYPES>
+__device__ __forceinline__ void shade_unit(const JobDev* __restrict__ jobs, uint32_t n_jobs, const Queue& in,
+                                           uint32_t n, uint32_t unit, const uint4* __restrict__ hits,
+                                           const Queue& out, uint4* records, uint32_t max_depth,
+                                           unsigned long long* counters, ShadeLds<BS>& L)
+    {
+    const uint32_t first = unit * BS;
+    const uint32_t i = first + threadIdx.x;
+    const uint32_t sl = q_slot(in, i, first, min(n, first + BS) - 1u);
+    bool hit = false;
+    uint4 hh = make_uint4(0, 0, 0, 0);
+    if (i < n)
+        {
+        hh = hits[i];
+        hit = hh.y != EMPTY;
+        if (!hit)
+            {
+            // miss: lights whose cone contains the direction (TracerPathMethods.h:50-73)
+            const uint4 mt = in.meta[sl];
+            const float4 b = in.d[sl];
+            const float4 a4 = in.att[sl];
+            const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
+            const JobDev& J = (n_jobs == 1 || __ballot(mt.x != j0) == 0) ? jobs[j0] : jobs[mt.x];
+            const uint32_t k = mt.z & 0xffffu, ls = mt.z >> 16;
+            records[rec_index(J, k, raster_ordinal(J, mt.y), ls)] =
+                make_rec(mk3(a4.x, a4.y, a4.z), REC_LIGHTS | (light_mask(J.lights, mk3(b.x, b.y, b.z)) << 2));
+            }
+        }
+    uint32_t n_hit;
+    const uint32_t pos = block_rank<BS>(hit, L.sh, n_hit);
+    if (hit)
+        {
+        L.ent[pos] = make_uint4(i, sl, hh.y, hh.z);
+        L.t[pos] = __uint_as_float(hh.x);
+        }
+    __syncthreads();
+    bool cont = false;
+    v3 att, no, nd;
+    uint4 mt = make_uint4(0, 0, 0, 0);
+    if (threadIdx.x < n_hit)
+        {
+        const uint4 e = L.ent[threadIdx.x];
+        HitT h;
+        h.t = L.t[threadIdx.x];
+        h.geom = e.z;
+        h.prim = e.w;
+        const uint32_t hs = e.y;
+        mt = in.meta[hs];
+        const float4 a = in.o[hs];
+        const float4 b = in.d[hs];
+        const float4 at4 = in.att[hs];
+        att = mk3(at4.x, at4.y, at4.z);
+        const v3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z);
+        const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
+        if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
+            cont = shade_queued<TYPES>(jobs[j0], h, o, d, a.w, mt, records, max_depth, counters, att, no, nd);
+        else
+            cont = shade_queued<TYPES>(jobs[mt.x], h, o, d, a.w, mt, records, max_depth, counters, att, no, nd);
+        }
+    const uint32_t slot = block_append<BS>(cont, out, unit % QSEG, L.sh);
+    if (cont && slot != EMPTY)
+        {
+        out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
+        out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
+        out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
+        out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
+        }
+    }
+
 // Units of BS rays from unit0 + blockIdx.x; LOOP: grid-stride over every
 // further unit, else one unit per workgroup (the host sizes the grid from the
 // live-path fraction recent renders had at this bounce and launches the LOOP
@@ -1067,36 +1167,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(LOOP ? 7 : 1
     {
     if (!take_view(view, VIEW_BOUNCE, in, n))
         return;
-    __shared__ uint32_t sh[BS / 64 + 1];
+    __shared__ ShadeLds<BS> lds;
     const uint32_t units = (n + BS - 1) / BS;
     for (uint32_t unit = unit0 + blockIdx.x; unit < units; unit += gridDim.x)
         {
-        const uint32_t first = unit * BS;
-        const uint32_t i = first + threadIdx.x;
-        const uint32_t sl = q_slot(in, i, first, min(n, first + BS) - 1u);
-        bool cont = false;
-        v3 att, no, nd;
-        uint4 mt = make_uint4(0, 0, 0, 0);
-        if (i < n)
-            {
-            mt = in.meta[sl];
-            const uint32_t j0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mt.x);
-            if (n_jobs == 1 || __ballot(mt.x != j0) == 0)
-                cont = shade_queued<TYPES>(jobs[j0], in, hits, i, sl, mt, records, max_depth, counters, att, no, nd);
-            else
-                cont = shade_queued<TYPES>(jobs[mt.x], in, hits, i, sl, mt, records, max_depth, counters, att, no,
-                                           nd);
-            }
-        const uint32_t slot = block_append<BS>(cont, out, unit % QSEG, sh);
-        if (cont && slot != EMPTY)
-            {
-            out.o[slot] = make_float4(no.x, no.y, no.z, 1e-3f);
-            out.d[slot] = make_float4(nd.x, nd.y, nd.z, __uint_as_float(mt.x));
-            out.att[slot] = make_float4(att.x, att.y, att.z, 0.0f);
-            out.meta[slot] = make_uint4(mt.x, mt.y, mt.z, mt.w + 1);
-            }
+        shade_unit<BS, TYPES>(jobs, n_jobs, in, n, unit, hits, out, records, max_depth, counters, lds);
         if (!LOOP)
             break;
+        __syncthreads(); // the next unit rewrites the hit list
         }
     }
 
@@ -1133,6 +1211,13 @@ __device__ __forceinline__ bool leaf_test_uniform(const GeomDev* geoms, uint32_t
         return leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
     }
 
+// Tile-list entries examined (counting kernels only): all, and those of
+// spherocylinders, convex polyhedra and mesh triangles.
+struct EntryCount
+    {
+    uint32_t all = 0, cyl = 0, poly = 0, mesh = 0;
+    };
+
 // Closest hits of one wavefront's 64 primary rays (lane ray o, d of job `job`
 // at pixel px, py; `valid` lanes only): tile lists when all 64 rays form one
 // full tile of a job that has them, wavefront-packet BVH search when they
@@ -1140,7 +1225,7 @@ __device__ __forceinline__ bool leaf_test_uniform(const GeomDev* geoms, uint32_t
 template<int SS, bool COUNT, uint32_t TYPES>
 __device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ jobs, bool valid, uint32_t job,
                                                   uint32_t px, uint32_t py, v3 o, v3 d, LdsStack<SS>& st, HitT& best,
-                                                  uint32_t& n_nodes, uint32_t& n_prims, uint32_t& n_entries,
+                                                  uint32_t& n_nodes, uint32_t& n_prims, EntryCount& n_entries,
                                                   uint32_t& err)
     {
     const uint32_t job0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)job);
@@ -1191,7 +1276,16 @@ __device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ job
             if (__ballot(want) == 0)
                 break;
             if (COUNT)
-                n_entries++;
+                {
+                n_entries.all++;
+                if constexpr (TYPES != T_SPHERES)
+                    {
+                    const uint32_t type = J.scene.geoms[__float_as_uint(J.prim_lo[cur[7]].w)].type;
+                    n_entries.cyl += type == G_CYLINDER;
+                    n_entries.poly += type == G_POLYHEDRON;
+                    n_entries.mesh += type == G_MESH;
+                    }
+                }
             if (want)
                 {
                 float lx, ly, lz, hx, hy, hz;
@@ -1383,7 +1477,8 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
     st.gstride = gstride;
     st.sp = 0;
     const uint32_t lane = lane_id();
-    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0, n_entries = 0;
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    EntryCount n_entries;
     uint32_t claim = 0;
     if (lane == 0)
         claim = atomicAdd(work, chunk);
@@ -1422,7 +1517,10 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
         atomicAdd(counters + CTR_PRIMARY + 0, (unsigned long long)n_rays);
         atomicAdd(counters + CTR_PRIMARY + 1, (unsigned long long)n_nodes);
         atomicAdd(counters + CTR_PRIMARY + 2, (unsigned long long)n_prims);
-        atomicAdd(counters + CTR_TILE_ENTRIES, (unsigned long long)n_entries);
+        atomicAdd(counters + CTR_TILE_ENTRIES, (unsigned long long)n_entries.all);
+        atomicAdd(counters + CTR_TILE_ENTRIES + 1, (unsigned long long)n_entries.cyl);
+        atomicAdd(counters + CTR_TILE_ENTRIES + 2, (unsigned long long)n_entries.poly);
+        atomicAdd(counters + CTR_TILE_ENTRIES + 3, (unsigned long long)n_entries.mesh);
         }
     if (err)
         atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);
@@ -1454,7 +1552,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t chunk = BLOCK * groups;
-    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0, n_entries = 0;
+    uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
+    EntryCount n_entries;
     // Chunk u of n_units goes to queue segment u % QSEG, or with bands to
     // segment u / G (16 contiguous bands of G chunks, i.e. of the image); the
     // claims then visit the bands in turn, claim c -> chunk (c % QSEG) G + c /
@@ -1533,7 +1632,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         atomicAdd(counters + CTR_PRIMARY + 0, (unsigned long long)n_rays);
         atomicAdd(counters + CTR_PRIMARY + 1, (unsigned long long)n_nodes);
         atomicAdd(counters + CTR_PRIMARY + 2, (unsigned long long)n_prims);
-        atomicAdd(counters + CTR_TILE_ENTRIES, (unsigned long long)n_entries);
+        atomicAdd(counters + CTR_TILE_ENTRIES, (unsigned long long)n_entries.all);
+        atomicAdd(counters + CTR_TILE_ENTRIES + 1, (unsigned long long)n_entries.cyl);
+        atomicAdd(counters + CTR_TILE_ENTRIES + 2, (unsigned long long)n_entries.poly);
+        atomicAdd(counters + CTR_TILE_ENTRIES + 3, (unsigned long long)n_entries.mesh);
         }
     if (err)
         atomicOr((unsigned int*)(counters + CTR_OVERFLOW), 1u);

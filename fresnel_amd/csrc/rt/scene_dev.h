@@ -63,7 +63,8 @@ struct GeomDev
 // Device work-counter slots (64-bit each): rays traced, BVH nodes visited
 // and primitive tests of the primary, secondary and tail traversals; paths
 // stopped at the depth cap; tile-list entries examined by primary rays (per
-// ray, each entry a wavefront reads counts once per lane); ray-queue segment
+// ray, each entry a wavefront reads counts once per lane), then those of
+// spherocylinders, convex polyhedra and mesh triangles; ray-queue segment
 // overflow flag; traversal-stack overflow flag.
 constexpr int CTR_SECONDARY = 0, CTR_PRIMARY = 3, CTR_TAIL = 6, CTR_CAPPED = 9, CTR_TILE_ENTRIES = 10,
               CTR_QUEUE_OVERFLOW = 14, CTR_OVERFLOW = 15, N_CTR = 16;
