@@ -18,6 +18,9 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
 * ``trav_packet`` primary rays traced as wavefront packets (2) or per ray (1).
 * ``primary_tiles`` primary rays tested against per-tile candidate lists (2)
                   or searched in the BVH (1).
+* ``tile_group`` largest tile count of a hierarchy subtree listed as one
+                  tile-list entry (scenes other than spheres-only; 1: one
+                  entry per primitive).
 * ``fuse_primary`` primary traversal and depth-0 shading in one kernel (2) or
                   two (1).
 * ``fuse_bounce`` the same for each bounce of the secondary rays.
@@ -43,6 +46,7 @@ CANDIDATES = {
     'trav_refill': [16, 32, 48, 65],
     'trav_packet': [1, 2],
     'primary_tiles': [1, 2],
+    'tile_group': [1, 4, 9, 16],
     'fuse_primary': [1, 2],
     'fuse_bounce': [1, 2],
     'inflight': [1, 2],

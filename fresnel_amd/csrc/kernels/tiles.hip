@@ -242,6 +242,143 @@ __global__ void k_tile_emit(const float4* __restrict__ lo, const float4* __restr
             }
     }
 
+// ---------------------------------------------------------------- hierarchy cut
+// In scenes other than spheres-only a tile list may hold subtrees of the
+// binary hierarchy in place of their primitives.  The cut is the set of nodes
+// whose tile rectangle covers at most `group` tiles while their parent's
+// covers more (or the parent's box cuts the camera plane), plus the leaf
+// primitives of such parents; the root is an element when it is small itself.
+// A wavefront that reaches a subtree entry tests its box once per lane and
+// searches the subtree only in the lanes whose ray passes it, instead of
+// walking every primitive of a compact cluster (a mesh instance's triangles)
+// as entries of their own.  The subtree search applies the slab and cull
+// tests of the BVH search, and every box contains its primitives' boxes, so
+// each ray tests the same primitives that could win and its closest hit is
+// unchanged.
+// Box of binary node n: the union of its children's boxes (an EMPTY child's
+// NaN box drops out of fminf/fmaxf).  Every thread derives a node's box this
+// way, so a parent and the node itself classify it alike.
+__device__ __forceinline__ void node_box(const Node* __restrict__ nodes, uint32_t n, float4& lo, float4& hi)
+    {
+    const Node& N = nodes[n];
+    lo = make_float4(fminf(N.a_lo.x, N.a_hi.z), fminf(N.a_lo.y, N.a_hi.w), fminf(N.a_lo.z, N.b_box.x), 0.0f);
+    hi = make_float4(fmaxf(N.a_lo.w, N.b_box.y), fmaxf(N.a_hi.x, N.b_box.z), fmaxf(N.a_hi.y, N.b_box.w), 0.0f);
+    }
+
+// Whether a subtree box is split further: its rectangle covers more than
+// `group` tiles or the camera plane cuts it.  A box no full tile sees is not
+// split (nothing below it is seen either).
+__device__ __forceinline__ bool box_large(const TileCam& c, float4 lo, float4 hi, uint32_t group)
+    {
+    int tx0, tx1, ty0, ty1;
+    float q;
+    bool straddle = false;
+    if (!tile_rect(c, lo, hi, tx0, tx1, ty0, ty1, q, straddle))
+        return straddle;
+    return (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1)) > group;
+    }
+
+// Bounds index of leaf primitive `prim` of geometry `slot` (prefix[k] is the
+// first bounds index of active slot slots[k]).
+__device__ __forceinline__ uint32_t leaf_bounds_index(const uint32_t* __restrict__ prefix,
+                                                      const uint32_t* __restrict__ slots, uint32_t n_slots,
+                                                      uint32_t slot, uint32_t prim)
+    {
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < n_slots; k++)
+        base = (slots[k] == slot) ? prefix[k] : base;
+    return base + prim;
+    }
+
+// Entries of one cut element: the tiles of its rectangle, its bound, its
+// reference.  Without EMIT, cnt[idx] = their number; with EMIT, they are
+// written at cnt[idx] (the exclusive scan of the counts).
+template<bool EMIT>
+__device__ __forceinline__ void cut_entries(const TileCam& c, bool ok, int tx0, int tx1, int ty0, int ty1, float q,
+                                            uint32_t refv, uint32_t* cnt, uint32_t idx, uint32_t* tile,
+                                            uint32_t* qbits, uint32_t* ref, uint32_t* order)
+    {
+    if (!EMIT)
+        {
+        cnt[idx] = ok ? (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1)) : 0u;
+        return;
+        }
+    if (!ok)
+        return;
+    uint32_t e = cnt[idx];
+    const uint32_t e_end = cnt[idx + 1]; // the count pass saw the same rectangle
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1 && e < e_end; tx++, e++)
+            {
+            tile[e] = (uint32_t)ty * c.tiles_x + (uint32_t)tx;
+            qbits[e] = __float_as_uint(q);
+            ref[e] = refv;
+            order[e] = e;
+            }
+    }
+
+// Cut elements of the binary hierarchy (n_inner inner nodes, root 0): thread
+// n < n_inner handles the two children of node n when n is split, thread
+// n_inner the root; element slots 2 n + c and 2 n_inner, plus a zero slot
+// 2 n_inner + 1 for the scan's total.  A leaf primitive whose box the camera
+// plane cuts raises *flags, as in k_tile_count.
+template<bool EMIT>
+__global__ void k_tile_cut(const Node* __restrict__ nodes, uint32_t n_inner, const float4* __restrict__ lo,
+                           const float4* __restrict__ hi, const uint32_t* __restrict__ prefix,
+                           const uint32_t* __restrict__ slots, uint32_t n_slots, TileCam c, uint32_t group,
+                           uint32_t* cnt, uint32_t* flags, uint32_t* tile, uint32_t* qbits, uint32_t* ref,
+                           uint32_t* order)
+    {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n > n_inner)
+        return;
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    float q = 0.0f;
+    if (n == n_inner)
+        {
+        float4 a, b;
+        node_box(nodes, 0, a, b);
+        bool straddle = false;
+        const bool ok = tile_rect(c, a, b, tx0, tx1, ty0, ty1, q, straddle)
+                        && (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1)) <= group;
+        cut_entries<EMIT>(c, ok, tx0, tx1, ty0, ty1, q, NODE_REF | 0u, cnt, 2 * n_inner, tile, qbits, ref, order);
+        if (!EMIT)
+            cnt[2 * n_inner + 1] = 0;
+        return;
+        }
+    float4 a, b;
+    node_box(nodes, n, a, b);
+    const bool split = box_large(c, a, b, group);
+    const uint4 lk = nodes[n].link;
+    for (uint32_t k = 0; k < 2; k++)
+        {
+        const uint32_t child = k ? lk.y : lk.x, slot = k ? lk.w : lk.z;
+        bool ok = false;
+        uint32_t refv = 0;
+        if (split && child != EMPTY)
+            {
+            bool straddle = false;
+            if (child & LEAF)
+                {
+                refv = leaf_bounds_index(prefix, slots, n_slots, slot, child & ~LEAF);
+                const float4 pa = lo[refv], pb = hi[refv];
+                ok = tile_rect(c, pa, pb, tx0, tx1, ty0, ty1, q, straddle);
+                if (straddle && !EMIT)
+                    atomicOr(flags, 1u);
+                }
+            else
+                {
+                float4 ca, cb;
+                node_box(nodes, child, ca, cb);
+                refv = NODE_REF | child;
+                ok = tile_rect(c, ca, cb, tx0, tx1, ty0, ty1, q, straddle)
+                     && (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1)) <= group;
+                }
+            }
+        cut_entries<EMIT>(c, ok, tx0, tx1, ty0, ty1, q, refv, cnt, 2 * n + k, tile, qbits, ref, order);
+        }
+    }
+
 __global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t n,
                              uint32_t* dst)
     {
@@ -259,22 +396,23 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v)
 
 // Sorted entries packed as 8 words each, so a wavefront reads an entry with
 // one scalar load: [0] bound bits, then for spheres-only scenes the sphere
-// (x, y, z, r), geometry slot and primitive, otherwise the primitive's box
-// (lo.xyz, hi.xyz) and its index into the bounds arrays.
+// (x, y, z, r), geometry slot and primitive, otherwise the box (lo.xyz,
+// hi.xyz) and the reference: a primitive's index into the bounds arrays, or
+// NODE_REF | the root of a subtree (`nodes`: the binary hierarchy).
 template<bool SPHERES>
 __global__ void k_tile_pack(const uint32_t* __restrict__ order, const uint32_t* __restrict__ ref0,
                             const uint32_t* __restrict__ q0, const float4* __restrict__ lo,
-                            const float4* __restrict__ hi, const GeomDev* __restrict__ geoms, uint32_t n,
-                            uint4* __restrict__ out)
+                            const float4* __restrict__ hi, const GeomDev* __restrict__ geoms,
+                            const Node* __restrict__ nodes, uint32_t n, uint4* __restrict__ out)
     {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n)
         return;
     const uint32_t e = order[j];
     const uint32_t r = ref0[e];
-    const float4 a = lo[r], b = hi[r];
     if (SPHERES)
         {
+        const float4 a = lo[r], b = hi[r];
         const uint32_t slot = __float_as_uint(a.w), prim = __float_as_uint(b.w);
         const float4 sp = geoms[slot].a[prim];
         out[2 * j] = make_uint4(q0[e], __float_as_uint(sp.x), __float_as_uint(sp.y), __float_as_uint(sp.z));
@@ -282,6 +420,14 @@ __global__ void k_tile_pack(const uint32_t* __restrict__ order, const uint32_t* 
         }
     else
         {
+        float4 a, b;
+        if (r & NODE_REF)
+            node_box(nodes, r & ~NODE_REF, a, b);
+        else
+            {
+            a = lo[r];
+            b = hi[r];
+            }
         out[2 * j] = make_uint4(q0[e], __float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(a.z));
         out[2 * j + 1] = make_uint4(__float_as_uint(b.x), __float_as_uint(b.y), __float_as_uint(b.z), r);
         }

@@ -1279,12 +1279,13 @@ __device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ job
                 {
                 n_entries.all++;
                 if constexpr (TYPES != T_SPHERES)
-                    {
-                    const uint32_t type = J.scene.geoms[__float_as_uint(J.prim_lo[cur[7]].w)].type;
-                    n_entries.cyl += type == G_CYLINDER;
-                    n_entries.poly += type == G_POLYHEDRON;
-                    n_entries.mesh += type == G_MESH;
-                    }
+                    if (!(cur[7] & NODE_REF))
+                        {
+                        const uint32_t type = J.scene.geoms[__float_as_uint(J.prim_lo[cur[7]].w)].type;
+                        n_entries.cyl += type == G_CYLINDER;
+                        n_entries.poly += type == G_POLYHEDRON;
+                        n_entries.mesh += type == G_MESH;
+                        }
                 }
             if (want)
                 {
@@ -1322,20 +1323,49 @@ __device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ job
                     {
                     float t;
                     bool hit;
-                    if (COUNT)
-                        n_prims++;
                     if constexpr (TYPES == T_SPHERES)
                         {
                         float de;
                         bool first;
+                        if (COUNT)
+                            n_prims++;
                         hit = sphere_t(o, d, mk3(sph.x, sph.y, sph.z), sph.w, t, de, first) && (tnear < t);
                         }
                     else
                         {
                         const uint32_t r = cur[7];
-                        slot = __float_as_uint(J.prim_lo[r].w);
-                        prim = __float_as_uint(J.prim_hi[r].w);
-                        hit = leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
+                        hit = false;
+                        if (r & NODE_REF)
+                            {
+                            // a subtree: the BVH search below its root, from this
+                            // lane's closest hit so far (same slab and cull tests)
+                            if (!(cull_bound(tmin, tmax, J.scene.cull_margin) > best.t))
+                                {
+                                TraceState ts;
+                                ts.o = o;
+                                ts.d = d;
+                                ts.inv = inv;
+                                ts.tnear = tnear;
+                                ts.margin = J.scene.cull_margin;
+                                ts.best = best;
+                                ts.node = r & ~NODE_REF;
+                                ts.sph = false;
+                                st.sp = 0;
+                                while (trace_visit2<LdsStack<SS>, COUNT, TYPES>(J.scene.nodes, geoms, ts, st, n_nodes,
+                                                                                n_prims, err))
+                                    {
+                                    }
+                                best = ts.best;
+                                }
+                            }
+                        else
+                            {
+                            if (COUNT)
+                                n_prims++;
+                            slot = __float_as_uint(J.prim_lo[r].w);
+                            prim = __float_as_uint(J.prim_hi[r].w);
+                            hit = leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
+                            }
                         }
                     if (hit && better(t, slot, prim, best))
                         {

@@ -71,6 +71,9 @@ constexpr int CTR_SECONDARY = 0, CTR_PRIMARY = 3, CTR_TAIL = 6, CTR_CAPPED = 9, 
 
 constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t EMPTY = 0xffffffffu;
+// Tile-list entry reference of a hierarchy subtree (NODE_REF | its root node);
+// other references are indices into the primitive bounds arrays.
+constexpr uint32_t NODE_REF = 0x80000000u;
 struct __attribute__((aligned(64))) Node
     {
     float4 a_lo;  // child 0: lo.xyz, hi.x

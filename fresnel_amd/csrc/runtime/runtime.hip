@@ -237,6 +237,7 @@ struct DeviceObj
     uint32_t primary_chunk = 1024;      // the same for primary rays
     uint32_t trav_packet = 2;           // primary rays: 2 wavefront packets, 1 per ray
     uint32_t primary_tiles = 2;         // primary rays of pinhole/orthographic cameras: 2 tile lists, 1 BVH
+    uint32_t tile_group = 4;            // tile-list subtrees cover at most this many tiles (1: primitives only)
     uint32_t fuse_primary = 2;          // 2: primary traversal and depth-0 shading in one kernel, 1: two
     uint32_t fuse_bounce = 1;           // 2: secondary traversal and shading in one kernel, 1: two
     uint32_t inflight = 2;              // batches of one Path.sample in flight (1 or 2)
@@ -648,6 +649,7 @@ struct SceneObj
     bool sphere_leaves = false; // nodes4 leaf children hold spheres
     DBuf<uint32_t> idx4; // 4-wide node of each binary inner node (exclusive scan of even-depth flags)
     uint32_t n_prims = 0;
+    uint32_t n_active = 0; // geometries with primitives: entries of prefix / slots
     bool bvh_valid = false;
     std::vector<uint32_t> topo; // (slot, prim count) of the enabled geometries in the last build
     uint64_t builds = 0, refits = 0;
@@ -880,6 +882,7 @@ static void build_bvh(SceneObj& sc, bool refit_only)
         N += n;
         }
     sc.n_prims = N;
+    sc.n_active = (uint32_t)slots.size();
     sc.sphere_leaves = scene_types(sc) == T_SPHERES;
     if (N == 0)
         {
@@ -1179,6 +1182,7 @@ struct TileLists
     CamDev cam {};
     uint32_t W = 0, H = 0;
     uint32_t tiles_x = 0, tiles_y = 0;
+    uint32_t group = 0;        // tile_group the lists were built with
     uint32_t n_entries = 0;
     DBuf<uint32_t> counts, flags, tile0, q0, ref0, keysA, valsA, keysB, valsB, start, end, hist;
     DBuf<uint4> entries; // 2 per entry
@@ -1270,9 +1274,10 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
         return;
         }
     if (T.scene == sc.serial && T.builds == sc.builds && T.refits == sc.refits && T.W == tr.W && T.H == tr.H
-        && std::memcmp(&T.cam, &cam, sizeof(CamDev)) == 0)
+        && T.group == D.tile_group && std::memcmp(&T.cam, &cam, sizeof(CamDev)) == 0)
         return; // cached, usable or known unusable
     T.scene = sc.serial;
+    T.group = D.tile_group;
     T.builds = sc.builds;
     T.refits = sc.refits;
     T.cam = cam;
@@ -1303,22 +1308,32 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
         }
     const bool spheres = scene_types(sc) == T_SPHERES;
     const uint32_t n_tiles = c.tiles_x * c.tiles_y;
-    T.counts.ensure((size_t)N + 1);
+    // elements: primitives, or (other scenes, tile_group >= 2) the hierarchy
+    // cut of kernels/tiles.hip over the N - 1 inner nodes; one more count
+    // slot holds the scan's total
+    const bool cut = !spheres && D.tile_group >= 2 && N >= 2;
+    const uint32_t n_inner = N - 1;
+    const uint32_t n_el = cut ? 2 * n_inner + 2 : N + 1;
+    T.counts.ensure(n_el);
     T.flags.ensure(1);
     uint32_t h[2] = {0, 0};
     D.launch("tiles_count",
              [&]
              {
                  k_fill_u32<<<1, 64, 0, s>>>(T.flags.p, 1, 0u);
-                 if (spheres)
+                 if (cut)
+                     k_tile_cut<false><<<blocks(n_inner + 1, 256), 256, 0, s>>>(
+                         sc.nodes.p, n_inner, sc.blo.p, sc.bhi.p, sc.prefix.p, sc.slots.p, sc.n_active, c,
+                         D.tile_group, T.counts.p, T.flags.p, nullptr, nullptr, nullptr, nullptr);
+                 else if (spheres)
                      k_tile_count<true><<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
                                                                            T.counts.p, T.flags.p);
                  else
                      k_tile_count<false><<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
                                                                             T.counts.p, T.flags.p);
-                 scan_exclusive(D, s, T.counts.p, N + 1, T.scan_tmp);
+                 scan_exclusive(D, s, T.counts.p, n_el, T.scan_tmp);
              });
-    HIPCHECK(hipMemcpyAsync(&h[0], T.counts.p + N, 4, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(&h[0], T.counts.p + (n_el - 1), 4, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(&h[1], T.flags.p, 4, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     if (h[1])
@@ -1334,7 +1349,11 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
         D.launch("tiles_emit",
                  [&]
                  {
-                     if (spheres)
+                     if (cut)
+                         k_tile_cut<true><<<blocks(n_inner + 1, 256), 256, 0, s>>>(
+                             sc.nodes.p, n_inner, sc.blo.p, sc.bhi.p, sc.prefix.p, sc.slots.p, sc.n_active, c,
+                             D.tile_group, T.counts.p, T.flags.p, T.tile0.p, T.q0.p, T.ref0.p, T.valsA.p);
+                     else if (spheres)
                          k_tile_emit<true><<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
                                                                           T.counts.p, T.tile0.p, T.q0.p, T.ref0.p,
                                                                           T.valsA.p);
@@ -1360,10 +1379,10 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
                  {
                      if (spheres)
                          k_tile_pack<true><<<blocks(M, 256), 256, 0, s>>>(vt, T.ref0.p, T.q0.p, sc.blo.p, sc.bhi.p,
-                                                                          sc.dgeoms.p, M, T.entries.p);
+                                                                          sc.dgeoms.p, nullptr, M, T.entries.p);
                      else
                          k_tile_pack<false><<<blocks(M, 256), 256, 0, s>>>(vt, T.ref0.p, T.q0.p, sc.blo.p, sc.bhi.p,
-                                                                           sc.dgeoms.p, M, T.entries.p);
+                                                                           sc.dgeoms.p, sc.nodes.p, M, T.entries.p);
                      k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.start.p, n_tiles, 0u);
                      k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.end.p, n_tiles, 0u);
                      k_tile_bounds<<<blocks(M, 256), 256, 0, s>>>(kt, M, T.start.p, T.end.p);
@@ -3878,6 +3897,15 @@ extern "C"
                         {
                         REQUIRE(value <= 2, "primary_tiles must be 1 (BVH search) or 2 (tile lists)");
                         D.primary_tiles = (uint32_t)value;
+                        }
+                    }
+                else if (k == "tile_group")
+                    {
+                    prev = D.tile_group;
+                    if (value > 0)
+                        {
+                        REQUIRE(value <= 1024, "tile_group must be in [1, 1024]");
+                        D.tile_group = (uint32_t)value;
                         }
                     }
                 else if (k == "trav_packet")

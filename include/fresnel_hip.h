@@ -321,7 +321,11 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * a time (default 65); "trav_packet": primary rays traced as wavefront
  * packets (2, default) or one by one (1); "primary_tiles": primary rays of
  * pinhole-perspective and orthographic cameras tested against per-8x8-tile
- * candidate lists (2, default) or searched in the BVH (1); "fuse_primary":
+ * candidate lists (2, default) or searched in the BVH (1); "tile_group": in
+ * scenes other than spheres-only, those lists hold hierarchy subtrees whose
+ * tile rectangle covers at most this many tiles (a mesh instance's triangles
+ * become one entry, searched only by the rays that pass its box), 2..1024, or
+ * 1 for one entry per primitive (default 4); "fuse_primary":
  * primary traversal and depth-0 shading in one kernel (2, default) or two (1);
  * "fuse_bounce": the same for each bounce of the secondary rays (default 1:
  * the fused kernel holds 88 registers, and the latency-bound secondary

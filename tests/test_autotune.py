@@ -8,7 +8,7 @@ from fresnel_amd import autotune
 class _FakeDev:
     def __init__(self):
         self.knobs = {'stack_lds': 8, 'batch_paths': 1 << 25, 'tail_rays': 32768, 'trav_chunk': 256, 'trav_refill': 65, 'trav_packet': 2, 'primary_tiles': 2,
-                      'primary_chunk': 1024, 'fuse_primary': 2, 'fuse_bounce': 1, 'inflight': 2, 'xcd_bands': 2}
+                      'tile_group': 4, 'primary_chunk': 1024, 'fuse_primary': 2, 'fuse_bounce': 1, 'inflight': 2, 'xcd_bands': 2}
 
     def describe(self):
         return "Enabled HIP devices:\n[0]: AMD Instinct MI355X, 256 CUs\n"
@@ -41,4 +41,4 @@ def test_apply_and_current():
     prev = autotune.apply(dev, {'stack_lds': 16, 'tail_rays': 100})
     assert prev == {'stack_lds': 8, 'tail_rays': 32768}
     assert autotune.current(dev) == {'stack_lds': 16, 'batch_paths': 1 << 25, 'tail_rays': 100, 'trav_chunk': 256, 'trav_refill': 65, 'trav_packet': 2, 'primary_tiles': 2,
-                                      'primary_chunk': 1024, 'fuse_primary': 2, 'fuse_bounce': 1, 'inflight': 2, 'xcd_bands': 2}
+                                      'tile_group': 4, 'primary_chunk': 1024, 'fuse_primary': 2, 'fuse_bounce': 1, 'inflight': 2, 'xcd_bands': 2}

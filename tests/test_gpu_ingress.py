@@ -170,6 +170,11 @@ def test_row_updates_match_checker(kind, gpu_device, cpu_device):
             # as in the reference), and the closest hit then depends on the hierarchy
             q = numpy.atleast_2d(new[index])
             new[index] = (q / numpy.linalg.norm(q, axis=1, keepdims=True)).reshape(new[index].shape)
+        if which == 'radius':
+            # radii >= 0: a negative radius shrinks or inverts the bounds (the reference's A -+ r) while the
+            # intersector, which squares it, reports hits outside them, so the closest hit again depends on
+            # the hierarchy's culling
+            new[index] = numpy.abs(new[index])
         d[which] = new
         getattr(g, which)[index] = new[index]
         img = _render(gpu_device, s)

@@ -189,11 +189,26 @@ def _c3_small(device):
     return bench.c3_scene(device, n_poly=20000, n_cyl=200000, n_mesh=5000, side=120.0 * 0.1**(1 / 3))
 
 
-def test_c3_scaled(gpu_device, cpu_device):
-    """The mixed-geometry scene at a tenth of its counts (same densities): primary hits and image."""
-    gs, cs = _c3_small(gpu_device), _c3_small(cpu_device)
-    primary_parity("c3 20k polyhedra + 200k spherocylinders + 5k meshes 384x216", gs, cs, 384, 216, 3, (1, 2))
-    image_rmse("c3 scaled mix", gs, cs, 384, 216, 2, 1, seed=4)
+@pytest.fixture(scope='module')
+def c3_pair(gpu_device, cpu_device):
+    return _c3_small(gpu_device), _c3_small(cpu_device)
+
+
+@pytest.mark.parametrize('group', [1, 4, 16], ids=['primitive_entries', 'subtrees_4_tiles', 'subtrees_16_tiles'])
+def test_c3_scaled(c3_pair, gpu_device, group):
+    """The mixed-geometry scene at a tenth of its counts (same densities): primary hits and image, with tile
+    lists of primitives (tile_group 1) and of hierarchy subtrees covering at most 4 or 16 tiles."""
+    gs, cs = c3_pair
+    d = gpu_device._device
+    saved = d.tune('tile_group', 0)
+    try:
+        d.tune('tile_group', group)
+        primary_parity(f"c3 20k polyhedra + 200k spherocylinders + 5k meshes 384x216 (tile_group {group})", gs, cs,
+                       384, 216, 3, (1, 2))
+        if group <= 4:
+            image_rmse(f"c3 scaled mix (tile_group {group})", gs, cs, 384, 216, 2, 1, seed=4)
+    finally:
+        d.tune('tile_group', saved)
 
 
 def test_c4_scaled_batched_plain_refit(gpu_device):
