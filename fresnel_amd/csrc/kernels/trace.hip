@@ -423,6 +423,154 @@ __device__ inline void trace_closest(const SceneDev& sc, v3 o, v3 d, float tnear
     best = r.best;
     }
 
+// One node visit of the binary-hierarchy search with the primitive tests
+// postponed: leaf children that pass the slab and cull tests are appended to
+// the lane's pending list (ps, pp; np entries, at most two per visit) instead
+// of being tested; inner children are visited or pushed as in trace_visit2.
+template<class Stack, bool COUNT>
+__device__ __forceinline__ void visit2_collect(const Node* __restrict__ nodes, TraceState& r, Stack& st,
+                                               uint32_t (&ps)[2], uint32_t (&pp)[2], uint32_t& np,
+                                               uint32_t& n_nodes, uint32_t& err)
+    {
+    const GlobalNode* n = (const GlobalNode*)(nodes + r.node);
+    const float4 A = make_float4(n->a_lo.x, n->a_lo.y, n->a_lo.z, n->a_lo.w);
+    const float4 B = make_float4(n->a_hi.x, n->a_hi.y, n->a_hi.z, n->a_hi.w);
+    const float4 C = make_float4(n->b_box.x, n->b_box.y, n->b_box.z, n->b_box.w);
+    const uint4 lk = make_uint4(n->link.x, n->link.y, n->link.z, n->link.w);
+    if (COUNT)
+        n_nodes++;
+    float t0min, t0max, t1min, t1max;
+    slab(r.o, r.inv, A.x, A.y, A.z, A.w, B.x, B.y, t0min, t0max);
+    slab(r.o, r.inv, B.z, B.w, C.x, C.y, C.z, C.w, t1min, t1max);
+    const float c0 = cull_bound(t0min, t0max, r.margin);
+    const float c1 = cull_bound(t1min, t1max, r.margin);
+    bool h0 = (t0min <= t0max) && (t0max >= r.tnear) && !(c0 > r.best.t);
+    bool h1 = (t1min <= t1max) && (t1max >= r.tnear) && !(c1 > r.best.t);
+    if (h0 && (lk.x & LEAF))
+        {
+        ps[0] = lk.z;
+        pp[0] = lk.x & ~LEAF;
+        np = 1;
+        h0 = false;
+        }
+    if (h1 && (lk.y & LEAF))
+        {
+        // constant indices keep the list in registers
+        if (np == 0)
+            {
+            ps[0] = lk.w;
+            pp[0] = lk.y & ~LEAF;
+            }
+        else
+            {
+            ps[1] = lk.w;
+            pp[1] = lk.y & ~LEAF;
+            }
+        np++;
+        h1 = false;
+        }
+    if (h0 && h1)
+        {
+        const bool swap = t1min < t0min;
+        st.push(swap ? lk.x : lk.y, swap ? c0 : c1, err);
+        r.node = swap ? lk.y : lk.x;
+        return;
+        }
+    if (h0 || h1)
+        {
+        r.node = h0 ? lk.x : lk.y;
+        return;
+        }
+    r.node = EMPTY;
+    while (st.sp > 0)
+        {
+        uint32_t nn;
+        float tm;
+        st.pop(nn, tm);
+        if (!(tm > r.best.t))
+            {
+            r.node = nn;
+            return;
+            }
+        }
+    }
+
+// Postponed primitive tests of type G: the lanes whose pending entry k is a
+// primitive of that type run its intersector together.
+template<uint32_t G, uint32_t TYPES, bool COUNT>
+__device__ __forceinline__ void test_pending(const GeomDev* __restrict__ geoms, TraceState& r, bool has, uint32_t ty,
+                                             uint32_t slot, uint32_t prim, uint32_t& n_prims)
+    {
+    if constexpr ((TYPES & (1u << G)) != 0)
+        {
+        const bool mine = has && ty == G;
+        if (__ballot(mine) == 0)
+            return;
+        if (mine)
+            {
+            float t;
+            if (COUNT)
+                n_prims++;
+            if (intersect_prim<false, 1u << G>(geoms[slot], prim, r.o, r.d, r.tnear, t, nullptr)
+                && better(t, slot, prim, r.best))
+                {
+                r.best.t = t;
+                r.best.geom = slot;
+                r.best.prim = prim;
+                }
+            }
+        }
+    }
+
+// Closest hit over the binary hierarchy for scenes of several primitive
+// types, one ray per lane.  A lane that reaches leaves keeps them pending and
+// waits; once the waiting lanes are at least twice as many as those still
+// searching (or none can go on), the pending primitives are tested one
+// geometry type at a time,
+// so each intersector runs once for all the lanes that need it instead of a
+// few lanes at a time per node visit.  Culling meanwhile uses the closest hit
+// found so far, which only prunes less; every primitive whose bounds pass the
+// slab test and that could still win is tested, so the result is the same
+// (t, geometry, primitive) minimum as trace_closest's.
+template<class Stack, bool COUNT, uint32_t TYPES>
+__device__ inline void trace_closest_grouped(const SceneDev& sc, v3 o, v3 d, float tnear, HitT& best, Stack& st,
+                                             uint32_t& n_nodes, uint32_t& n_prims, uint32_t& err)
+    {
+    TraceState r;
+    trace_begin(sc, o, d, tnear, r, st);
+    uint32_t ps[2] = {0, 0}, pp[2] = {0, 0};
+    uint32_t np = 0;
+    for (;;)
+        {
+        for (;;)
+            {
+            const bool go = r.node != EMPTY && np == 0;
+            const uint32_t n_go = (uint32_t)__popcll(__ballot(go));
+            const uint32_t n_wait = (uint32_t)__popcll(__ballot(np != 0));
+            if (n_go == 0 || n_wait >= 2 * n_go)
+                break;
+            if (go)
+                visit2_collect<Stack, COUNT>(sc.nodes, r, st, ps, pp, np, n_nodes, err);
+            }
+        if (__ballot(np != 0) == 0)
+            break;
+#pragma unroll
+        for (uint32_t k = 0; k < 2; k++)
+            {
+            const bool has = np > k;
+            const uint32_t slot = ps[k], prim = pp[k];
+            const uint32_t ty = has ? sc.geoms[slot].type : 0xffu;
+            test_pending<G_SPHERE, TYPES, COUNT>(sc.geoms, r, has, ty, slot, prim, n_prims);
+            test_pending<G_CYLINDER, TYPES, COUNT>(sc.geoms, r, has, ty, slot, prim, n_prims);
+            test_pending<G_POLYHEDRON, TYPES, COUNT>(sc.geoms, r, has, ty, slot, prim, n_prims);
+            test_pending<G_MESH, TYPES, COUNT>(sc.geoms, r, has, ty, slot, prim, n_prims);
+            test_pending<G_POLYGON, TYPES, COUNT>(sc.geoms, r, has, ty, slot, prim, n_prims);
+            }
+        np = 0;
+        }
+    best = r.best;
+    }
+
 // A ray queue is cut into QSEG segments with one append counter each, the
 // counters QSEG_STRIDE words (256 bytes) apart: a producer unit (a claimed
 // chunk of a persistent kernel, or a workgroup) appends to segment
@@ -887,7 +1035,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TYPES == 
                 }
             const SceneDev& sc = jobs[job].scene;
             HitT h;
-            trace_closest<LdsStack<SS>, COUNT, TYPES>(sc, o, d, tnear, h, st, n_nodes, n_prims, err);
+            if constexpr (TYPES == T_SPHERES)
+                trace_closest<LdsStack<SS>, COUNT, TYPES>(sc, o, d, tnear, h, st, n_nodes, n_prims, err);
+            else
+                trace_closest_grouped<LdsStack<SS>, COUNT, TYPES>(sc, o, d, tnear, h, st, n_nodes, n_prims, err);
             hits[i] = make_uint4(__float_as_uint(h.t), h.geom, h.prim, 0);
             if (COUNT)
                 n_rays++;

@@ -308,6 +308,8 @@ __device__ inline bool triangle(float& u, float& v, float& w, float& t, float& d
 // The callbacks of fresnel/cpu/Geometry*.cc with `t < tfar` left to the caller.
 // Geometry-type sets a kernel is compiled for: a spheres-only scene uses
 // kernels without the other intersectors (fewer registers, more wavefronts).
+// Instantiated for one type, intersect_prim holds only that intersector (the
+// other cases return false), for callers that group tests by type.
 constexpr uint32_t T_SPHERES = 1u << G_SPHERE;
 constexpr uint32_t T_ALL = 0x1fu;
 
@@ -336,6 +338,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         {
     case G_SPHERE: // fresnel/cpu/GeometrySphere.cc:81-117
         {
+        if constexpr (!(TYPES & (1u << G_SPHERE)))
+            return false;
         const float4 s = g.a[prim];
         const v3 p = xyz(s);
         float t, de;
@@ -353,6 +357,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         }
     case G_CYLINDER: // fresnel/cpu/GeometryCylinder.cc:76-118
         {
+        if constexpr (!(TYPES & (1u << G_CYLINDER)))
+            return false;
         const float4 A = g.a[prim];
         const float4 B = g.b[prim];
         float t, de = 0;
@@ -371,6 +377,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         }
     case G_POLYHEDRON: // fresnel/cpu/GeometryConvexPolyhedron.cc:140-335
         {
+        if constexpr (!(TYPES & (1u << G_POLYHEDRON)))
+            return false;
         const v3 pos = xyz(g.a[prim]);
         const quat q = q4(g.b[prim]);
         const v3 dl = rotate(conj(q), d);
@@ -484,6 +492,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         }
     case G_MESH: // fresnel/cpu/GeometryMesh.cc:115-192
         {
+        if constexpr (!(TYPES & (1u << G_MESH)))
+            return false;
         const uint32_t inst = prim / g.n_aux, face = prim - inst * g.n_aux;
         const v3 p3 = xyz(g.a[inst]);
         const quat q = q4(g.b[inst]);
@@ -510,6 +520,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         }
     default: // G_POLYGON, fresnel/cpu/GeometryPolygon.cc:150-232
         {
+        if constexpr (!(TYPES & (1u << G_POLYGON)))
+            return false;
         const float4 pa = g.a[prim];
         const quat q = q4(g.b[prim]);
         const v3 pw = mk3(pa.x, pa.y, 0.0f);
