@@ -665,6 +665,8 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
             done[i] = time.perf_counter() - c0
         return done, [image_digest(t) for t in tracers]
 
+    steps = []  # (start, duration) of each server step of the last served stream
+
     def run_served_stream(arrive):
         scenes, tracers = fresh()
         srv = serve.Server(dev, hbm_budget=budget, target_paths=target_paths)
@@ -672,6 +674,7 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
         handles = [None] * len(specs)
         done = numpy.full(len(specs), numpy.nan)
         nxt = 0
+        steps.clear()
         c0 = time.perf_counter()
         while numpy.isnan(done).any():
             now = time.perf_counter() - c0
@@ -680,6 +683,7 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
                 handles[i] = srv.submit(scenes[i], tracers[i], specs[i]['samples'], 1)
                 nxt += 1
             a, q = srv.step()
+            steps.append((now, time.perf_counter() - c0 - now))
             now = time.perf_counter() - c0
             for i, h in enumerate(handles):
                 if h is not None and numpy.isnan(done[i]) and h.state >= serve.DONE:
@@ -706,6 +710,11 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
             row[name] = {"mpaths_per_s": round(total_paths / span / 1e6, 2), "makespan_s": round(span, 3),
                          "latency_s": {p: round(float(numpy.percentile(lat, q)), 4)
                                        for p, q in (("p50", 50), ("p95", 95), ("p99", 99))}}
+            if name == 'served' and steps:
+                dur = numpy.array([d for _, d in steps])
+                k = int(dur.argmax())
+                row[name]["step_ms"] = {"n": len(steps), "p50": round(float(numpy.median(dur)) * 1e3, 2),
+                                        "max": round(float(dur[k]) * 1e3, 2), "max_at_s": round(steps[k][0], 3)}
             log(f"served leg lambda={lam} {name}: {row[name]}")
         same = digests['served'] == digests['plain']
         identical = identical and same
