@@ -53,6 +53,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|
 """
 
 import argparse
+import gc
 import hashlib
 import json
 import math
@@ -703,7 +704,14 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
         row = {"lambda_per_s": lam if lam > 0 else "burst"}
         digests = {}
         for name, fn in (('plain', run_plain_stream), ('served', run_served_stream)):
-            done, dig = fn(arrive)
+            # the driver loop's latencies should not include Python's collector:
+            # collect before the stream, none during it (both modes alike)
+            gc.collect()
+            gc.disable()
+            try:
+                done, dig = fn(arrive)
+            finally:
+                gc.enable()
             digests[name] = dig
             lat = done - arrive
             span = done.max()
