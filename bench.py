@@ -666,7 +666,7 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
             done[i] = time.perf_counter() - c0
         return done, [image_digest(t) for t in tracers]
 
-    steps = []  # (start, duration) of each server step of the last served stream
+    steps = []  # (start, duration, active requests after it) of each server step of the last served stream
 
     def run_served_stream(arrive):
         scenes, tracers = fresh()
@@ -684,7 +684,7 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
                 handles[i] = srv.submit(scenes[i], tracers[i], specs[i]['samples'], 1)
                 nxt += 1
             a, q = srv.step()
-            steps.append((now, time.perf_counter() - c0 - now))
+            steps.append((now, time.perf_counter() - c0 - now, a))
             now = time.perf_counter() - c0
             for i, h in enumerate(handles):
                 if h is not None and numpy.isnan(done[i]) and h.state >= serve.DONE:
@@ -719,10 +719,11 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
                          "latency_s": {p: round(float(numpy.percentile(lat, q)), 4)
                                        for p, q in (("p50", 50), ("p95", 95), ("p99", 99))}}
             if name == 'served' and steps:
-                dur = numpy.array([d for _, d in steps])
+                dur = numpy.array([d for _, d, _ in steps])
                 k = int(dur.argmax())
                 row[name]["step_ms"] = {"n": len(steps), "p50": round(float(numpy.median(dur)) * 1e3, 2),
                                         "max": round(float(dur[k]) * 1e3, 2), "max_at_s": round(steps[k][0], 3)}
+                row[name]["max_active_requests"] = int(max(a for _, _, a in steps))
             log(f"served leg lambda={lam} {name}: {row[name]}")
         same = digests['served'] == digests['plain']
         identical = identical and same
@@ -736,9 +737,9 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
 
 C5_DESCRIPTION = ("spheres (FCC generator) or spherocylinders (nearest-neighbour generator), N log-uniform in "
                   "[1e3, 2e6], resolution uniform over {256^2, 1280x720, 1920x1080, 3840x2160}, samples uniform "
-                  "over {8, 64, 256}, light_samples 1; Poisson arrivals; 240 GB budget")
+                  "over {8, 64, 256}, light_samples 1; Poisson arrivals; 240 GiB budget")
 SMALL_DESCRIPTION = ("small requests: spheres or spherocylinders, N log-uniform in [1e3, 1e5], 256^2 or 512^2, "
-                     "8 samples, light_samples 1; Poisson arrivals; 240 GB budget")
+                     "8 samples, light_samples 1; Poisson arrivals; 240 GiB budget")
 
 
 def run_served(args):
@@ -794,6 +795,12 @@ def main():
                                        "24 requests: " + C5_DESCRIPTION)
             out["served_small"] = served_leg(small_requests(96), [200.0, 0.0], 32 << 20,
                                              "96 requests: " + SMALL_DESCRIPTION)
+            # a budget that binds: the server's two scratches take 86 of its 98.8 GB, so the
+            # requests' 17.7 GB cannot all be active at once, and the device's Path.sample
+            # scratch is released
+            out["served_budget"] = served_leg(make_requests(24), [0.0], 256 << 20,
+                                              "24 requests: " + C5_DESCRIPTION.replace("240 GiB", "92 GiB"),
+                                              budget=92 << 30)
             log(f"batched and served legs: {time.perf_counter() - t0:.1f} s")
         print(json.dumps(out), flush=True)
 

@@ -122,9 +122,9 @@ def test_scratch_bytes_formula(lib):
 
 
 def test_full_budget_with_scratches_stays_in_hbm(lib):
-    """A 240 GB budget holds the server's two render scratches (256 Mi paths per step), the device's two
+    """A 240 GiB budget holds the server's two render scratches (256 Mi paths per step), the device's two
     Path.sample scratches (256 Mi-path batches) and the admitted requests: admission gets only what is left,
-    so the peak of everything together stays within the budget and inside the 288 GB of one MI355X."""
+    so the peak of everything together stays within the budget and inside the 288 GiB of one MI355X."""
     GB = 1 << 30
     budget = 240 * GB
     fixed = 300 << 20  # overflow stacks and counters per scratch (upper bound)
