@@ -1911,15 +1911,19 @@ __global__ __launch_bounds__(BLOCK) void k_bounce(const JobDev* __restrict__ job
     }
 
 // Persistent traversal with per-lane refill: a lane whose ray is finished
-// takes the next unclaimed ray of its wavefront's chunk as soon as at least
-// `refill` lanes of the wavefront are idle (or all are), so lanes do not wait
-// for the slowest ray of a 64-ray batch.  Chunks of `chunk` rays are claimed
-// per atomic, one chunk ahead.  Same per-ray arithmetic as k_traverse.
+// starts a new one as soon as at least `refill` lanes of the wavefront are
+// idle (or all are), so lanes do not wait for the slowest ray of a 64-ray
+// batch.  Every lane holds its next ray already loaded: a refill starts the
+// prefetched ray from registers and issues the loads of the one after it,
+// which arrive while the wavefront keeps searching, so a refill never waits
+// on memory.  Rays are claimed per wavefront from the chunks of Claims (XCD
+// bands as in k_traverse).  Same per-ray arithmetic as k_traverse.
 template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
 __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q,
                                                         uint32_t n, const QueueView* __restrict__ view, uint4* hits,
-                                                        uint32_t* work, uint32_t chunk, uint32_t refill,
-                                                        uint2* gstack, uint32_t gstride, unsigned long long* counters)
+                                                        uint32_t* work, uint32_t chunk, uint32_t bands,
+                                                        uint32_t refill, uint2* gstack, uint32_t gstride,
+                                                        unsigned long long* counters)
     {
     if (!take_view(view, VIEW_BOUNCE, q, n))
         return;
@@ -1934,14 +1938,52 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
     const uint32_t lane = lane_id();
     const uint64_t below = (1ull << lane) - 1ull;
     uint32_t n_nodes = 0, n_prims = 0, err = 0, n_rays = 0;
-    uint32_t claim = 0, ahead = 0;
-    if (lane == 0)
-        {
-        claim = atomicAdd(work, chunk);
-        ahead = atomicAdd(work, chunk);
-        }
-    uint32_t pool = __shfl(claim, 0, 64);
-    uint32_t pool_end = min(n, pool + chunk);
+    Claims cl;
+    cl.work = work;
+    cl.chunk = chunk;
+    cl.n = n;
+    cl.ranges = (bands && !PRIMARY) ? 8u : 1u;
+    cl.r = (cl.ranges == 8u) ? (blockIdx.x & 7u) : 0u;
+    cl.tried = 0;
+    uint32_t pool, pool_end;
+    cl.next(q, lane, pool, pool_end);
+    // the lane's prefetched ray: index, origin + tnear, direction + job
+    uint32_t pf = EMPTY;
+    float4 pa = make_float4(0, 0, 0, 0), pb = make_float4(0, 0, 1, 0);
+    // Give every lane without a prefetched ray the next unclaimed ray, in
+    // lane order, and issue its loads (not waited for here).
+    auto prefetch = [&]()
+    {
+        for (;;)
+            {
+            const uint64_t need = __ballot(pf == EMPTY);
+            if (need == 0 || pool == EMPTY)
+                return;
+            const uint32_t avail = pool_end - pool;
+            const uint32_t rank = (uint32_t)__popcll(need & below);
+            if (pf == EMPTY && rank < avail)
+                {
+                pf = pool + rank;
+                if (PRIMARY)
+                    {
+                    pa = make_float4(0, 0, 0, 1e-3f);
+                    pb = make_float4(0, 0, 1, 0);
+                    }
+                else
+                    {
+                    const uint32_t sl = q_slot(q, pf, pool, pool_end - 1u);
+                    pa = q.o[sl];
+                    pb = q.d[sl];
+                    }
+                }
+            const uint32_t took = min((uint32_t)__popcll(need), avail);
+            pool += took;
+            if (pool < pool_end)
+                return;
+            cl.next(q, lane, pool, pool_end);
+            }
+    };
+    prefetch();
     bool active = false;
     uint32_t i = 0;
     TraceState r;
@@ -1950,15 +1992,14 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
     const GeomDev* geoms = nullptr;
     for (;;)
         {
-        const uint64_t idle = __ballot(!active);
-        const uint32_t n_idle = (uint32_t)__popcll(idle);
-        if (pool < pool_end && (n_idle >= refill || n_idle == 64))
+        const uint32_t n_idle = (uint32_t)__popcll(__ballot(!active));
+        const uint64_t ready = __ballot(!active && pf != EMPTY);
+        if (ready != 0 && (n_idle >= refill || n_idle == 64))
             {
-            const uint32_t avail = pool_end - pool;
-            const uint32_t rank = (uint32_t)__popcll(idle & below);
-            if (!active && rank < avail)
+            if (!active && pf != EMPTY)
                 {
-                i = pool + rank;
+                i = pf;
+                pf = EMPTY;
                 v3 o, d;
                 float tnear;
                 uint32_t job;
@@ -1970,13 +2011,10 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
                     }
                 else
                     {
-                    const uint32_t sl = q_slot(q, i, pool, pool_end - 1u);
-                    const float4 a = q.o[sl];
-                    const float4 b = q.d[sl];
-                    o = mk3(a.x, a.y, a.z);
-                    d = mk3(b.x, b.y, b.z);
-                    tnear = a.w;
-                    job = __float_as_uint(b.w);
+                    o = mk3(pa.x, pa.y, pa.z);
+                    d = mk3(pb.x, pb.y, pb.z);
+                    tnear = pa.w;
+                    job = __float_as_uint(pb.w);
                     }
                 const SceneDev& sc = jobs[job].scene;
                 nodes2 = sc.nodes;
@@ -1985,14 +2023,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_dyn(const JobDev* __restrict
                 trace_begin(sc, o, d, tnear, r, st);
                 active = true;
                 }
-            pool += min(n_idle, avail);
-            if (pool >= pool_end)
-                {
-                pool = __shfl(ahead, 0, 64);
-                pool_end = min(n, pool + chunk);
-                if (lane == 0 && pool < n)
-                    ahead = atomicAdd(work, chunk);
-                }
+            prefetch();
             }
         if (__ballot(active) == 0)
             break;

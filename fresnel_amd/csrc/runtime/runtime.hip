@@ -379,7 +379,8 @@ static void launch_traverse(DeviceObj& D, const char* nm, hipStream_t s, const T
                  [&]
                  {
                      kern<<<grid, TRAV_BLOCK, 0, s>>>(a.jobs, a.n_jobs, a.q, a.n, a.view, a.hits, a.work, a.chunk,
-                                                     D.trav_refill, a.gstack, a.gstride, a.counters);
+                                                     D.xcd_bands == 2 ? 1u : 0u, D.trav_refill, a.gstack,
+                                                     a.gstride, a.counters);
                  });
         return;
         }
