@@ -127,29 +127,45 @@ def perspective_fit(scene):
     return cam
 
 
-def c2_scene(device):
-    import fresnel_amd as fresnel
+def c2_arrays():
+    """Input arrays of the 1,048,576-sphere lattice: positions, radii, colours."""
     pos = fcc_lattice()
     n = len(pos)
     rng = numpy.random.default_rng(1)
     radius = rng.uniform(0.4, 0.6, n).astype(numpy.float32)
+    return dict(position=pos, radius=radius, color=hsv_colors(n, seed=2))
+
+
+def c2_scene(device, a=None):
+    import fresnel_amd as fresnel
+    a = a if a is not None else c2_arrays()
     scene = fresnel.Scene(device, lights=fresnel.light.cloudy())
-    fresnel.geometry.Sphere(scene, position=pos, radius=radius, color=hsv_colors(n, seed=2), outline_width=0.05,
+    fresnel.geometry.Sphere(scene, position=a['position'], radius=a['radius'], color=a['color'], outline_width=0.05,
                             material=fresnel.material.Material(primitive_color_mix=1.0, roughness=0.3, specular=0.5,
                                                                metal=0.0))
     scene.camera = perspective_fit(scene)
     return scene
 
 
-def c1_scene(device):
+def c1_arrays():
+    """Input array of the 4096-sphere scene: positions."""
+    return dict(position=numpy.random.default_rng(0).uniform(-10, 10, (4096, 3)).astype(numpy.float32))
+
+
+def c1_scene(device, a=None):
     import fresnel_amd as fresnel
-    pos = numpy.random.default_rng(0).uniform(-10, 10, (4096, 3)).astype(numpy.float32)
+    a = a if a is not None else c1_arrays()
     scene = fresnel.Scene(device, lights=fresnel.light.cloudy())
-    fresnel.geometry.Sphere(scene, position=pos, radius=0.5, outline_width=0.1,
+    fresnel.geometry.Sphere(scene, position=a['position'], radius=0.5, outline_width=0.1,
                             material=fresnel.material.Material(color=fresnel.color.linear([0.1, 0.8, 0.1]),
                                                                roughness=0.8, specular=0.2))
     scene.camera = perspective_fit(scene)
     return scene
+
+
+def input_digests(arrays):
+    """SHA-256 of each generated input array (its float32/uint32 bytes, C order)."""
+    return {k: hashlib.sha256(numpy.ascontiguousarray(v).tobytes()).hexdigest() for k, v in sorted(arrays.items())}
 
 
 def icosphere(radius=0.4, subdivisions=2):
@@ -187,22 +203,33 @@ def nn_segments(n_points, side, rng):
     return numpy.stack([a, b], axis=1).astype(numpy.float32)
 
 
-def c3_scene(device, n_poly=200000, n_cyl=2000000, n_mesh=50000, side=120.0, seed=3):
-    import fresnel_amd as fresnel
+def c3_arrays(n_poly=200000, n_cyl=2000000, n_mesh=50000, side=120.0, seed=3):
+    """Input arrays of the mixed scene, drawn in this order from one generator: polyhedron positions and
+    orientations and colours, spherocylinder end points, mesh instance positions; and the mesh itself."""
     rng = numpy.random.default_rng(seed)
+    a = dict(poly_position=rng.uniform(-side / 2, side / 2, (n_poly, 3)))
+    a['poly_orientation'] = random_quaternions(n_poly, rng)
+    a['poly_color'] = hsv_colors(n_poly, seed=4)
+    a['cyl_points'] = nn_segments(n_cyl // 2, side, rng)
+    a['mesh_position'] = rng.uniform(-side / 2, side / 2, (n_mesh, 3))
+    a['mesh_triangles'] = icosphere(0.4, 2)
+    return a
+
+
+def c3_scene(device, a=None, n_poly=200000, n_cyl=2000000, n_mesh=50000, side=120.0, seed=3):
+    import fresnel_amd as fresnel
+    a = a if a is not None else c3_arrays(n_poly, n_cyl, n_mesh, side, seed)
     scene = fresnel.Scene(device, lights=fresnel.light.cloudy())
     cube = numpy.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.25, 0.25)])
     info = fresnel.util.convex_polyhedron_from_vertices(cube)
-    fresnel.geometry.ConvexPolyhedron(scene, info, position=rng.uniform(-side / 2, side / 2, (n_poly, 3)),
-                                      orientation=random_quaternions(n_poly, rng), color=hsv_colors(n_poly, seed=4),
+    fresnel.geometry.ConvexPolyhedron(scene, info, position=a['poly_position'], orientation=a['poly_orientation'],
+                                      color=a['poly_color'],
                                       material=fresnel.material.Material(primitive_color_mix=1.0, roughness=0.1,
                                                                          metal=0.0), outline_width=0.03)
-    seg = nn_segments(n_cyl // 2, side, rng)
-    fresnel.geometry.Cylinder(scene, points=seg, radius=0.1, color=[0.8, 0.7, 0.3],
+    fresnel.geometry.Cylinder(scene, points=a['cyl_points'], radius=0.1, color=[0.8, 0.7, 0.3],
                               material=fresnel.material.Material(color=fresnel.color.linear([0.8, 0.7, 0.3]),
                                                                  roughness=0.5, metal=1.0))
-    fresnel.geometry.Mesh(scene, icosphere(0.4, 2), position=rng.uniform(-side / 2, side / 2, (n_mesh, 3)),
-                          color=[0.3, 0.4, 0.9],
+    fresnel.geometry.Mesh(scene, a['mesh_triangles'], position=a['mesh_position'], color=[0.3, 0.4, 0.9],
                           material=fresnel.material.Material(color=fresnel.color.linear([0.3, 0.4, 0.9]),
                                                              roughness=0.9, metal=0.0))
     scene.camera = perspective_fit(scene)
@@ -210,15 +237,15 @@ def c3_scene(device, n_poly=200000, n_cyl=2000000, n_mesh=50000, side=120.0, see
 
 
 WORKLOADS = {
-    'c2': dict(build=c2_scene, W=1920, H=1080, samples=256, light_samples=1, prim_bytes=16, node_bytes=128,
+    'c2': dict(build=c2_scene, arrays=c2_arrays, W=1920, H=1080, samples=256, light_samples=1, prim_bytes=16, node_bytes=128,
                name="1,048,576 spheres, 64^3 FCC lattice a=1.5 jitter 0.05, r~U[0.4,0.6], HSV colors, "
                "roughness 0.3 specular 0.5, outline 0.05, light.cloudy(), perspective fit; "
                "1920x1080, samples=256, light_samples=1"),
-    'c1': dict(build=c1_scene, W=256, H=256, samples=16, light_samples=8, prim_bytes=16, node_bytes=128,
+    'c1': dict(build=c1_scene, arrays=c1_arrays, W=256, H=256, samples=16, light_samples=8, prim_bytes=16, node_bytes=128,
                name="4096 spheres uniform in a cube of side 20 (seed 0), r=0.5, outline 0.1, "
                "color linear(0.1,0.8,0.1) roughness 0.8 specular 0.2, light.cloudy(), perspective fit; "
                "256x256, samples=16, light_samples=8"),
-    'c3': dict(build=c3_scene, W=1920, H=1080, samples=128, light_samples=1, prim_bytes=28, node_bytes=64,
+    'c3': dict(build=c3_scene, arrays=c3_arrays, W=1920, H=1080, samples=128, light_samples=1, prim_bytes=28, node_bytes=64,
                name="200k cuboid polyhedra 1x1x0.5 random orientations + 2M spherocylinders r=0.1 joining "
                "1st/2nd nearest neighbours of 1M points + 50k icosphere meshes (320 triangles, r=0.4), "
                "cube of side 120, materials rough 0.1/0.5/0.9 metal 0/1/0, light.cloudy(), perspective fit; "
@@ -327,7 +354,10 @@ def run_plain(args):
     W, H, S, L = wl['W'], wl['H'], wl['samples'], wl['light_samples']
     dev = fresnel.Device(mode='gpu', n=1)
     t_build = time.perf_counter()
-    scene = wl['build'](dev)
+    arrays = wl['arrays']()
+    digests = input_digests(arrays)
+    scene = wl['build'](dev, arrays)
+    del arrays
     tracer = fresnel.tracer.Path(dev, w=W, h=H)
     print(f"# scene built on host in {time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
     d = dev._device
@@ -451,7 +481,8 @@ def run_plain(args):
 
     out = base_record(round(value, 3), "Mpaths/s", args.steps, args.warmup, round(mean_s * 1e3, 2),
                       {"workload": wl['name'], "mode": "plain", "paths_per_frame": paths,
-                       "rays_per_frame": int(seg_rays), "mrays_per_s": round(seg_rays / mean_s / 1e6, 2)})
+                       "rays_per_frame": int(seg_rays), "mrays_per_s": round(seg_rays / mean_s / 1e6, 2),
+                       "inputs_sha256": digests})
     out.update({
         "latency_ms": {"p50": round(float(numpy.percentile(times, 50)) * 1e3, 2),
                        "p99": round(float(numpy.percentile(times, 99)) * 1e3, 2)},
