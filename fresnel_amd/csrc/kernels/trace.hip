@@ -1345,7 +1345,7 @@ typedef const uint32_t __attribute__((address_space(4))) ConstU;
 typedef const GeomDev __attribute__((address_space(4))) ConstGeom;
 
 // Leaf test of a primitive every lane of the wavefront shares: the geometry
-// record and, for spheres, the sphere arrive through scalar loads.
+// record and the primitive's data arrive through scalar loads.
 template<uint32_t TYPES>
 __device__ __forceinline__ bool leaf_test_uniform(const GeomDev* geoms, uint32_t slot, uint32_t prim, v3 o, v3 d,
                                                   float tnear, float& t)
@@ -1359,7 +1359,10 @@ __device__ __forceinline__ bool leaf_test_uniform(const GeomDev* geoms, uint32_t
         return sphere_t(o, d, mk3(sp[0], sp[1], sp[2]), sp[3], t, de, first) && (tnear < t);
         }
     else
-        return leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
+        {
+        const GeomDev g = uniform_geom(geoms, slot);
+        return intersect_prim<false, TYPES, true>(g, prim, o, d, tnear, t, nullptr);
+        }
     }
 
 // Tile-list entries examined (counting kernels only): all, and those of
@@ -1513,9 +1516,12 @@ __device__ __forceinline__ void primary_group_hit(const JobDev* __restrict__ job
                             {
                             if (COUNT)
                                 n_prims++;
-                            slot = __float_as_uint(J.prim_lo[r].w);
-                            prim = __float_as_uint(J.prim_hi[r].w);
-                            hit = leaf_test<TYPES>(geoms[slot], prim, o, d, tnear, t);
+                            // the entry's primitive is the same in every lane: its
+                            // record and data arrive through scalar loads
+                            slot = __float_as_uint(((ConstF*)J.prim_lo)[4 * (size_t)r + 3]);
+                            prim = __float_as_uint(((ConstF*)J.prim_hi)[4 * (size_t)r + 3]);
+                            const GeomDev g = uniform_geom(geoms, slot);
+                            hit = intersect_prim<false, TYPES, true>(g, prim, o, d, tnear, t, nullptr);
                             }
                         }
                     if (hit && better(t, slot, prim, best))

@@ -304,6 +304,40 @@ __device__ inline bool triangle(float& u, float& v, float& w, float& t, float& d
     return true;
     }
 
+// p[i]; with UNI (p and i the same in every lane of the wavefront) through the
+// constant address space, so that the load is a scalar one.
+template<bool UNI>
+__device__ __forceinline__ float4 ld4(const float4* p, uint32_t i)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (UNI)
+        return ((const __attribute__((address_space(4))) float4*)p)[i];
+#endif
+    return p[i];
+    }
+
+// The fields of geometry record `slot` that intersect_prim reads without
+// attributes, loaded through the constant address space (slot the same in
+// every lane: scalar loads).
+__device__ __forceinline__ GeomDev uniform_geom(const GeomDev* geoms, uint32_t slot)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const GeomDev __attribute__((address_space(4)))* gp = (const GeomDev __attribute__((address_space(4)))*)geoms + slot;
+#else
+    const GeomDev* gp = geoms + slot;
+#endif
+    GeomDev g;
+    g.type = gp->type;
+    g.n_aux = gp->n_aux;
+    g.rounding = gp->rounding;
+    g.xy_radius = gp->xy_radius;
+    g.a = gp->a;
+    g.b = gp->b;
+    g.aux0 = gp->aux0;
+    g.aux1 = gp->aux1;
+    return g;
+    }
+
 // ---------------------------------------------------------------- intersect one primitive
 // The callbacks of fresnel/cpu/Geometry*.cc with `t < tfar` left to the caller.
 // Geometry-type sets a kernel is compiled for: a spheres-only scene uses
@@ -313,13 +347,13 @@ __device__ inline bool triangle(float& u, float& v, float& w, float& t, float& d
 constexpr uint32_t T_SPHERES = 1u << G_SPHERE;
 constexpr uint32_t T_ALL = 0x1fu;
 
-template<bool ATTR, uint32_t TYPES = T_ALL>
+template<bool ATTR, uint32_t TYPES = T_ALL, bool UNI = false>
 __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 d, float tnear, float& t_out,
                                       Attr* attr)
     {
     if constexpr (TYPES == T_SPHERES)
         {
-        const float4 s = g.a[prim];
+        const float4 s = ld4<UNI>(g.a, prim);
         const v3 p = xyz(s);
         float t, de;
         bool first;
@@ -340,7 +374,7 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         {
         if constexpr (!(TYPES & (1u << G_SPHERE)))
             return false;
-        const float4 s = g.a[prim];
+        const float4 s = ld4<UNI>(g.a, prim);
         const v3 p = xyz(s);
         float t, de;
         bool first;
@@ -359,8 +393,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         {
         if constexpr (!(TYPES & (1u << G_CYLINDER)))
             return false;
-        const float4 A = g.a[prim];
-        const float4 B = g.b[prim];
+        const float4 A = ld4<UNI>(g.a, prim);
+        const float4 B = ld4<UNI>(g.b, prim);
         float t, de = 0;
         v3 N = mk3(0, 0, 0);
         uint32_t ci = 0;
@@ -379,8 +413,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         {
         if constexpr (!(TYPES & (1u << G_POLYHEDRON)))
             return false;
-        const v3 pos = xyz(g.a[prim]);
-        const quat q = q4(g.b[prim]);
+        const v3 pos = xyz(ld4<UNI>(g.a, prim));
+        const quat q = q4(ld4<UNI>(g.b, prim));
         const v3 dl = rotate(conj(q), d);
         const v3 ol = rotate(conj(q), o - pos);
         float t0 = -3.40282346638528859812e+38f, t1 = 3.40282346638528859812e+38f;
@@ -388,13 +422,13 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         const uint32_t P = g.n_aux;
         for (uint32_t i = 0; i < P && t0 < t1; ++i)
             {
-            const float4 pl = g.aux0[i];
+            const float4 pl = ld4<UNI>(g.aux0, i);
             const v3 n = xyz(pl);
             const float denom = dot(n, dl);
             const float t = -(pl.w + dot(n, ol)) / denom;
             if (fabsf(denom) <= EPS5F)
                 {
-                if (dot(ol - xyz(g.aux1[i]), n) > 0)
+                if (dot(ol - xyz(ld4<UNI>(g.aux1, i)), n) > 0)
                     return false;
                 }
             else if (denom < 0)
@@ -435,16 +469,16 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
             // an exit plane that was never found keeps the reference's zero
             // normal and origin and plane 0 (only for degenerate plane sets)
             const bool set = (tnear < t0) ? true : (t1 != 3.40282346638528859812e+38f);
-            const v3 n_hit = set ? xyz(g.aux0[ih]) : mk3(0, 0, 0);
-            const v3 p_hit = set ? xyz(g.aux1[ih]) : mk3(0, 0, 0);
+            const v3 n_hit = set ? xyz(ld4<UNI>(g.aux0, ih)) : mk3(0, 0, 0);
+            const v3 p_hit = set ? xyz(ld4<UNI>(g.aux1, ih)) : mk3(0, 0, 0);
             attr->Ng = rotate(q, n_hit);
             attr->color = lerp(g.color_by_face, ld3(g.color, prim), ld3(g.aux_color, ih));
             float min_d = 3.40282346638528859812e+38f;
             const v3 r_hit = ol + th * dl;
             for (uint32_t i = 0; i < P; ++i)
                 {
-                const v3 n = xyz(g.aux0[i]);
-                const v3 p = xyz(g.aux1[i]);
+                const v3 n = xyz(ld4<UNI>(g.aux0, i));
+                const v3 p = xyz(ld4<UNI>(g.aux1, i));
                 v3 u = cross(n, n_hit);
                 if (fabsf(dot(u, u)) > EPS5F)
                     {
@@ -495,11 +529,11 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         if constexpr (!(TYPES & (1u << G_MESH)))
             return false;
         const uint32_t inst = prim / g.n_aux, face = prim - inst * g.n_aux;
-        const v3 p3 = xyz(g.a[inst]);
-        const quat q = q4(g.b[inst]);
+        const v3 p3 = xyz(ld4<UNI>(g.a, inst));
+        const quat q = q4(ld4<UNI>(g.b, inst));
         const v3 dl = rotate(conj(q), d);
         const v3 ol = rotate(conj(q), o - p3);
-        const v3 v0 = xyz(g.aux0[3 * face]), v1 = xyz(g.aux0[3 * face + 1]), v2_ = xyz(g.aux0[3 * face + 2]);
+        const v3 v0 = xyz(ld4<UNI>(g.aux0, 3 * face)), v1 = xyz(ld4<UNI>(g.aux0, 3 * face + 1)), v2_ = xyz(ld4<UNI>(g.aux0, 3 * face + 2));
         const v3 qend = ol + dl;
         float u = 0, v = 0, w = 0, t = 0, de = 0;
         v3 n = mk3(0, 0, 0);
@@ -522,8 +556,8 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         {
         if constexpr (!(TYPES & (1u << G_POLYGON)))
             return false;
-        const float4 pa = g.a[prim];
-        const quat q = q4(g.b[prim]);
+        const float4 pa = ld4<UNI>(g.a, prim);
+        const quat q = q4(ld4<UNI>(g.b, prim));
         const v3 pw = mk3(pa.x, pa.y, 0.0f);
         const v3 dl = rotate(conj(q), d);
         const v3 ol = rotate(conj(q), o - pw);
@@ -544,7 +578,7 @@ __device__ inline bool intersect_prim(const GeomDev& g, uint32_t prim, v3 o, v3 
         uint32_t j = nv - 1;
         for (uint32_t i = 0; i < nv; i++)
             {
-            const float4 fi = g.aux0[i], fj = g.aux0[j];
+            const float4 fi = ld4<UNI>(g.aux0, i), fj = ld4<UNI>(g.aux0, j);
             v2 vi, vj;
             vi.x = fi.x;
             vi.y = fi.y;
