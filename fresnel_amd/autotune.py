@@ -40,7 +40,7 @@ import time
 CANDIDATES = {
     'stack_lds': [8, 16],
     'batch_paths': [32 << 20, 64 << 20, 128 << 20, 256 << 20],
-    'tail_rays': [8192, 32768, 131072],
+    'tail_rays': [8192, 32768, 131072, 524288, 1048576],
     'trav_chunk': [64, 256, 1024],
     'primary_chunk': [256, 1024, 4096],
     'trav_refill': [16, 32, 48, 65],

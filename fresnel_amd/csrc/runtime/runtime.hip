@@ -232,7 +232,7 @@ struct DeviceObj
     bool stats_on = false;
     bool count_on = false;
     uint64_t batch_paths = 256ull << 20; // light-sample paths per batch (sizes the queues)
-    uint32_t tail_rays = 131072;        // finish paths in one kernel below this many rays
+    uint32_t tail_rays = 524288;        // finish paths in one kernel below this many rays
     uint32_t trav_chunk = 256;          // rays a secondary-ray traversal wavefront claims per atomic
     uint32_t primary_chunk = 1024;      // the same for primary rays
     uint32_t trav_packet = 2;           // primary rays: 2 wavefront packets, 1 per ray

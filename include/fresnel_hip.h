@@ -311,7 +311,7 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
 /* ------------------------------------------------------ tuning (no reference
  * counterpart).  key "batch_paths": light-sample paths per wavefront batch
  * (default 256 Mi; sizes the ray queues, ≈ 160 B per path per render scratch); "tail_rays": below this many live
- * rays a batch finishes its paths in one kernel (default 131072); "stack_lds":
+ * rays a batch finishes its paths in one kernel (default 524288); "stack_lds":
  * traversal stack entries kept in LDS per thread, 8 or 16 (default 8);
  * "trav_chunk": rays a secondary-ray traversal wavefront claims per
  * work-counter atomic, a multiple of 64 up to 65536 (default 256);
