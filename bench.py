@@ -735,7 +735,7 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
         row = {"lambda_per_s": lam if lam > 0 else "burst"}
         digests = {}
         for name, fn in (('plain', run_plain_stream), ('served', run_served_stream)):
-            # the driver loop's latencies should not include Python's collector:
+            # the request loop's latencies should not include Python's collector:
             # collect before the stream, none during it (both modes alike)
             gc.collect()
             gc.disable()
