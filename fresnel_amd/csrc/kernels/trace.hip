@@ -976,10 +976,13 @@ struct Claims
         }
     };
 
-// The spheres-only variant is held to 71 registers (7 wavefronts per SIMD, no
-// spills): its search is latency-bound, so wavefronts in flight matter.
+// The search is latency-bound (dependent node loads), so wavefronts in flight
+// matter: the spheres-only variant is held to 71 registers (7 wavefronts per
+// SIMD, no spills; 8 spill and run 7 % slower), the others to 64 (8 per SIMD,
+// a few values spill; on C3 the traversal takes 51 ms per frame at 8, 53 at
+// 7, 57 at 6 and 65 at the 5 its 87 registers allow).
 template<int BLOCK, int SS, bool COUNT, bool PRIMARY, uint32_t TYPES>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TYPES == T_SPHERES ? 7 : 1))) void k_traverse(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TYPES == T_SPHERES ? 7 : 8))) void k_traverse(
     const JobDev* __restrict__ jobs, uint32_t n_jobs, Queue q, uint32_t n, const QueueView* __restrict__ view,
     uint4* hits, uint32_t* work, uint32_t chunk, uint32_t bands, uint2* gstack, uint32_t gstride,
     unsigned long long* counters)
@@ -1718,11 +1721,14 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_packet(const JobDev* __restr
 // groups of 64 rays (primary_group_hit) and the workgroup shades them
 // together (shade_primary_block), so no hit is stored and no primary ray is
 // generated twice.  Same arithmetic as k_traverse_packet + k_shade_primary.
-// Held to 128 registers (4 wavefronts per SIMD; a few values spill): with the
-// double-precision sine and cosine of the depth-0 lobe sample it would take
-// 147 (3 wavefronts) and run 1.7x longer.
+// The spheres-only variant is held to 128 registers (4 wavefronts per SIMD; a
+// few values spill): with the double-precision sine and cosine of the depth-0
+// lobe sample it would take 147 (3 wavefronts) and run 1.7x longer, and at 5
+// wavefronts it spills more and runs 17 % longer.  The variant for other
+// scenes, whose list search waits on primitive loads, runs best at 6 (C3's
+// primary kernel: 31.2 ms per frame at 4, 29.8 at 5, 27.7 at 6, 28.2 at 8).
 template<int BLOCK, int SS, bool COUNT, uint32_t TYPES>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_primary(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TYPES == T_SPHERES ? 4 : 6))) void k_primary(
     const JobDev* __restrict__ jobs, uint32_t n_jobs, uint32_t n, uint32_t* work, uint32_t groups, uint32_t bands,
     uint2* gstack, uint32_t gstride, Queue out, uint4* records, unsigned long long* counters)
     {
