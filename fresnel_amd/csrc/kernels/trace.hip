@@ -2078,7 +2078,10 @@ __device__ __forceinline__ void finish_path(const JobDev& J, const uint4& mt, v3
     for (;;)
         {
         HitT h;
-        trace_closest<LdsStack<SS>, COUNT, TYPES>(J.scene, o, d, 1e-3f, h, st, n_nodes, n_prims, err);
+        if constexpr (TYPES == T_SPHERES)
+            trace_closest<LdsStack<SS>, COUNT, TYPES>(J.scene, o, d, 1e-3f, h, st, n_nodes, n_prims, err);
+        else
+            trace_closest_grouped<LdsStack<SS>, COUNT, TYPES>(J.scene, o, d, 1e-3f, h, st, n_nodes, n_prims, err);
         if (COUNT)
             n_rays++;
         v3 no, nd;
