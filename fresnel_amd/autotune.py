@@ -12,12 +12,15 @@ for every setting; only speed changes.  Knobs (see fh_device_tune):
                   paths in one kernel.
 * ``trav_chunk``  rays a secondary-ray traversal wavefront claims per
                   work-counter atomic.
-* ``primary_chunk`` the same for primary rays.
+* ``primary_chunk`` the most primary rays a workgroup claims at once (a
+                  batch with fewer than 16 such claims per CU claims less).
 * ``trav_refill`` idle lanes of a traversal wavefront that trigger taking new
                   rays (65: trace 64 rays at a time).
 * ``trav_packet`` primary rays traced as wavefront packets (2) or per ray (1).
 * ``primary_tiles`` primary rays tested against per-tile candidate lists (2)
-                  or searched in the BVH (1).
+                  or searched in the BVH (1).  With lists on, a tracer builds
+                  them once ``tile_min_rays`` primary rays (not tuned here) have
+                  used one scene and camera.
 * ``tile_group`` largest tile count of a hierarchy subtree listed as one
                   tile-list entry (scenes other than spheres-only; 1: one
                   entry per primitive).

@@ -234,10 +234,11 @@ struct DeviceObj
     uint64_t batch_paths = 256ull << 20; // light-sample paths per batch (sizes the queues)
     uint32_t tail_rays = 524288;        // finish paths in one kernel below this many rays
     uint32_t trav_chunk = 256;          // rays a secondary-ray traversal wavefront claims per atomic
-    uint32_t primary_chunk = 1024;      // the same for primary rays
+    uint32_t primary_chunk = 1024;      // the most primary rays a workgroup claims at once
     uint32_t trav_packet = 2;           // primary rays: 2 wavefront packets, 1 per ray
     uint32_t primary_tiles = 2;         // primary rays of pinhole/orthographic cameras: 2 tile lists, 1 BVH
     uint32_t tile_group = 4;            // tile-list subtrees cover at most this many tiles (1: primitives only)
+    uint64_t tile_min_rays = 4ull << 20; // tile lists once this many primary rays used one scene and camera
     uint32_t fuse_primary = 2;          // 2: primary traversal and depth-0 shading in one kernel, 1: two
     uint32_t fuse_bounce = 1;           // 2: secondary traversal and shading in one kernel, 1: two
     uint32_t inflight = 2;              // batches of one Path.sample in flight (1 or 2)
@@ -1185,6 +1186,8 @@ struct TileLists
     uint32_t tiles_x = 0, tiles_y = 0;
     uint32_t group = 0;        // tile_group the lists were built with
     uint32_t n_entries = 0;
+    bool decided = false;      // lists built for the key above, or known unusable
+    uint64_t rays = 0;         // primary rays rendered with the key above while undecided
     DBuf<uint32_t> counts, flags, tile0, q0, ref0, keysA, valsA, keysB, valsB, start, end, hist;
     DBuf<uint4> entries; // 2 per entry
     std::deque<DBuf<uint32_t>> scan_tmp;
@@ -1263,28 +1266,46 @@ struct TracerObj
 
 // Bring a tracer's per-tile candidate lists up to date for scene `sc` seen
 // through `cam` (pinhole perspective or orthographic cameras only; a scene a
-// perspective camera's plane cuts keeps the BVH search).
-static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev& cam)
+// perspective camera's plane cuts keeps the BVH search), for a render of
+// `n_pri` primary rays.  Building the lists costs about as much as the BVH
+// search loses on a few million primary rays, so they are built once the
+// primary rays rendered with one scene, camera and image size reach
+// D.tile_min_rays (this render's included); until then primary rays take the
+// BVH search.  Either way the hits are the same.
+static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev& cam, uint64_t n_pri)
     {
     TileLists& T = tr.tiles;
     const bool pinhole = (cam.model == 1 && cam.a == 0.0f) || cam.model == 0;
     if (D.primary_tiles != 2 || !pinhole || sc.n_prims == 0 || tr.W < TILE || tr.H < TILE)
         {
         T.valid = false;
+        T.decided = false;
         T.scene = 0;
         return;
         }
     if (T.scene == sc.serial && T.builds == sc.builds && T.refits == sc.refits && T.W == tr.W && T.H == tr.H
         && T.group == D.tile_group && std::memcmp(&T.cam, &cam, sizeof(CamDev)) == 0)
-        return; // cached, usable or known unusable
-    T.scene = sc.serial;
-    T.group = D.tile_group;
-    T.builds = sc.builds;
-    T.refits = sc.refits;
-    T.cam = cam;
-    T.W = tr.W;
-    T.H = tr.H;
-    T.valid = false;
+        {
+        if (T.decided)
+            return; // cached, usable or known unusable
+        T.rays += n_pri;
+        }
+    else
+        {
+        T.scene = sc.serial;
+        T.group = D.tile_group;
+        T.builds = sc.builds;
+        T.refits = sc.refits;
+        T.cam = cam;
+        T.W = tr.W;
+        T.H = tr.H;
+        T.valid = false;
+        T.decided = false;
+        T.rays = n_pri;
+        }
+    if (T.rays < D.tile_min_rays)
+        return;
+    T.decided = true;
     hipStream_t s = D.stream;
     const uint32_t N = sc.n_prims;
     TileCam c;
@@ -1470,6 +1491,17 @@ static uint32_t segment_capacity(uint64_t n_pri, uint64_t n_paths, uint32_t max_
     return (uint32_t)c;
     }
 
+// Workgroups of 256 rays a primary-kernel workgroup claims at once: those of
+// D.primary_chunk, or fewer when the batch holds under 16 such claims per CU,
+// so that small renders still spread over every CU (a 256 x 256 image at 8
+// samples has 2,048 claims of 256 rays against 512 of 1,024).
+static uint32_t primary_groups(const DeviceObj& D, uint64_t n_pri)
+    {
+    const uint64_t most = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
+    const uint64_t per_claim_round = (uint64_t)std::max(1, D.n_cu) * 16 * PRIMARY_BLOCK;
+    return (uint32_t)std::min<uint64_t>(most, std::max<uint64_t>(1, n_pri / per_claim_round));
+    }
+
 // Divisors of the primary-ray order (primary_order) once K is final.
 static void set_order_divisors(JobDev& J)
     {
@@ -1508,6 +1540,7 @@ struct BatchExec
     std::vector<uint32_t> pix_base;
     uint64_t n_pri = 0, n_paths = 0, n_pix = 0, n_rec = 0;
     uint32_t nj = 0, gstride = 0, seg_cap = 0, tail_threads = 0, rest_grid = 0;
+    uint32_t pgroups = 1; // primary claim size in workgroups of rays (primary_groups)
     uint32_t n_steps = 0;
     int cur = 1;
     Queue Q[2];
@@ -1564,8 +1597,8 @@ struct BatchExec
         uint32_t max_l = 1;
         for (const auto& J : jobs)
             max_l = std::max(max_l, J.L);
-        const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
-        seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * groups, D.trav_chunk);
+        pgroups = primary_groups(D, n_pri);
+        seg_cap = segment_capacity(n_pri, n_paths, max_l, PRIMARY_BLOCK * pgroups, D.trav_chunk);
         const uint64_t cap = (uint64_t)QSEG * seg_cap;
         // growing a buffer frees it first: an earlier batch on this stream
         // (the only user of this scratch) must have finished with it
@@ -1642,7 +1675,7 @@ struct BatchExec
     std::vector<uint64_t> graph_key(uint32_t steps) const
         {
         std::vector<uint64_t> k = {(uint64_t)(uintptr_t)s, nj, n_pri, n_paths, n_pix, n_rec, seg_cap, gstride,
-                                   tail_threads, types, steps, D.trav_chunk, D.primary_chunk, D.xcd_bands,
+                                   tail_threads, types, steps, D.trav_chunk, pgroups, D.xcd_bands,
                                    D.fuse_primary, D.fuse_bounce, D.tail_rays, (uint64_t)D.stack_lds, D.trav_refill,
                                    D.trav_packet, D.count_on ? 1u : 0u, rest_grid};
         for (const void* p : {(const void*)R.qo[0].p, (const void*)R.qo[1].p, (const void*)R.qd[0].p,
@@ -1661,7 +1694,7 @@ struct BatchExec
     // The batch's kernels: counter clears, primary rays, `steps` bounce steps.
     void enqueue_kernels(uint32_t steps)
         {
-        const uint32_t groups = std::max<uint32_t>(1, D.primary_chunk / PRIMARY_BLOCK);
+        const uint32_t groups = pgroups;
         clear(R.counters32.p, 8 * QSEG_STRIDE);
         clear(R.qseg.p, QSEG_FLAG + 1);
         if (D.fuse_primary == 2)
@@ -1975,7 +2008,8 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
     for (auto* sc : uniq)
         sync_scene(*sc);
     for (size_t k = 0; k < scenes.size(); k++)
-        update_tiles(D, *scenes[k], *tracers[k], make_cam(scenes[k]->camera));
+        update_tiles(D, *scenes[k], *tracers[k], make_cam(scenes[k]->camera),
+                     (uint64_t)tracers[k]->W * tracers[k]->H * samples);
     // a readback of an output still in flight finishes before the accumulation
     for (auto* tr : tracers)
         tr->order_after_copies(D.stream, D.stream2);
@@ -2215,7 +2249,7 @@ static void server_plan(ServerObj& S)
             continue;
         paths[kd] += fit * pps;
         r.tracer->finish_copies(); // its accumulation may start on either stream
-        update_tiles(D, *r.scene, *r.tracer, make_cam(r.scene->camera));
+        update_tiles(D, *r.scene, *r.tracer, make_cam(r.scene->camera), (uint64_t)r.tracer->W * r.tracer->H * fit);
         JobDev J = make_job(*r.scene, *r.tracer, r.scene->camera);
         J.K = fit;
         J.n_base = r.tracer->n_samples; // samples planned before this batch
@@ -3943,6 +3977,12 @@ extern "C"
                     if (value > 0)
                         D.tail_rays = (uint32_t)value;
                     }
+                else if (k == "tile_min_rays")
+                    {
+                    prev = (int64_t)D.tile_min_rays;
+                    if (value > 0)
+                        D.tile_min_rays = (uint64_t)value;
+                    }
                 else
                     throw Error(FH_ERR_INVALID, "unknown tuning key: " + k);
                 if (previous)
@@ -4078,7 +4118,7 @@ extern "C"
                 REQUIRE(sc.dev == T.dev, "Scene and Tracer devices do not match");
                 std::lock_guard<std::mutex> lock(D.mu);
                 sync_scene(sc);
-                update_tiles(D, sc, T, make_cam(sc.camera));
+                update_tiles(D, sc, T, make_cam(sc.camera), (uint64_t)T.W * T.H);
                 JobDev J = make_job(sc, T, sc.camera);
                 J.K = 1;
                 J.n_base = sample - 1;

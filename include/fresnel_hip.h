@@ -315,13 +315,20 @@ int fh_policy_simulate(uint32_t n, const uint64_t* footprint, const uint64_t* pa
  * traversal stack entries kept in LDS per thread, 8 or 16 (default 8);
  * "trav_chunk": rays a secondary-ray traversal wavefront claims per
  * work-counter atomic, a multiple of 64 up to 65536 (default 256);
- * "primary_chunk": the same for primary rays (default 1024); "trav_refill": a traversal lane
+ * "primary_chunk": the most primary rays a primary-kernel workgroup claims at
+ * once, a multiple of 64 up to 65536 (default 1024; a batch with fewer than 16
+ * such claims per CU claims less, down to 256); "trav_refill": a traversal lane
  * whose ray is done takes a new one once this many lanes of its wavefront are
  * idle, 1..64, or 65 for the batch-synchronous kernel that traces 64 rays at
  * a time (default 65); "trav_packet": primary rays traced as wavefront
  * packets (2, default) or one by one (1); "primary_tiles": primary rays of
  * pinhole-perspective and orthographic cameras tested against per-8x8-tile
- * candidate lists (2, default) or searched in the BVH (1); "tile_group": in
+ * candidate lists (2, default) or searched in the BVH (1); "tile_min_rays":
+ * with lists on, a tracer builds them once this many primary rays have been
+ * rendered with one scene, camera and image size (this render's included;
+ * default 4 Mi: building costs about what the BVH search loses on a few
+ * million rays) and searches the BVH before; 1 builds them at the first
+ * render; "tile_group": in
  * scenes other than spheres-only, those lists hold hierarchy subtrees whose
  * tile rectangle covers at most this many tiles (a mesh instance's triangles
  * become one entry, searched only by the rays that pass its box), 2..1024, or

@@ -47,8 +47,16 @@ def device_(request):
 
 @pytest.fixture(scope='session')
 def gpu_device():
-    """The GPU device (tests using it must be marked gpu)."""
-    return fresnel.Device(mode='gpu', n=1)
+    """The GPU device (tests using it must be marked gpu).
+
+    Primary rays of pinhole and orthographic cameras take the per-tile candidate lists from the
+    first render here (by default a tracer builds them once a few million primary rays have used one
+    scene and camera, and takes the BVH search before): the parity tests render small images and
+    are meant to compare the list search with the checker.  `test_tile_list_threshold` covers the
+    default."""
+    dev = fresnel.Device(mode='gpu', n=1)
+    dev._device.tune('tile_min_rays', 1)
+    return dev
 
 
 @pytest.fixture(scope='session')

@@ -310,6 +310,48 @@ def test_thin_lens_primary_rays(gpu_device, cpu_device):
     primary_parity("thin lens f_stop=0.2", gs, cs, W, H, 31, (1, 2, 3), spheres=_sphere_arrays(gs))
 
 
+def test_tile_list_threshold(gpu_device):
+    """The default policy: a tracer searches the BVH for primary rays until 4 Mi primary rays have used one
+    scene and camera, then builds the per-tile lists and takes them; the images equal those of a tracer
+    that takes the lists from its first render."""
+    d = gpu_device._device
+    saved = d.tune('tile_min_rays', 0)
+    rng = numpy.random.default_rng(5)
+    s = fresnel.Scene(gpu_device, lights=fresnel.light.cloudy())
+    fresnel.geometry.Sphere(s, position=rng.uniform(-10, 10, (3000, 3)).astype(numpy.float32), radius=0.5,
+                            outline_width=0.1)
+    s.camera = fresnel.camera.Orthographic(position=(30, 30, 30), look_at=(0, 0, 0), up=(0, 1, 0), height=35)
+    W, H, S = 256, 256, 24  # 1.5 Mi primary rays per sample() call
+
+    def renders(tracer, seeds=None):
+        out = []
+        d.countersEnable(True)
+        try:
+            for k in range(3):
+                if seeds is not None:
+                    tracer.seed = seeds[k] - 1  # sample() resets, which advances the seed by one
+                tracer.sample(s, samples=S, light_samples=1)
+                out.append((tracer.seed, int(d.countersLast()[10]), numpy.array(tracer.linear_output[:])))
+        finally:
+            d.countersEnable(False)
+        return out
+
+    try:
+        d.tune('tile_min_rays', 4 << 20)
+        default = renders(fresnel.tracer.Path(gpu_device, w=W, h=H))
+        d.tune('tile_min_rays', 1)
+        lists = renders(fresnel.tracer.Path(gpu_device, w=W, h=H), seeds=[seed for seed, _, _ in default])
+    finally:
+        d.tune('tile_min_rays', saved)
+    print("tile-list entries tested per sample() call: default", [e for _, e, _ in default], "lists from the start",
+          [e for _, e, _ in lists], flush=True)
+    # 1.5 Mi and 3 Mi primary rays: BVH search; 4.5 Mi: lists
+    assert default[0][1] == 0 and default[1][1] == 0 and default[2][1] > 0
+    assert all(e > 0 for _, e, _ in lists)
+    for (_, _, a), (_, _, b) in zip(default, lists):
+        assert numpy.array_equal(a, b)
+
+
 def test_tile_lists_not_reused_across_scenes(gpu_device):
     """A Path tracer reused after its scene was destroyed renders a new, smaller scene with the same
     camera exactly as a fresh tracer does (per-tile lists are keyed on the scene, not its address)."""
