@@ -206,6 +206,37 @@ __global__ __launch_bounds__(256) void k_scan_top(uint32_t* sums, uint32_t n)
         run += v[k];
         }
     }
+// Exclusive scan of n <= SCAN_SINGLE_MAX uint32 in place in one workgroup
+// (1024 threads x 16 consecutive items): the scans of small builds take one
+// launch instead of three.
+constexpr uint32_t SCAN_SINGLE_MAX = 16384;
+__global__ __launch_bounds__(1024) void k_scan_single(uint32_t* data, uint32_t n)
+    {
+    __shared__ uint32_t sh[1024];
+    const uint32_t base = threadIdx.x * 16;
+    uint32_t v[16], s = 0;
+    for (int k = 0; k < 16; k++)
+        {
+        v[k] = (base + k < n) ? data[base + k] : 0u;
+        s += v[k];
+        }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1)
+        {
+        const uint32_t t = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+        }
+    uint32_t run = sh[threadIdx.x] - s;
+    for (int k = 0; k < 16; k++)
+        {
+        if (base + k < n)
+            data[base + k] = run;
+        run += v[k];
+        }
+    }
 __global__ void k_scan_add(uint32_t* data, uint32_t n, const uint32_t* sums)
     {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -117,6 +117,15 @@ template<class T> struct DBuf
         }
     };
 
+// Grow-only buffer used by the kernels of stream s: work still queued there
+// may read the old allocation, so the stream finishes before it is replaced.
+template<class T> static void grow(DBuf<T>& b, size_t n, hipStream_t s)
+    {
+    if (b.p && b.cap < n)
+        HIPCHECK(hipStreamSynchronize(s));
+    b.ensure(n);
+    }
+
 static unsigned blocks(size_t n, unsigned b)
     {
     return (unsigned)((n + b - 1) / b);
@@ -198,6 +207,21 @@ struct PinnedBuf
 // Bounce steps a batch can be given: the last one finishes every path left.
 constexpr uint32_t MAX_STEPS = 64;
 
+// Temporaries of hierarchy builds and of tile-list builds.  Both run on the
+// device stream, one at a time under the device lock, so every scene and
+// tracer shares one set of each (grow-only).
+struct BvhTemps
+    {
+    DBuf<uint32_t> keys, keys2, vals2, hist;
+    std::deque<DBuf<uint32_t>> scan_tmp;
+    DBuf<float> partial, cb;
+    };
+struct TileTemps
+    {
+    DBuf<uint32_t> counts, flags, tile0, q0, ref0, keysA, valsA, keysB, valsB, hist;
+    std::deque<DBuf<uint32_t>> scan_tmp;
+    };
+
 struct DeviceObj
     {
     int hip_dev = 0;
@@ -226,6 +250,8 @@ struct DeviceObj
     uint32_t tag_acts_on = 0;
     hipEvent_t acc_done[2] = {nullptr, nullptr}; // last accumulation enqueued on stream / stream2
     DBuf<unsigned long long> counters;
+    BvhTemps bvh_tmp;
+    TileTemps tile_tmp;
     uint32_t* scratch = nullptr;  // device address of this GPU's allocation-free scratch array
     uint32_t* h_pinned = nullptr; // 16 uint32 of pinned host memory for counts
     uint32_t* h_counts = nullptr; // pinned copy of a queue's segment counters and flag
@@ -642,9 +668,8 @@ struct SceneObj
     DBuf<GeomDev> dgeoms;
     uint32_t n_slots = 0;
     DBuf<float4> blo, bhi;
-    DBuf<uint32_t> keys, vals, keys2, vals2, hist, parent, flags, prefix, slots;
-    std::deque<DBuf<uint32_t>> scan_tmp;
-    DBuf<float> partial, cb, root_box, extent;
+    DBuf<uint32_t> vals, parent, flags, prefix, slots; // vals: primitive order of the leaves (refit reads it)
+    DBuf<float> root_box, extent;
     float cull_margin = 0.0f; // CULL_K * 3 * largest primitive half-extent
     DBuf<Node> nodes;
     DBuf<Node4> nodes4;
@@ -822,10 +847,15 @@ static uint32_t prim_count(const GeometryObj& g)
 static void scan_exclusive(DeviceObj& D, hipStream_t s, uint32_t* data, uint32_t n,
                            std::deque<DBuf<uint32_t>>& tmp, size_t level = 0)
     {
+    if (n <= SCAN_SINGLE_MAX)
+        {
+        k_scan_single<<<1, 1024, 0, s>>>(data, n);
+        return;
+        }
     const uint32_t n_sums = blocks(n, 2048);
     while (tmp.size() <= level)
         tmp.emplace_back();
-    tmp[level].ensure(std::max<uint32_t>(n_sums, 1));
+    grow(tmp[level], std::max<uint32_t>(n_sums, 1), s);
     uint32_t* sums = tmp[level].p;
     k_scan_blocks<<<n_sums, 256, 0, s>>>(data, n, sums);
     if (n_sums <= 2048)
@@ -844,7 +874,7 @@ static bool radix_sort_pairs(DeviceObj& D, hipStream_t s, const char* name, uint
     {
     const uint32_t nb = blocks(N, SORT_TILE);
     const uint32_t nh = 256 * nb;
-    hist.ensure(nh);
+    grow(hist, nh, s);
     uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
     bool flipped = false;
     for (uint32_t shift = bit0; shift < bit1; shift += 8)
@@ -942,29 +972,30 @@ static void build_bvh(SceneObj& sc, bool refit_only)
     if (!refit_only)
         {
         // centroid bounds -> morton -> sort -> hierarchy
+        BvhTemps& X = D.bvh_tmp;
         const uint32_t n_part = std::min<uint32_t>(blocks(N, 256), 1024);
-        sc.partial.ensure(6 * n_part);
-        sc.cb.ensure(6);
+        grow(X.partial, 6 * n_part, s);
+        grow(X.cb, 6, s);
         D.launch("lbvh_centroid",
                  [&]
                  {
-                     k_centroid_partial<<<n_part, 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, sc.partial.p);
-                     k_centroid_final<<<1, 256, 0, s>>>(sc.partial.p, n_part, sc.cb.p);
+                     k_centroid_partial<<<n_part, 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, X.partial.p);
+                     k_centroid_final<<<1, 256, 0, s>>>(X.partial.p, n_part, X.cb.p);
                  });
-        sc.keys.ensure(N);
+        grow(X.keys, N, s);
         sc.vals.ensure(N);
-        sc.keys2.ensure(N);
-        sc.vals2.ensure(N);
+        grow(X.keys2, N, s);
+        grow(X.vals2, N, s);
         D.launch("lbvh_morton",
-                 [&] { k_morton<<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, sc.cb.p, sc.keys.p, sc.vals.p); });
+                 [&] { k_morton<<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, X.cb.p, X.keys.p, sc.vals.p); });
         // four 8-bit passes: the sorted data is back in keys/vals
-        radix_sort_pairs(D, s, "lbvh_sort", sc.keys.p, sc.vals.p, sc.keys2.p, sc.vals2.p, N, 0, 32, sc.hist,
-                         sc.scan_tmp);
+        radix_sort_pairs(D, s, "lbvh_sort", X.keys.p, sc.vals.p, X.keys2.p, X.vals2.p, N, 0, 32, X.hist,
+                         X.scan_tmp);
         sc.parent.ensure(2 * N - 1);
         D.launch("lbvh_karras",
                  [&]
                  {
-                     k_karras<<<blocks(N - 1, 256), 256, 0, s>>>(sc.keys.p, sc.vals.p, sc.blo.p, sc.bhi.p, (int)N,
+                     k_karras<<<blocks(N - 1, 256), 256, 0, s>>>(X.keys.p, sc.vals.p, sc.blo.p, sc.bhi.p, (int)N,
                                                                  sc.nodes.p, sc.parent.p);
                  });
         // 4-wide topology: inner nodes at even depth, numbered in index order
@@ -974,7 +1005,7 @@ static void build_bvh(SceneObj& sc, bool refit_only)
                  [&]
                  {
                      k_depth_even<<<blocks(N, 256), 256, 0, s>>>(sc.parent.p, N - 1, sc.idx4.p);
-                     scan_exclusive(D, s, sc.idx4.p, N, sc.scan_tmp);
+                     scan_exclusive(D, s, sc.idx4.p, N, X.scan_tmp);
                  });
         sc.builds++;
         }
@@ -1005,15 +1036,16 @@ static float bvh_sah(SceneObj& sc)
     DeviceObj& D = *sc.dev;
     const uint32_t n_nodes = sc.n_prims - 1;
     const uint32_t n_part = std::min<uint32_t>(blocks(n_nodes, 256), 1024);
-    sc.partial.ensure(std::max<uint32_t>(6 * n_part, n_part + 1));
+    DBuf<float>& partial = D.bvh_tmp.partial;
+    grow(partial, std::max<uint32_t>(6 * n_part, n_part + 1), D.stream);
     D.launch("lbvh_sah",
              [&]
              {
-                 k_sah_partial<<<n_part, 256, 0, D.stream>>>(sc.nodes.p, n_nodes, sc.partial.p + 1);
-                 k_sah_final<<<1, 256, 0, D.stream>>>(sc.partial.p + 1, n_part, sc.partial.p);
+                 k_sah_partial<<<n_part, 256, 0, D.stream>>>(sc.nodes.p, n_nodes, partial.p + 1);
+                 k_sah_final<<<1, 256, 0, D.stream>>>(partial.p + 1, n_part, partial.p);
              });
     float v = 0.0f;
-    HIPCHECK(hipMemcpyAsync(&v, sc.partial.p, sizeof(float), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipMemcpyAsync(&v, partial.p, sizeof(float), hipMemcpyDeviceToHost, D.stream));
     HIPCHECK(hipStreamSynchronize(D.stream));
     return v;
     }
@@ -1188,9 +1220,8 @@ struct TileLists
     uint32_t n_entries = 0;
     bool decided = false;      // lists built for the key above, or known unusable
     uint64_t rays = 0;         // primary rays rendered with the key above while undecided
-    DBuf<uint32_t> counts, flags, tile0, q0, ref0, keysA, valsA, keysB, valsB, start, end, hist;
-    DBuf<uint4> entries; // 2 per entry
-    std::deque<DBuf<uint32_t>> scan_tmp;
+    DBuf<uint32_t> start, end; // entry range of each tile
+    DBuf<uint4> entries;       // 2 per entry
     };
 
 struct TracerObj
@@ -1336,27 +1367,28 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
     const bool cut = !spheres && D.tile_group >= 2 && N >= 2;
     const uint32_t n_inner = N - 1;
     const uint32_t n_el = cut ? 2 * n_inner + 2 : N + 1;
-    T.counts.ensure(n_el);
-    T.flags.ensure(1);
+    TileTemps& X = D.tile_tmp;
+    grow(X.counts, n_el, s);
+    grow(X.flags, 1, s);
     uint32_t h[2] = {0, 0};
     D.launch("tiles_count",
              [&]
              {
-                 k_fill_u32<<<1, 64, 0, s>>>(T.flags.p, 1, 0u);
+                 k_fill_u32<<<1, 64, 0, s>>>(X.flags.p, 1, 0u);
                  if (cut)
                      k_tile_cut<false><<<blocks(n_inner + 1, 256), 256, 0, s>>>(
                          sc.nodes.p, n_inner, sc.blo.p, sc.bhi.p, sc.prefix.p, sc.slots.p, sc.n_active, c,
-                         D.tile_group, T.counts.p, T.flags.p, nullptr, nullptr, nullptr, nullptr);
+                         D.tile_group, X.counts.p, X.flags.p, nullptr, nullptr, nullptr, nullptr);
                  else if (spheres)
                      k_tile_count<true><<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
-                                                                           T.counts.p, T.flags.p);
+                                                                           X.counts.p, X.flags.p);
                  else
                      k_tile_count<false><<<blocks(N + 1, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
-                                                                            T.counts.p, T.flags.p);
-                 scan_exclusive(D, s, T.counts.p, n_el, T.scan_tmp);
+                                                                            X.counts.p, X.flags.p);
+                 scan_exclusive(D, s, X.counts.p, n_el, X.scan_tmp);
              });
-    HIPCHECK(hipMemcpyAsync(&h[0], T.counts.p + (n_el - 1), 4, hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(&h[1], T.flags.p, 4, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(&h[0], X.counts.p + (n_el - 1), 4, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(&h[1], X.flags.p, 4, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     if (h[1])
         return; // the camera plane cuts a primitive's box
@@ -1365,8 +1397,8 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
     T.end.ensure(n_tiles);
     if (M > 0)
         {
-        for (auto* b : {&T.tile0, &T.q0, &T.ref0, &T.keysA, &T.valsA, &T.keysB, &T.valsB})
-            b->ensure(M);
+        for (auto* b : {&X.tile0, &X.q0, &X.ref0, &X.keysA, &X.valsA, &X.keysB, &X.valsB})
+            grow(*b, M, s);
         T.entries.ensure(2 * (size_t)M);
         D.launch("tiles_emit",
                  [&]
@@ -1374,36 +1406,36 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
                      if (cut)
                          k_tile_cut<true><<<blocks(n_inner + 1, 256), 256, 0, s>>>(
                              sc.nodes.p, n_inner, sc.blo.p, sc.bhi.p, sc.prefix.p, sc.slots.p, sc.n_active, c,
-                             D.tile_group, T.counts.p, T.flags.p, T.tile0.p, T.q0.p, T.ref0.p, T.valsA.p);
+                             D.tile_group, X.counts.p, X.flags.p, X.tile0.p, X.q0.p, X.ref0.p, X.valsA.p);
                      else if (spheres)
                          k_tile_emit<true><<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
-                                                                          T.counts.p, T.tile0.p, T.q0.p, T.ref0.p,
-                                                                          T.valsA.p);
+                                                                          X.counts.p, X.tile0.p, X.q0.p, X.ref0.p,
+                                                                          X.valsA.p);
                      else
                          k_tile_emit<false><<<blocks(N, 256), 256, 0, s>>>(sc.blo.p, sc.bhi.p, sc.dgeoms.p, N, c,
-                                                                           T.counts.p, T.tile0.p, T.q0.p, T.ref0.p,
-                                                                           T.valsA.p);
-                     HIPCHECK(hipMemcpyAsync(T.keysA.p, T.q0.p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+                                                                           X.counts.p, X.tile0.p, X.q0.p, X.ref0.p,
+                                                                           X.valsA.p);
+                     HIPCHECK(hipMemcpyAsync(X.keysA.p, X.q0.p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
                  });
         // by bound (4 passes: the result is back in keysA/valsA), then stably by tile
-        radix_sort_pairs(D, s, "tiles_sort", T.keysA.p, T.valsA.p, T.keysB.p, T.valsB.p, M, 0, 32, T.hist,
-                         T.scan_tmp);
+        radix_sort_pairs(D, s, "tiles_sort", X.keysA.p, X.valsA.p, X.keysB.p, X.valsB.p, M, 0, 32, X.hist,
+                         X.scan_tmp);
         uint32_t bits = 8;
         while (bits < 32 && (n_tiles - 1) >> bits)
             bits += 8;
-        D.launch("tiles_sort", [&] { k_gather_u32<<<blocks(M, 256), 256, 0, s>>>(T.tile0.p, T.valsA.p, M, T.keysB.p); });
-        const bool flipped = radix_sort_pairs(D, s, "tiles_sort", T.keysB.p, T.valsA.p, T.keysA.p, T.valsB.p, M, 0,
-                                              bits, T.hist, T.scan_tmp);
-        const uint32_t* kt = flipped ? T.keysA.p : T.keysB.p;
-        const uint32_t* vt = flipped ? T.valsB.p : T.valsA.p;
+        D.launch("tiles_sort", [&] { k_gather_u32<<<blocks(M, 256), 256, 0, s>>>(X.tile0.p, X.valsA.p, M, X.keysB.p); });
+        const bool flipped = radix_sort_pairs(D, s, "tiles_sort", X.keysB.p, X.valsA.p, X.keysA.p, X.valsB.p, M, 0,
+                                              bits, X.hist, X.scan_tmp);
+        const uint32_t* kt = flipped ? X.keysA.p : X.keysB.p;
+        const uint32_t* vt = flipped ? X.valsB.p : X.valsA.p;
         D.launch("tiles_bounds",
                  [&]
                  {
                      if (spheres)
-                         k_tile_pack<true><<<blocks(M, 256), 256, 0, s>>>(vt, T.ref0.p, T.q0.p, sc.blo.p, sc.bhi.p,
+                         k_tile_pack<true><<<blocks(M, 256), 256, 0, s>>>(vt, X.ref0.p, X.q0.p, sc.blo.p, sc.bhi.p,
                                                                           sc.dgeoms.p, nullptr, M, T.entries.p);
                      else
-                         k_tile_pack<false><<<blocks(M, 256), 256, 0, s>>>(vt, T.ref0.p, T.q0.p, sc.blo.p, sc.bhi.p,
+                         k_tile_pack<false><<<blocks(M, 256), 256, 0, s>>>(vt, X.ref0.p, X.q0.p, sc.blo.p, sc.bhi.p,
                                                                            sc.dgeoms.p, sc.nodes.p, M, T.entries.p);
                      k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.start.p, n_tiles, 0u);
                      k_fill_u32<<<blocks(n_tiles, 256), 256, 0, s>>>(T.end.p, n_tiles, 0u);
@@ -2136,6 +2168,7 @@ struct ServerObj
     hipStream_t st[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> free_events;
     std::vector<hipEvent_t> all_events;
+    std::vector<std::unique_ptr<PinnedBuf>> stage_pool; // staging of finished batches, reused
     std::deque<ServerBatch> batches;
     // last accumulation enqueued per batch kind (spheres-only / mixed): a
     // request's batches are all of one kind, so only batches of one kind
@@ -2339,6 +2372,7 @@ static void server_finish_batch(ServerObj& S, size_t i)
     if (S.last_acc_batch[kd] == b.serial)
         S.last_acc[kd] = nullptr; // finished: later batches of its kind need not wait
     S.free_events.push_back(b.done);
+    S.stage_pool.push_back(std::move(b.stage)); // its copies have completed
     S.batches.erase(S.batches.begin() + (std::ptrdiff_t)i);
     if (q_flag)
         throw Error(FH_ERR_STATE, "internal error: a ray queue segment overflowed");
@@ -2475,7 +2509,13 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
                 presize_scratch(D, S.rs[k], S.target_paths);
                 S.presized[k] = true;
                 }
-            b.stage.reset(new PinnedBuf());
+            if (S.stage_pool.empty())
+                b.stage.reset(new PinnedBuf());
+            else
+                {
+                b.stage = std::move(S.stage_pool.back());
+                S.stage_pool.pop_back();
+                }
             b.stage->ensure(stage_bytes(b.jobs.size()));
             b.e.reset(new BatchExec(D, S.rs[k], S.st[k], b.jobs, b.types, b.stage->p));
             b.e->start();
