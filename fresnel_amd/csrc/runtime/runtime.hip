@@ -227,7 +227,8 @@ struct DeviceObj
     int hip_dev = 0;
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr; // second render stream: two batches in flight
-    hipStream_t copy_stream = nullptr; // asynchronous output readback
+    hipStream_t copy_stream = nullptr; // asynchronous output readback; scene uploads
+    hipEvent_t upload_ev = nullptr;    // end of the scene uploads queued on copy_stream
     hipStream_t lstream = nullptr; // stream of the launches being enqueued (timing events)
     int n_cu = 0;
     std::string name;
@@ -671,6 +672,8 @@ struct SceneObj
     DBuf<uint32_t> vals, parent, flags, prefix, slots; // vals: primitive order of the leaves (refit reads it)
     DBuf<float> root_box, extent;
     float cull_margin = 0.0f; // CULL_K * 3 * largest primitive half-extent
+    bool margin_pending = false; // cull_margin not yet read back from the last build (resolve_margin)
+    bool build_pending = false;  // a build or refit was queued on the device stream since its last sync
     DBuf<Node> nodes;
     DBuf<Node4> nodes4;
     bool sphere_leaves = false; // nodes4 leaf children hold spheres
@@ -895,6 +898,32 @@ static bool radix_sort_pairs(DeviceObj& D, hipStream_t s, const char* name, uint
 
 static uint32_t scene_types(const SceneObj& sc);
 
+// Scene data reaches the device by copies on the copy stream.  The device
+// stream waits for the copies queued so far by event before its build
+// kernels, and the host waits for them too (the host buffers they read may
+// change or go away after return), but not for earlier scenes' builds still
+// queued on the device stream.
+static void uploads_done(DeviceObj& D)
+    {
+    HIPCHECK(hipEventRecord(D.upload_ev, D.copy_stream));
+    HIPCHECK(hipStreamWaitEvent(D.stream, D.upload_ev, 0));
+    HIPCHECK(hipEventSynchronize(D.upload_ev));
+    }
+
+// The traversal's cull margin, CULL_K * 3 * the largest primitive half-extent,
+// read back from the last build once it is needed (jobs, tile lists).
+static void resolve_margin(SceneObj& sc)
+    {
+    if (!sc.margin_pending)
+        return;
+    DeviceObj& D = *sc.dev;
+    float r_max = 0.0f;
+    HIPCHECK(hipMemcpyAsync(&r_max, sc.extent.p, sizeof(float), hipMemcpyDeviceToHost, D.stream));
+    HIPCHECK(hipStreamSynchronize(D.stream));
+    sc.cull_margin = CULL_K * (3.0f * r_max);
+    sc.margin_pending = false;
+    }
+
 static void build_bvh(SceneObj& sc, bool refit_only)
     {
     DeviceObj& D = *sc.dev;
@@ -924,8 +953,9 @@ static void build_bvh(SceneObj& sc, bool refit_only)
     REQUIRE(N < (1u << 30), "too many primitives in one scene");
     sc.prefix.ensure(prefix.size());
     sc.slots.ensure(slots.size());
-    HIPCHECK(hipMemcpyAsync(sc.prefix.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemcpyAsync(sc.slots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(sc.prefix.p, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, D.copy_stream));
+    HIPCHECK(hipMemcpyAsync(sc.slots.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, D.copy_stream));
+    uploads_done(D);
     sc.blo.ensure(N);
     sc.bhi.ensure(N);
     const uint32_t na = (uint32_t)slots.size();
@@ -945,10 +975,7 @@ static void build_bvh(SceneObj& sc, bool refit_only)
                  k_extent_partial<<<n_part, 256, 0, s>>>(sc.blo.p, sc.bhi.p, N, sc.extent.p + 1);
                  k_extent_final<<<1, 256, 0, s>>>(sc.extent.p + 1, n_part, sc.extent.p);
              });
-    float r_max = 0.0f;
-    HIPCHECK(hipMemcpyAsync(&r_max, sc.extent.p, sizeof(float), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    sc.cull_margin = CULL_K * (3.0f * r_max);
+    sc.margin_pending = true;
     }
     sc.nodes.ensure(std::max<uint32_t>(N - 1, 1));
     sc.root_box.ensure(6);
@@ -1073,6 +1100,10 @@ static void update_bvh(SceneObj& sc)
 static void sync_scene(SceneObj& sc)
     {
     DeviceObj& D = *sc.dev;
+    // uploads below may rewrite buffers an earlier build of this scene still reads
+    if (sc.build_pending)
+        HIPCHECK(hipStreamSynchronize(D.stream));
+    sc.build_pending = false;
     bool any_dirty = false;
     std::vector<uint32_t> topo;
     for (auto& kv : sc.geoms)
@@ -1083,7 +1114,7 @@ static void sync_scene(SceneObj& sc)
         if (g.dirty || g.shape_hi > g.shape_lo || g.color_hi > g.color_lo)
             {
             // colour-only changes leave the hierarchy as it is
-            if (upload_geometry(g, D.stream) && g.enabled)
+            if (upload_geometry(g, D.copy_stream) && g.enabled)
                 any_dirty = true;
             }
         if (g.enabled)
@@ -1118,24 +1149,29 @@ static void sync_scene(SceneObj& sc)
         t.aux_color = g.daux_color.p;
         }
     sc.dgeoms.ensure(n_slots);
-    HIPCHECK(hipMemcpyAsync(sc.dgeoms.p, table.data(), n_slots * sizeof(GeomDev), hipMemcpyHostToDevice, D.stream));
+    HIPCHECK(hipMemcpyAsync(sc.dgeoms.p, table.data(), n_slots * sizeof(GeomDev), hipMemcpyHostToDevice,
+                            D.copy_stream));
+    uploads_done(D); // before the table goes away and before the device stream's kernels read it
     sc.n_slots = n_slots;
     const bool topo_same = sc.bvh_valid && topo == sc.topo;
     if (!sc.bvh_valid || !topo_same)
         {
         build_bvh(sc, false);
+        sc.build_pending = true;
         if (sc.allow_refit)
             sc.sah_built = sc.sah_last = bvh_sah(sc);
         }
     else if (any_dirty)
+        {
         update_bvh(sc);
+        sc.build_pending = true;
+        }
     sc.topo = topo;
-    // the table upload above must complete before the host vector goes away
-    HIPCHECK(hipStreamSynchronize(D.stream));
     }
 
-static SceneDev scene_dev(const SceneObj& sc)
+static SceneDev scene_dev(SceneObj& sc)
     {
+    resolve_margin(sc);
     SceneDev d;
     d.nodes = sc.nodes.p;
     d.nodes4 = sc.nodes4.p;
@@ -1350,6 +1386,7 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
     c.model = cam.model;
     c.tiles_x = tr.W / TILE;
     c.tiles_y = tr.H / TILE;
+    resolve_margin(sc);
     c.margin = sc.cull_margin;
     // orthographic ray origins lie on the image plane within this distance of p
     // (screen coordinates |xs| <= W / 2H + 2 / H, |ys| <= 1/2 + 2 / H)
@@ -2686,6 +2723,7 @@ extern "C"
                 HIPCHECK(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
                 HIPCHECK(hipStreamCreateWithFlags(&D.stream2, hipStreamNonBlocking));
                 HIPCHECK(hipStreamCreateWithFlags(&D.copy_stream, hipStreamNonBlocking));
+                HIPCHECK(hipEventCreateWithFlags(&D.upload_ev, hipEventDisableTiming));
                 for (auto& e : D.acc_done)
                     HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 // overflow stacks are sized for the largest resident traversal grid
@@ -2733,6 +2771,8 @@ extern "C"
                 for (auto e : dev->obj.acc_done)
                     if (e)
                         (void)hipEventDestroy(e);
+                if (dev->obj.upload_ev)
+                    (void)hipEventDestroy(dev->obj.upload_ev);
                 hipStream_t s = dev->obj.stream, s2 = dev->obj.stream2, s3 = dev->obj.copy_stream;
                 if (s3)
                     (void)hipStreamSynchronize(s3);
@@ -4230,6 +4270,7 @@ extern "C"
                 DeviceObj& D = *sc.dev;
                 std::lock_guard<std::mutex> lock(D.mu);
                 sync_scene(sc);
+                HIPCHECK(hipStreamSynchronize(D.stream)); // the hierarchy build has completed
                 const uint64_t n = sc.n_prims == 0 ? 0 : std::max<uint32_t>(sc.n_prims - 1, 1);
                 *n_nodes = n;
                 if (nodes && n)
