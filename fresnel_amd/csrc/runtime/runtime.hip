@@ -2463,9 +2463,12 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
         S.active.push_back(it.id);
         }
     // Scenes of admitted requests are built (uploads, hierarchy) a few per
-    // step: with batches running, for about 2 ms of host time, so the device
-    // keeps tracing earlier requests meanwhile; with none running, only until
-    // one request can be planned.
+    // step: with batches running, for about 5 ms of host time, so the device
+    // keeps tracing earlier requests meanwhile and the next batch holds the
+    // requests built meanwhile (96 small requests as a burst: 21-25 steps and
+    // 105-119 ms with 2 ms, 11-13 steps and 92-108 ms with 5 ms, 9-10 steps and
+    // 95-109 ms with 10 ms); with none running, only until one request can be
+    // planned.
     {
     bool running = false;
     for (auto& b : S.batches)
@@ -2487,7 +2490,7 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
             }
         r.built = true;
         const double spent = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build).count();
-        if (!running || spent > 2e-3)
+        if (!running || spent > 5e-3)
             break;
         }
     }
