@@ -119,10 +119,16 @@ template<class T> struct DBuf
 
 // Grow-only buffer used by the kernels of stream s: work still queued there
 // may read the old allocation, so the stream finishes before it is replaced.
+// A replacement holds an eighth more than asked, so sizes that creep up from
+// one build to the next (tile-list entries of a moving scene) do not
+// reallocate every time.
 template<class T> static void grow(DBuf<T>& b, size_t n, hipStream_t s)
     {
     if (b.p && b.cap < n)
+        {
         HIPCHECK(hipStreamSynchronize(s));
+        n += n / 8;
+        }
     b.ensure(n);
     }
 
@@ -1436,7 +1442,7 @@ static void update_tiles(DeviceObj& D, SceneObj& sc, TracerObj& tr, const CamDev
         {
         for (auto* b : {&X.tile0, &X.q0, &X.ref0, &X.keysA, &X.valsA, &X.keysB, &X.valsB})
             grow(*b, M, s);
-        T.entries.ensure(2 * (size_t)M);
+        grow(T.entries, 2 * (size_t)M, s);
         D.launch("tiles_emit",
                  [&]
                  {
