@@ -264,8 +264,10 @@ int fh_batch_render(fh_scene* scenes, fh_tracer* tracers, uint32_t n, uint32_t s
  * fh_policy_scratch_bytes), retires started batches that have completed,
  * plans a step (every active request takes some camera samples) when none is
  * waiting, starts planned batches on the server's two render scratches and
- * streams (a started batch is enqueued whole; the device finds each bounce's
- * rays itself), and retires requests whose samples have all been
+ * the device's two render streams (a started batch is enqueued whole; the
+ * device finds each bounce's rays itself; a Path render on the same device
+ * between steps waits for the batches in flight), and retires requests whose
+ * samples have all been
  * accumulated.  When nothing can start it waits for the oldest started batch.
  * Work stays in flight between calls; fh_server_destroy finishes the batches
  * already started.
