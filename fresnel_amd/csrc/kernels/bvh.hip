@@ -14,8 +14,6 @@
 namespace fh
 {
 constexpr int SORT_BLOCK = 256;
-constexpr int SORT_ITEMS = 16;
-constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 
 // ---------------------------------------------------------------- bounds
 __global__ void k_prim_bounds(const GeomDev* __restrict__ geoms, const uint32_t* __restrict__ prefix,
@@ -131,14 +129,17 @@ __global__ void k_morton(const float4* __restrict__ lo, const float4* __restrict
     }
 
 // ---------------------------------------------------------------- radix sort
+// ITEMS keys per thread: a workgroup sorts SORT_BLOCK * ITEMS keys of a pass
+// (its ranks take ITEMS dependent rounds).
+template<int ITEMS>
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t N,
                                                            uint32_t shift, uint32_t n_blocks, uint32_t* hist)
     {
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * SORT_TILE;
-    for (int r = 0; r < SORT_ITEMS; r++)
+    const uint32_t base = blockIdx.x * (SORT_BLOCK * ITEMS);
+    for (int r = 0; r < ITEMS; r++)
         {
         const uint32_t i = base + r * SORT_BLOCK + threadIdx.x;
         if (i < N)
@@ -244,6 +245,7 @@ __global__ void k_scan_add(uint32_t* data, uint32_t n, const uint32_t* sums)
         data[i] += sums[i / 2048];
     }
 
+template<int ITEMS>
 __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                               const uint32_t* __restrict__ vin, uint32_t* kout,
                                                               uint32_t* vout, uint32_t N, uint32_t shift,
@@ -259,8 +261,8 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_radix_scatter(const uint32_t* __
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const uint32_t base = blockIdx.x * SORT_TILE;
-    for (int r = 0; r < SORT_ITEMS; r++)
+    const uint32_t base = blockIdx.x * (SORT_BLOCK * ITEMS);
+    for (int r = 0; r < ITEMS; r++)
         {
         const uint32_t i = base + r * SORT_BLOCK + threadIdx.x;
         const bool valid = i < N;

@@ -884,7 +884,11 @@ static bool radix_sort_pairs(DeviceObj& D, hipStream_t s, const char* name, uint
                              uint32_t* v1, uint32_t N, uint32_t bit0, uint32_t bit1, DBuf<uint32_t>& hist,
                              std::deque<DBuf<uint32_t>>& tmp)
     {
-    const uint32_t nb = blocks(N, SORT_TILE);
+    // 16 keys per thread while that gives every CU two workgroups, else 4: a
+    // workgroup ranks its keys in that many dependent rounds, so a smaller sort
+    // spread over more workgroups finishes sooner
+    const bool wide = blocks(N, SORT_BLOCK * 16) >= 2u * (uint32_t)std::max(1, D.n_cu);
+    const uint32_t nb = blocks(N, SORT_BLOCK * (wide ? 16 : 4));
     const uint32_t nh = 256 * nb;
     grow(hist, nh, s);
     uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
@@ -894,9 +898,15 @@ static bool radix_sort_pairs(DeviceObj& D, hipStream_t s, const char* name, uint
         D.launch(name,
                  [&]
                  {
-                     k_radix_hist<<<nb, SORT_BLOCK, 0, s>>>(kin, N, shift, nb, hist.p);
+                     if (wide)
+                         k_radix_hist<16><<<nb, SORT_BLOCK, 0, s>>>(kin, N, shift, nb, hist.p);
+                     else
+                         k_radix_hist<4><<<nb, SORT_BLOCK, 0, s>>>(kin, N, shift, nb, hist.p);
                      scan_exclusive(D, s, hist.p, nh, tmp);
-                     k_radix_scatter<<<nb, SORT_BLOCK, 0, s>>>(kin, vin, kout, vout, N, shift, nb, hist.p);
+                     if (wide)
+                         k_radix_scatter<16><<<nb, SORT_BLOCK, 0, s>>>(kin, vin, kout, vout, N, shift, nb, hist.p);
+                     else
+                         k_radix_scatter<4><<<nb, SORT_BLOCK, 0, s>>>(kin, vin, kout, vout, N, shift, nb, hist.p);
                  });
         std::swap(kin, kout);
         std::swap(vin, vout);
