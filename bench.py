@@ -47,7 +47,10 @@ Other workloads (--workload), each printing one JSON line of the same form:
   c5  served stream: Poisson arrivals of sphere/cylinder scenes with
       log-uniform sizes, mixed resolutions and sample counts: one request at a
       time (plain) vs the render server; Mpaths/s and p50/p95/p99 latency per
-      arrival rate; images compared bit for bit.
+      arrival rate; images compared bit for bit.  --small takes the small-request
+      stream instead (10^3-10^5 primitives, 256^2 or 512^2, 8 samples; 32 Mi-path
+      steps); --modes plain or --modes served runs one mode alone (for a kernel
+      trace of it), without the comparison.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5] [--no-legs]
 """
@@ -665,10 +668,11 @@ def small_requests(n_req, seed=7):
     return reqs
 
 
-def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
+def served_leg(specs, rates, target_paths, describe, budget=240 << 30, modes=('plain', 'served')):
     """Poisson streams of `specs` at each rate (req/s; 0 = a burst: every request at time 0): one request at a
     time as it arrives (plain, Path.sample) vs the render server.  Returns the summary with Mpaths/s and
-    p50/p95/p99 latency (arrival to complete image) per rate, images compared bit for bit."""
+    p50/p95/p99 latency (arrival to complete image) per rate, images compared bit for bit.  `modes` may
+    name one of the two alone (a kernel trace of one mode); the comparison then is left out."""
     import fresnel_amd as fresnel
     from fresnel_amd import serve
     dev = fresnel.Device(mode='gpu', n=1)
@@ -735,6 +739,8 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
         row = {"lambda_per_s": lam if lam > 0 else "burst"}
         digests = {}
         for name, fn in (('plain', run_plain_stream), ('served', run_served_stream)):
+            if name not in modes:
+                continue
             # the request loop's latencies should not include Python's collector:
             # collect before the stream, none during it (both modes alike)
             gc.collect()
@@ -756,14 +762,17 @@ def served_leg(specs, rates, target_paths, describe, budget=240 << 30):
                                         "max": round(float(dur[k]) * 1e3, 2), "max_at_s": round(steps[k][0], 3)}
                 row[name]["max_active_requests"] = int(max(a for _, _, a in steps))
             log(f"served leg lambda={lam} {name}: {row[name]}")
-        same = digests['served'] == digests['plain']
-        identical = identical and same
-        row["served_vs_plain"] = round(row['served']['mpaths_per_s'] / row['plain']['mpaths_per_s'], 3)
-        row["bit_identical"] = same
+        if len(digests) == 2:
+            same = digests['served'] == digests['plain']
+            identical = identical and same
+            row["served_vs_plain"] = round(row['served']['mpaths_per_s'] / row['plain']['mpaths_per_s'], 3)
+            row["bit_identical"] = same
         table.append(row)
+    both = all("served_vs_plain" in r for r in table)
     return {"workload": describe, "requests": len(specs), "total_paths": total_paths,
             "target_paths_per_step": target_paths, "hbm_budget_bytes": budget, "by_rate": table,
-            "best_served_vs_plain": max(r["served_vs_plain"] for r in table), "bit_identical": identical}
+            "best_served_vs_plain": max(r["served_vs_plain"] for r in table) if both else None,
+            "bit_identical": identical if both else None}
 
 
 C5_DESCRIPTION = ("spheres (FCC generator) or spherocylinders (nearest-neighbour generator), N log-uniform in "
@@ -776,18 +785,25 @@ SMALL_DESCRIPTION = ("small requests: spheres or spherocylinders, N log-uniform 
 def run_served(args):
     """Poisson request streams: plain (one request at a time) vs the render server."""
     rates = [float(x) for x in args.rates.split(',')]
-    summary = served_leg(make_requests(args.requests), rates, args.target_paths,
-                         f"{args.requests} requests: " + C5_DESCRIPTION)
-    top = [r for r in summary['by_rate'] if r['lambda_per_s'] != 'burst'][-1]
-    out = base_record(top['served']['mpaths_per_s'], "Mpaths/s", len(rates), 0,
-                      round(top['served']['makespan_s'] * 1e3, 2), {
-                          "workload": summary['workload'], "mode": "served", "total_paths": summary['total_paths'],
-                          "target_paths_per_step": args.target_paths})
-    out["latency_s"] = top['served']['latency_s']
+    modes = tuple(args.modes.split(','))
+    if args.small:
+        summary = served_leg(small_requests(args.requests), rates, args.target_paths or 32 << 20,
+                             f"{args.requests} requests: " + SMALL_DESCRIPTION, modes=modes)
+    else:
+        summary = served_leg(make_requests(args.requests), rates, args.target_paths or 256 << 20,
+                             f"{args.requests} requests: " + C5_DESCRIPTION, modes=modes)
+    rows = [r for r in summary['by_rate'] if r['lambda_per_s'] != 'burst'] or summary['by_rate']
+    top = rows[-1]
+    mode = 'served' if 'served' in modes else 'plain'
+    out = base_record(top[mode]['mpaths_per_s'], "Mpaths/s", len(rates), 0,
+                      round(top[mode]['makespan_s'] * 1e3, 2), {
+                          "workload": summary['workload'], "mode": mode, "total_paths": summary['total_paths'],
+                          "target_paths_per_step": summary['target_paths_per_step']})
+    out["latency_s"] = top[mode]['latency_s']
     out["by_rate"] = summary['by_rate']
     out["bit_identical"] = summary['bit_identical']
     print(json.dumps(out), flush=True)
-    if not summary['bit_identical']:
+    if summary['bit_identical'] is False:
         sys.exit(1)
 
 
@@ -806,7 +822,11 @@ def main():
     ap.add_argument('--frames', type=int, default=64, help='c4: trajectory frames')
     ap.add_argument('--requests', type=int, default=24, help='c5: requests per stream')
     ap.add_argument('--rates', default='5,10,20,50,100,200,0', help='c5: arrival rates (req/s; 0 = burst)')
-    ap.add_argument('--target-paths', type=int, default=256 << 20, help='c5: paths per server step')
+    ap.add_argument('--target-paths', type=int, default=0,
+                    help='c5: paths per server step (default 256 Mi; 32 Mi with --small)')
+    ap.add_argument('--small', action='store_true',
+                    help='c5: the small-request stream (10^3-10^5 primitives, 256^2 or 512^2, 8 samples)')
+    ap.add_argument('--modes', default='plain,served', help='c5: plain, served or both (comma-separated)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
