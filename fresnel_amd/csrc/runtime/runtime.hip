@@ -236,6 +236,7 @@ struct DeviceObj
     hipStream_t stream3 = nullptr; // first render stream (`stream` builds scenes and tile lists)
     hipStream_t copy_stream = nullptr; // asynchronous output readback; scene uploads
     hipEvent_t upload_ev = nullptr;    // end of the scene uploads queued on copy_stream
+    hipEvent_t build_ev = nullptr;     // device-stream work a render batch waits for (InflightRun)
     hipStream_t lstream = nullptr; // stream of the launches being enqueued (timing events)
     int n_cu = 0;
     std::string name;
@@ -2019,43 +2020,50 @@ static void update_live_frac(DeviceObj& D, const float* live)
         D.live_frac[b] = std::max(live[b], 0.75f * D.live_frac[b]);
     }
 
-// Run batches (job lists with their geometry-type sets), alternating between
+// Batches (job lists with their geometry-type sets) run alternating between
 // the device's two render scratches and streams when D.inflight is 2: each
 // batch is enqueued whole (primary rays, D.steps_est bounce steps,
 // accumulation) without waiting for the device, so one batch's kernels
 // overlap the other's.  Accumulations are chained in batch order (each waits
 // for the previous one's event), so every pixel's samples are accumulated in
-// order.  Returns with both streams idle.
-static void run_inflight(DeviceObj& D, std::deque<std::pair<std::vector<JobDev>, uint32_t>>& pending)
+// order.  A batch's render stream waits, by event, for the work queued on the
+// device stream before it (scene builds, tile lists, output clears), so the
+// device stream can go on building the scenes of later batches meanwhile.
+// finish() returns with both render streams idle.
+struct InflightRun
     {
-    HIPCHECK(hipStreamSynchronize(D.stream)); // scene, tile lists and counters ready for both streams
-    HIPCHECK(hipStreamSynchronize(D.stream2));
-    HIPCHECK(hipStreamSynchronize(D.stream3));
-    // both streams are idle: the least recently used graphs beyond 32 can go
-    while (D.graph_cache.size() > 32)
-        {
-        auto lru = std::min_element(D.graph_cache.begin(), D.graph_cache.end(),
-                                    [](const DeviceObj::GraphEntry& a, const DeviceObj::GraphEntry& b)
-                                    { return a.used < b.used; });
-        (void)hipGraphExecDestroy(lru->exec);
-        D.graph_cache.erase(lru);
-        }
-    size_t bytes = 0;
-    for (const auto& b : pending)
-        bytes += stage_bytes(b.first.size());
-    D.stage.ensure(bytes);
+    DeviceObj& D;
     std::vector<std::unique_ptr<BatchExec>> batches;
     hipEvent_t last_acc = nullptr;
     int slot = 0;
     size_t off = 0;
-    const uint32_t steps = D.steps_est;
-    while (!pending.empty())
+    uint32_t steps = 0;
+
+    // `stage` bytes of pinned staging for the job tables of every batch to come
+    InflightRun(DeviceObj& d, size_t stage) : D(d)
         {
-        const size_t n_jobs = pending.front().first.size();
-        batches.emplace_back(new BatchExec(D, slot ? D.rs2 : D.rs, slot ? D.stream2 : D.stream3,
-                                           std::move(pending.front().first), pending.front().second,
-                                           D.stage.p + off));
-        pending.pop_front();
+        HIPCHECK(hipStreamSynchronize(D.stream2));
+        HIPCHECK(hipStreamSynchronize(D.stream3));
+        // both streams are idle: the least recently used graphs beyond 32 can go
+        while (D.graph_cache.size() > 32)
+            {
+            auto lru = std::min_element(D.graph_cache.begin(), D.graph_cache.end(),
+                                        [](const DeviceObj::GraphEntry& a, const DeviceObj::GraphEntry& b)
+                                        { return a.used < b.used; });
+            (void)hipGraphExecDestroy(lru->exec);
+            D.graph_cache.erase(lru);
+            }
+        D.stage.ensure(stage);
+        steps = D.steps_est;
+        }
+
+    void add(std::vector<JobDev> jobs, uint32_t types)
+        {
+        const size_t n_jobs = jobs.size();
+        hipStream_t s = slot ? D.stream2 : D.stream3;
+        HIPCHECK(hipEventRecord(D.build_ev, D.stream));
+        HIPCHECK(hipStreamWaitEvent(s, D.build_ev, 0));
+        batches.emplace_back(new BatchExec(D, slot ? D.rs2 : D.rs, s, std::move(jobs), types, D.stage.p + off));
         off += stage_bytes(n_jobs);
         BatchExec& e = *batches.back();
         e.start();
@@ -2065,17 +2073,35 @@ static void run_inflight(DeviceObj& D, std::deque<std::pair<std::vector<JobDev>,
         if (D.inflight == 2)
             slot = 1 - slot;
         }
-    HIPCHECK(hipStreamSynchronize(D.stream3));
-    HIPCHECK(hipStreamSynchronize(D.stream2));
-    uint32_t need = 1;
-    float live[MAX_STEPS] = {};
-    for (const auto& e : batches)
+
+    void finish()
         {
-        need = std::max(need, e->steps_needed());
-        e->note_live(live);
+        HIPCHECK(hipStreamSynchronize(D.stream3));
+        HIPCHECK(hipStreamSynchronize(D.stream2));
+        uint32_t need = 1;
+        float live[MAX_STEPS] = {};
+        for (const auto& e : batches)
+            {
+            need = std::max(need, e->steps_needed());
+            e->note_live(live);
+            }
+        update_steps_est(D, need);
+        update_live_frac(D, live);
         }
-    update_steps_est(D, need);
-    update_live_frac(D, live);
+    };
+
+static void run_inflight(DeviceObj& D, std::deque<std::pair<std::vector<JobDev>, uint32_t>>& pending)
+    {
+    size_t bytes = 0;
+    for (const auto& b : pending)
+        bytes += stage_bytes(b.first.size());
+    InflightRun run(D, bytes);
+    while (!pending.empty())
+        {
+        run.add(std::move(pending.front().first), pending.front().second);
+        pending.pop_front();
+        }
+    run.finish();
     }
 
 // Render `samples` Path passes for each (scene, tracer) pair.
@@ -2089,19 +2115,6 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
         REQUIRE(scenes[k]->dev == tracers[k]->dev, "Scene and Tracer devices do not match");
         REQUIRE(tracers[k]->type == T_PATH, "not a Path tracer");
         }
-    // a scene appearing twice is synchronized once
-    std::vector<SceneObj*> uniq;
-    for (auto* sc : scenes)
-        if (std::find(uniq.begin(), uniq.end(), sc) == uniq.end())
-            uniq.push_back(sc);
-    for (auto* sc : uniq)
-        sync_scene(*sc);
-    for (size_t k = 0; k < scenes.size(); k++)
-        update_tiles(D, *scenes[k], *tracers[k], make_cam(scenes[k]->camera),
-                     (uint64_t)tracers[k]->W * tracers[k]->H * samples);
-    // a readback of an output still in flight finishes before the accumulation
-    for (auto* tr : tracers)
-        tr->order_after_copies(D.stream3, D.stream2);
     uint64_t per_sample = 0;
     for (auto* tr : tracers)
         per_sample += (uint64_t)tr->W * tr->H * tr->L;
@@ -2109,22 +2122,104 @@ static void render_path(DeviceObj& D, const std::vector<SceneObj*>& scenes, cons
     std::vector<uint32_t> types;
     for (auto* sc : scenes)
         types.push_back(scene_types(*sc));
-    std::deque<std::pair<std::vector<JobDev>, uint32_t>> groups;
-    for (uint32_t done = 0; done < samples;)
+    // Several pairs that take more than one batch: batches of whole pairs (all
+    // samples of each, pairs in order), so the first batch starts once its own
+    // scenes and tile lists are built, and the device stream builds the later
+    // pairs' while earlier batches render.  Pairs whose samples do not fit one
+    // batch take the samples-first split below.
+    std::vector<std::pair<size_t, size_t>> chunks; // [first, last) pairs per batch
+    bool by_pairs = scenes.size() > 1 && K < samples;
+    if (by_pairs)
         {
-        const uint32_t k = std::min(K, samples - done);
-        std::vector<JobDev> jobs;
-        for (size_t j = 0; j < scenes.size(); j++)
+        size_t first = 0;
+        uint64_t paths = 0;
+        for (size_t j = 0; j < scenes.size() && by_pairs; j++)
             {
-            JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
-            J.K = k;
-            J.n_base = tracers[j]->n_samples + done;
-            jobs.push_back(J);
+            const uint64_t pj = (uint64_t)tracers[j]->W * tracers[j]->H * tracers[j]->L * samples;
+            by_pairs = pj <= D.batch_paths;
+            if (paths + pj > D.batch_paths)
+                {
+                chunks.push_back({first, j});
+                first = j;
+                paths = 0;
+                }
+            paths += pj;
             }
-        split_groups(jobs, types, groups);
-        done += k;
+        chunks.push_back({first, scenes.size()});
         }
-    run_inflight(D, groups);
+    if (by_pairs)
+        {
+        size_t bytes = 0;
+        for (const auto& c : chunks)
+            {
+            size_t n_sph = 0, n_mixed = 0;
+            for (size_t j = c.first; j < c.second; j++)
+                ((types[j] & ~T_SPHERES) == 0 ? n_sph : n_mixed)++;
+            bytes += (n_sph ? stage_bytes(n_sph) : 0) + (n_mixed ? stage_bytes(n_mixed) : 0);
+            }
+        InflightRun run(D, bytes);
+        std::vector<SceneObj*> synced; // a scene appearing twice is synchronized once
+        for (const auto& c : chunks)
+            {
+            for (size_t j = c.first; j < c.second; j++)
+                if (std::find(synced.begin(), synced.end(), scenes[j]) == synced.end())
+                    {
+                    sync_scene(*scenes[j]);
+                    synced.push_back(scenes[j]);
+                    }
+            std::vector<JobDev> jobs;
+            std::vector<uint32_t> ty;
+            for (size_t j = c.first; j < c.second; j++)
+                {
+                update_tiles(D, *scenes[j], *tracers[j], make_cam(scenes[j]->camera),
+                             (uint64_t)tracers[j]->W * tracers[j]->H * samples);
+                // a readback of an output still in flight finishes before the accumulation
+                tracers[j]->order_after_copies(D.stream3, D.stream2);
+                JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
+                J.K = samples;
+                J.n_base = tracers[j]->n_samples;
+                jobs.push_back(J);
+                ty.push_back(types[j]);
+                }
+            std::deque<std::pair<std::vector<JobDev>, uint32_t>> groups;
+            split_groups(jobs, ty, groups);
+            for (auto& g : groups)
+                run.add(std::move(g.first), g.second);
+            }
+        run.finish();
+        }
+    else
+        {
+        // a scene appearing twice is synchronized once
+        std::vector<SceneObj*> uniq;
+        for (auto* sc : scenes)
+            if (std::find(uniq.begin(), uniq.end(), sc) == uniq.end())
+                uniq.push_back(sc);
+        for (auto* sc : uniq)
+            sync_scene(*sc);
+        for (size_t k = 0; k < scenes.size(); k++)
+            update_tiles(D, *scenes[k], *tracers[k], make_cam(scenes[k]->camera),
+                         (uint64_t)tracers[k]->W * tracers[k]->H * samples);
+        // a readback of an output still in flight finishes before the accumulation
+        for (auto* tr : tracers)
+            tr->order_after_copies(D.stream3, D.stream2);
+        std::deque<std::pair<std::vector<JobDev>, uint32_t>> groups;
+        for (uint32_t done = 0; done < samples;)
+            {
+            const uint32_t k = std::min(K, samples - done);
+            std::vector<JobDev> jobs;
+            for (size_t j = 0; j < scenes.size(); j++)
+                {
+                JobDev J = make_job(*scenes[j], *tracers[j], scenes[j]->camera);
+                J.K = k;
+                J.n_base = tracers[j]->n_samples + done;
+                jobs.push_back(J);
+                }
+            split_groups(jobs, types, groups);
+            done += k;
+            }
+        run_inflight(D, groups);
+        }
     for (auto* tr : tracers)
         tr->n_samples += samples;
     D.end_stats();
@@ -2752,6 +2847,7 @@ extern "C"
                 HIPCHECK(hipStreamCreateWithFlags(&D.copy_stream, hipStreamNonBlocking));
                 HIPCHECK(hipStreamCreateWithFlags(&D.stream3, hipStreamNonBlocking));
                 HIPCHECK(hipEventCreateWithFlags(&D.upload_ev, hipEventDisableTiming));
+                HIPCHECK(hipEventCreateWithFlags(&D.build_ev, hipEventDisableTiming));
                 for (auto& e : D.acc_done)
                     HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
                 // overflow stacks are sized for the largest resident traversal grid
@@ -2801,6 +2897,8 @@ extern "C"
                         (void)hipEventDestroy(e);
                 if (dev->obj.upload_ev)
                     (void)hipEventDestroy(dev->obj.upload_ev);
+                if (dev->obj.build_ev)
+                    (void)hipEventDestroy(dev->obj.build_ev);
                 hipStream_t s = dev->obj.stream, s2 = dev->obj.stream2, s3 = dev->obj.copy_stream,
                             s4 = dev->obj.stream3;
                 if (s3)

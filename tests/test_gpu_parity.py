@@ -236,6 +236,33 @@ def test_batches_in_flight_equal_one_batch(gpu_device):
         assert numpy.array_equal(one_alone[k], one_batch[k]), f"scene {k}: batched differs from alone"
 
 
+def test_batches_of_whole_pairs_equal_alone(gpu_device):
+    """Pairs that take several batches go in batches of whole pairs, each batch starting once its own scenes
+    are built while the device stream builds the next pairs' scenes: every image equals its render alone,
+    bit for bit (spheres-only and mixed scenes, uneven sizes, one scene shared by two tracers)."""
+    d = gpu_device._device
+    saved = d.tune('batch_paths', 0)
+    shared = c1_scene(gpu_device, n=2500, w_seed=11)
+    builders = [lambda dv: c1_scene(dv, n=1500, w_seed=9), _spheres_and_cylinders, lambda dv: shared,
+                scenes.eight_polyhedra, lambda dv: shared, lambda dv: c1_scene(dv, n=4000, w_seed=10)]
+    sizes = [(64, 48), (40, 56), (48, 48), (56, 40), (64, 32), (72, 40)]
+    S = 6
+    try:
+        d.tune('batch_paths', 40000)  # about two pairs' 6 samples per batch: three or four batches
+        sc = [b(gpu_device) for b in builders]
+        trs = [fresnel.tracer.Path(gpu_device, w=w, h=h) for (w, h) in sizes]
+        for t in trs:
+            t.reset()
+            t._tracer.setLightSamples(1)
+        fresnel._gpu.batch_render([x._scene for x in sc], [t._tracer for t in trs], S)
+        batched = [numpy.array(t.linear_output[:]) for t in trs]
+    finally:
+        d.tune('batch_paths', saved)
+    for k, (b, (w, h)) in enumerate(zip(builders, sizes)):
+        alone = _linear(gpu_device, b, w, h, S, 1)[0]
+        assert numpy.array_equal(alone, batched[k]), f"pair {k} differs from its render alone"
+
+
 def test_graph_replays_equal_eager(gpu_device):
     """Batches captured into HIP graphs and replayed (tuning key 'graph' = 2) give the eager images bit for
     bit on every replay, with eager renders and synchronizations in between; several batches per render on
