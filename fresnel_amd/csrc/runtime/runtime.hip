@@ -2650,11 +2650,14 @@ static void server_step(ServerObj& S, double now, uint32_t& n_active, uint32_t& 
             break;
         if (!synced)
             {
-            HIPCHECK(hipStreamSynchronize(D.stream));
+            // the batches wait, by event, for the scene builds and tile lists
+            // queued on the device stream; the host does not
+            HIPCHECK(hipEventRecord(D.build_ev, D.stream));
             synced = true;
             }
         try
             {
+            HIPCHECK(hipStreamWaitEvent(S.st[k], D.build_ev, 0));
             if (S.free_events.empty())
                 {
                 hipEvent_t ev;
