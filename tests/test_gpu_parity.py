@@ -400,6 +400,32 @@ def test_served_requests_equal_plain_sample(gpu_device):
     assert srv.memory()[0] == 0
 
 
+def test_path_sample_between_server_steps(gpu_device):
+    """Path.sample on the same device between server steps (it renders on the streams the server's batches
+    occupy) leaves both its own image and the requests' images equal to their renders alone."""
+    from fresnel_amd import serve
+    builders = [lambda d: c1_scene(d, n=2000, w_seed=12), scenes.four_cylinders, lambda d: c1_scene(d, n=600, w_seed=13)]
+    specs = [((64, 48), 9, 1), ((80, 40), 5, 2), ((56, 56), 6, 1)]
+    plain = [_linear(gpu_device, b, w, h, S, L)[0] for b, ((w, h), S, L) in zip(builders, specs)]
+    extra = lambda d: c1_scene(d, n=1500, w_seed=14)
+    extra_alone = _linear(gpu_device, extra, 72, 40, 4, 1)[0]
+    sc = [b(gpu_device) for b in builders]
+    trs = [fresnel.tracer.Path(gpu_device, w=w, h=h) for ((w, h), _, _) in specs]
+    srv = serve.Server(gpu_device, hbm_budget=1 << 34, target_paths=6000)
+    reqs = [srv.submit(s, t, S, L) for s, t, (_, S, L) in zip(sc, trs, specs)]
+    srv.step()
+    srv.step()  # batches in flight on the device's render streams
+    between = _linear(gpu_device, extra, 72, 40, 4, 1)[0]
+    steps = 2
+    while not all(r.done for r in reqs):
+        srv.step()
+        steps += 1
+        assert steps < 100
+    assert numpy.array_equal(between, extra_alone), "Path.sample between server steps differs"
+    for k, (r, t) in enumerate(zip(reqs, trs)):
+        assert numpy.array_equal(numpy.array(t.linear_output[:]), plain[k]), f"request {k} differs"
+
+
 def test_server_rejects_over_budget(gpu_device):
     from fresnel_amd import serve
     s = c1_scene(gpu_device, n=100)
