@@ -43,7 +43,8 @@ Other workloads (--workload), each printing one JSON line of the same form:
       samples 128 (plain).
   c4  trajectory: 64 frames of 262,144 Brownian spheres, 1280x720, samples 32:
       64 plain renders (rebuild, then refit) vs one batched submission of all
-      frames; images compared bit for bit.
+      frames; images compared bit for bit (--modes plain_rebuild, plain_refit or
+      batched runs fewer of the three, for a kernel trace of one).
   c5  served stream: Poisson arrivals of sphere/cylinder scenes with
       log-uniform sizes, mixed resolutions and sample counts: one request at a
       time (plain) vs the render server; Mpaths/s and p50/p95/p99 latency per
@@ -536,9 +537,10 @@ def trajectory_scene(device, pos, camera=None):
     return scene, g
 
 
-def trajectory_leg(frames_n):
+def trajectory_leg(frames_n, modes=('plain_rebuild', 'plain_refit', 'batched')):
     """`frames_n` frames of the trajectory: plain renders (rebuild / refit) vs one batched submission of every
-    frame, images compared bit for bit.  Returns (summary, seconds per mode)."""
+    frame, images compared bit for bit.  Returns (summary, seconds per mode).  `modes` may name fewer of the
+    three (a kernel trace of one); ratios and the comparison need plain_rebuild and batched."""
     import fresnel_amd as fresnel
     W, H, S, F = 1280, 720, 32, frames_n
     dev = fresnel.Device(mode='gpu', n=1)
@@ -580,19 +582,23 @@ def trajectory_leg(frames_n):
     del s2, t2
     for name, fn in (('plain_rebuild', lambda: plain(False)), ('plain_refit', lambda: plain(True)),
                      ('batched', batched)):
+        if name not in modes:
+            continue
         dt, dig = fn()
         results[name] = (dt, dig)
         log(f"c4 {name}: {dt:.3f} s, {paths / dt / 1e6:.1f} Mpaths/s")
-    ref = results['plain_rebuild'][1]
-    identical = all(v[1] == ref for v in results.values())
+    ref = next(iter(results.values()))[1]
+    identical = all(v[1] == ref for v in results.values()) if len(results) > 1 else None
+    both = 'plain_rebuild' in results and 'batched' in results
     summary = {
         "workload": f"{F} frames of 262,144 spheres (r=0.5, uniform in a cube of side 80 at frame 0, Brownian "
                     f"steps sigma 0.02 per axis per frame, seed 4), 1280x720, samples=32, light_samples=1",
         "paths": paths,
         "modes": {k: {"seconds": round(v[0], 4), "mpaths_per_s": round(paths / v[0] / 1e6, 2)}
                   for k, v in results.items()},
-        "batched_vs_plain": round(results['plain_rebuild'][0] / results['batched'][0], 3),
-        "batched_vs_plain_refit": round(results['plain_refit'][0] / results['batched'][0], 3),
+        "batched_vs_plain": round(results['plain_rebuild'][0] / results['batched'][0], 3) if both else None,
+        "batched_vs_plain_refit": (round(results['plain_refit'][0] / results['batched'][0], 3)
+                                   if 'plain_refit' in results and 'batched' in results else None),
         "bit_identical": identical,
     }
     return summary, {k: v[0] for k, v in results.items()}
@@ -600,13 +606,17 @@ def trajectory_leg(frames_n):
 
 def run_trajectory(args):
     """64 frames: 64 plain renders (rebuild / refit) vs one batched submission of every frame."""
-    summary, secs = trajectory_leg(args.frames)
-    dt_b = secs['batched']
+    modes = tuple(m for m in args.modes.split(',') if m in ('plain_rebuild', 'plain_refit', 'batched')) or \
+        ('plain_rebuild', 'plain_refit', 'batched')
+    summary, secs = trajectory_leg(args.frames, modes)
+    mode = 'batched' if 'batched' in secs else next(iter(secs))
+    dt_b = secs[mode]
     out = base_record(round(summary['paths'] / dt_b / 1e6, 3), "Mpaths/s", 1, 1, round(dt_b * 1e3, 2), {
-        "workload": summary['workload'], "mode": "batched (all frames in one submission)", "paths": summary['paths']})
+        "workload": summary['workload'],
+        "mode": "batched (all frames in one submission)" if mode == 'batched' else mode, "paths": summary['paths']})
     out.update({k: v for k, v in summary.items() if k not in ('workload', 'paths')})
     print(json.dumps(out), flush=True)
-    if not summary['bit_identical']:
+    if summary['bit_identical'] is False:
         sys.exit(1)
 
 
@@ -826,7 +836,8 @@ def main():
                     help='c5: paths per server step (default 256 Mi; 32 Mi with --small)')
     ap.add_argument('--small', action='store_true',
                     help='c5: the small-request stream (10^3-10^5 primitives, 256^2 or 512^2, 8 samples)')
-    ap.add_argument('--modes', default='plain,served', help='c5: plain, served or both (comma-separated)')
+    ap.add_argument('--modes', default='plain,served',
+                    help='c5: plain, served or both; c4: any of plain_rebuild, plain_refit, batched (comma-separated)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
